@@ -1,0 +1,157 @@
+/*
+ * srgpu.h -- C ABI of the MI355X (gfx950) sort-middle rasteriser.
+ *
+ * This is the drop-in boundary for SoftRast's frame / draw-call API
+ * (SoftRast/Renderer.h:110-177). Every entry point takes plain pointers and
+ * sizes and returns an int status (0 = SRG_OK, negative = error); nothing
+ * asserts. The C++ facade in include/sr/Renderer.h re-creates the
+ * reference's RenderContext / FrameBuffer / DrawCall types on top of it.
+ *
+ * Threading: a context is driven from one host thread, as the reference's
+ * RenderContext is (TaskSystem.cpp:25). Contexts are independent.
+ */
+#ifndef SRGPU_H
+#define SRGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SRG_API_VERSION 1u
+
+typedef struct srg_ctx srg_ctx;
+typedef struct srg_fb srg_fb;
+typedef uint64_t srg_handle; /* 0 is never a valid handle */
+
+enum srg_status {
+    SRG_OK = 0,
+    SRG_ERR_INVALID = -1,        /* bad argument or draw-call state */
+    SRG_ERR_NOMEM = -2,          /* host or device allocation failed */
+    SRG_ERR_HIP = -3,            /* a HIP runtime call failed */
+    SRG_ERR_INDEX_RANGE = -4,    /* an index addressed past the vertex buffer */
+    SRG_ERR_OVERFLOW = -5,       /* a device arena stayed too small after growing */
+    SRG_ERR_UNKNOWN_SHADER = -6, /* shader id has no device implementation */
+    SRG_ERR_STATE = -7,          /* call out of order (e.g. draw outside a frame) */
+    SRG_ERR_NO_DEVICE = -8,      /* no HIP device, or the kernels are not loadable */
+    SRG_ERR_HANDLE = -9          /* unknown or released handle */
+};
+
+/* Built-in pixel shaders: device implementations of Viewer/Shaders.h. */
+enum srg_shader {
+    SRG_SHADER_UNLIT_DIFFUSE = 0,    /* Shaders.h:71-104: texture sample, white without texture */
+    SRG_SHADER_VISUALIZE_NORMALS = 1,/* Shaders.h:106-121 */
+    SRG_SHADER_VISUALIZE_UVS = 2     /* Shaders.h:123-130 */
+};
+
+/* Back-end tile-queue ordering. Every policy yields the same image. */
+enum srg_tile_order {
+    SRG_ORDER_NATURAL = 0,  /* row-major, view by view (Renderer.cpp:264-306) */
+    SRG_ORDER_TRICOUNT = 1, /* longest-first by binned-triangle count */
+    SRG_ORDER_LEARNED = 2   /* longest-first by the learned tile-cost model */
+};
+
+enum srg_config_flags {
+    SRG_CFG_KEEP_PRIMID = 1u << 0, /* keep a per-pixel primitive-order plane (tests) */
+    SRG_CFG_STATS = 1u << 1,       /* count rasterised fragments and binned entries */
+    SRG_CFG_TIMING = 1u << 2,      /* time each kernel with HIP events */
+    SRG_CFG_TILE_COSTS = 1u << 3   /* record per-tile s_memtime cost in the back end */
+};
+
+typedef struct srg_config {
+    uint32_t struct_size;   /* sizeof(srg_config) */
+    int32_t device;         /* HIP device ordinal */
+    uint32_t flags;         /* srg_config_flags */
+    uint32_t tile_order;    /* srg_tile_order */
+    uint32_t max_prims;     /* initial primitive-record capacity, 0 = sized per frame */
+    uint32_t max_entries;   /* initial tile-entry capacity, 0 = sized per frame */
+    uint32_t wg_per_cu;     /* persistent back-end workgroups per CU, 0 = default */
+    uint32_t reserved[9];
+} srg_config;
+
+/* Mirrors DrawCall (Renderer.h:119-150). Buffers are registered handles plus
+ * a byte offset; strides and counts have the reference's meaning:
+ * index stride 1/2/4, position = 3 floats at pos_offset + i*position_stride,
+ * attributes = attribute_stride/4 (<= 8) floats per vertex, uv_offset the
+ * float index of (u, v) inside the attributes (SetAttributeBuffer, Renderer.cpp:119-126).
+ * mvp is column-major: clip = mvp * (x, y, z, 1). */
+typedef struct srg_draw {
+    uint32_t shader;              /* srg_shader */
+    srg_handle texture;           /* UnlitDiffuse uniforms; 0 = none (writes 0xFFFFFFFF) */
+    srg_handle index_buffer;   uint64_t index_offset;     uint32_t index_stride;     uint32_t index_num;
+    srg_handle position_buffer; uint64_t position_offset; uint32_t position_stride;  uint32_t position_num;
+    srg_handle attribute_buffer; uint64_t attribute_offset; uint32_t attribute_stride; uint32_t attribute_num;
+    uint32_t uv_offset;
+    srg_fb* fb;                   /* target framebuffer */
+    uint32_t plane;               /* its plane (srg_fb_write_plane at SetFrameBuffer time) */
+    float mvp[16];
+} srg_draw;
+
+typedef struct srg_frame_stats {
+    uint64_t tris_in;         /* triangles submitted */
+    uint64_t prims;           /* primitive records after clipping (incl. culled fans) */
+    uint64_t bin_entries;     /* triangle-tile entries */
+    uint64_t frags_raster;    /* covered pixel samples before the depth test (SRG_CFG_STATS) */
+    uint32_t tiles_queued;    /* tiles processed by the back end */
+    uint32_t retries;         /* frame re-runs after arena growth */
+    float ms_frontend;        /* SRG_CFG_TIMING: setup + scan + emit */
+    float ms_backend;         /* SRG_CFG_TIMING: raster + shade kernel */
+    float ms_total;           /* SRG_CFG_TIMING: whole frame on the device */
+} srg_frame_stats;
+
+/* ---- context: RenderContext ctor / Shutdown (Renderer.cpp:138-159) ---- */
+int srg_create(const srg_config* cfg, srg_ctx** out);
+void srg_destroy(srg_ctx* ctx);
+const char* srg_status_string(int status);
+
+/* ---- resources ---- */
+/* Copies `bytes` of host memory to a new device buffer. Registered data is
+ * treated as immutable; re-register (or srg_update_buffer) after changes. */
+int srg_register_buffer(srg_ctx* ctx, const void* host, size_t bytes, srg_handle* out);
+/* Wraps existing device memory (not copied, not owned). */
+int srg_register_device_buffer(srg_ctx* ctx, void* device_ptr, size_t bytes, srg_handle* out);
+int srg_update_buffer(srg_ctx* ctx, srg_handle h, size_t offset, const void* host, size_t bytes);
+int srg_release_buffer(srg_ctx* ctx, srg_handle h);
+/* Tex::TextureData (Texture.h:21-41): tiled + Morton-swizzled RGBA8 mip chain
+ * as produced by srg_texture_build_rgba8 / TextureData::CreateFromRGBA8. */
+int srg_register_texture(srg_ctx* ctx, const void* tiled_texels, uint64_t bytes, const uint32_t mip_offsets[14],
+                         uint32_t width_log2, uint32_t height_log2, uint32_t num_mips, srg_handle* out);
+int srg_release_texture(srg_ctx* ctx, srg_handle h);
+
+/* ---- framebuffers: FrameBuffer (Renderer.h:75-108, Renderer.cpp:23-57) ---- */
+enum srg_fb_flags { SRG_FB_COLOUR = 1u, SRG_FB_DEPTH = 2u };
+int srg_fb_create(srg_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags, srg_fb** out);
+void srg_fb_destroy(srg_ctx* ctx, srg_fb* fb);
+uint32_t srg_fb_write_plane(const srg_fb* fb);
+void srg_fb_swap_planes(srg_fb* fb);
+/* Copies a plane's tiled contents (tiles_x*tiles_y*4096 elements each) to host
+ * memory; any pointer may be NULL. primid needs SRG_CFG_KEEP_PRIMID. */
+int srg_fb_read_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, uint32_t* colour, float* depth, uint32_t* primid);
+/* Uploads tiled contents into a plane (e.g. a non-cleared depth buffer). */
+int srg_fb_write_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, const uint32_t* colour, const float* depth);
+
+/* ---- frame: BeginFrame / ClearFrameBuffer / DrawIndexed / EndFrame / Blit ---- */
+int srg_begin_frame(srg_ctx* ctx);
+int srg_clear(srg_ctx* ctx, srg_fb* fb, uint32_t colour, int clear_colour, int clear_depth);
+int srg_draw_indexed(srg_ctx* ctx, const srg_draw* draw);
+/* Synchronous like the reference: on return every tile is rastered and shaded. */
+int srg_end_frame(srg_ctx* ctx);
+/* Copies the write plane's colour to linear RGBA8 rows (width*height*4 bytes),
+ * swaps the planes, then calls on_finish(user) (Renderer.cpp:350-372). */
+int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear_rgba8, void (*on_finish)(void*), void* user);
+int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out);
+
+/* ---- host helper: TextureData::CreateFromRGBA8 (Texture.cpp:119-199) ----
+ * Tiles a power-of-two RGBA8 image (>= 32 in both dimensions) into 32x32
+ * Morton-swizzled tiles and, if calc_mips, appends the mip chain. Call with
+ * out_tiled == NULL to get the byte size in *out_bytes. */
+int srg_texture_build_rgba8(const uint8_t* linear_rgba8, uint32_t width, uint32_t height, int calc_mips,
+                            uint8_t* out_tiled, uint64_t* out_bytes, uint32_t mip_offsets[14],
+                            uint32_t* width_log2, uint32_t* height_log2, uint32_t* num_mips);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

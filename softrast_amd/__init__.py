@@ -1,0 +1,414 @@
+"""softrast_amd -- MI355X-native sort-middle rasteriser (SoftRast's hot path).
+
+The compute path is libsrgpu.so (hand-written HIP for gfx950 behind the C ABI
+in include/srgpu.h). This module binds that ABI with ctypes and mirrors the
+reference's host interface (SoftRast/Renderer.h:110-177, Texture.h:21-41):
+
+    ctx = RenderContext()
+    fb = FrameBuffer(ctx, 1280, 720)
+    ctx.BeginFrame()
+    ctx.ClearFrameBuffer(fb, 0)
+    call = DrawCall().SetFrameBuffer(fb).SetMVP(mvp) \\
+        .SetIndexBuffer(indices, 2, n).SetPositionBuffer(verts, 32, nv) \\
+        .SetAttributeBuffer(verts, 32, nv, 6).SetPixelShader(UnlitDiffuseShader, tex)
+    ctx.DrawIndexed(call)
+    ctx.EndFrame()
+    ctx.Blit(fb, out_rgba8)
+
+Errors raise SrgError carrying the library status. There is no CPU fallback:
+if the library is missing or no GPU is present, construction fails.
+"""
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+_PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libsrgpu.so"
+
+SRG_OK = 0
+SHADER_UNLIT_DIFFUSE = 0
+SHADER_VISUALIZE_NORMALS = 1
+SHADER_VISUALIZE_UVS = 2
+ORDER_NATURAL, ORDER_TRICOUNT, ORDER_LEARNED = 0, 1, 2
+CFG_KEEP_PRIMID, CFG_STATS, CFG_TIMING, CFG_TILE_COSTS = 1, 2, 4, 8
+FB_COLOUR, FB_DEPTH = 1, 2
+
+SRG_SYMBOLS = (
+    "srg_create", "srg_destroy", "srg_status_string", "srg_register_buffer", "srg_register_device_buffer",
+    "srg_update_buffer", "srg_release_buffer", "srg_register_texture", "srg_release_texture",
+    "srg_fb_create", "srg_fb_destroy", "srg_fb_write_plane", "srg_fb_swap_planes", "srg_fb_read_tiles",
+    "srg_fb_write_tiles", "srg_begin_frame", "srg_clear", "srg_draw_indexed", "srg_end_frame", "srg_blit",
+    "srg_get_frame_stats", "srg_texture_build_rgba8",
+)
+
+
+class SrgError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} ({status})")
+
+
+class SrgConfig(ctypes.Structure):
+    _fields_ = [("struct_size", ctypes.c_uint32), ("device", ctypes.c_int32), ("flags", ctypes.c_uint32),
+                ("tile_order", ctypes.c_uint32), ("max_prims", ctypes.c_uint32), ("max_entries", ctypes.c_uint32),
+                ("wg_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 9)]
+
+
+class SrgDraw(ctypes.Structure):
+    _fields_ = [("shader", ctypes.c_uint32), ("texture", ctypes.c_uint64),
+                ("index_buffer", ctypes.c_uint64), ("index_offset", ctypes.c_uint64),
+                ("index_stride", ctypes.c_uint32), ("index_num", ctypes.c_uint32),
+                ("position_buffer", ctypes.c_uint64), ("position_offset", ctypes.c_uint64),
+                ("position_stride", ctypes.c_uint32), ("position_num", ctypes.c_uint32),
+                ("attribute_buffer", ctypes.c_uint64), ("attribute_offset", ctypes.c_uint64),
+                ("attribute_stride", ctypes.c_uint32), ("attribute_num", ctypes.c_uint32),
+                ("uv_offset", ctypes.c_uint32), ("fb", ctypes.c_void_p), ("plane", ctypes.c_uint32),
+                ("mvp", ctypes.c_float * 16)]
+
+
+class SrgFrameStats(ctypes.Structure):
+    _fields_ = [("tris_in", ctypes.c_uint64), ("prims", ctypes.c_uint64), ("bin_entries", ctypes.c_uint64),
+                ("frags_raster", ctypes.c_uint64), ("tiles_queued", ctypes.c_uint32), ("retries", ctypes.c_uint32),
+                ("ms_frontend", ctypes.c_float), ("ms_backend", ctypes.c_float), ("ms_total", ctypes.c_float)]
+
+
+_lib = None
+
+
+def load_library(path=None):
+    """Load libsrgpu.so and declare its prototypes. Raises if it is absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is not built; run softrast_amd/build.py (no CPU fallback exists)")
+    lib = ctypes.CDLL(str(p))
+    vp, u32, i32, u64, sz = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, ctypes.c_size_t
+    pu64 = ctypes.POINTER(ctypes.c_uint64)
+    sigs = {
+        "srg_create": (i32, [ctypes.POINTER(SrgConfig), ctypes.POINTER(vp)]),
+        "srg_destroy": (None, [vp]),
+        "srg_status_string": (ctypes.c_char_p, [i32]),
+        "srg_register_buffer": (i32, [vp, vp, sz, pu64]),
+        "srg_register_device_buffer": (i32, [vp, vp, sz, pu64]),
+        "srg_update_buffer": (i32, [vp, u64, sz, vp, sz]),
+        "srg_release_buffer": (i32, [vp, u64]),
+        "srg_register_texture": (i32, [vp, vp, u64, ctypes.POINTER(u32), u32, u32, u32, pu64]),
+        "srg_release_texture": (i32, [vp, u64]),
+        "srg_fb_create": (i32, [vp, u32, u32, u32, ctypes.POINTER(vp)]),
+        "srg_fb_destroy": (None, [vp, vp]),
+        "srg_fb_write_plane": (u32, [vp]),
+        "srg_fb_swap_planes": (None, [vp]),
+        "srg_fb_read_tiles": (i32, [vp, vp, u32, vp, vp, vp]),
+        "srg_fb_write_tiles": (i32, [vp, vp, u32, vp, vp]),
+        "srg_begin_frame": (i32, [vp]),
+        "srg_clear": (i32, [vp, vp, u32, i32, i32]),
+        "srg_draw_indexed": (i32, [vp, ctypes.POINTER(SrgDraw)]),
+        "srg_end_frame": (i32, [vp]),
+        "srg_blit": (i32, [vp, vp, vp, vp, vp]),
+        "srg_get_frame_stats": (i32, [vp, ctypes.POINTER(SrgFrameStats)]),
+        "srg_texture_build_rgba8": (i32, [vp, u32, u32, i32, vp, pu64, ctypes.POINTER(u32),
+                                          ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def status_string(status):
+    try:
+        return load_library().srg_status_string(status).decode()
+    except Exception:
+        return "unknown"
+
+
+def _check(status, what):
+    if status != SRG_OK:
+        raise SrgError(status, what)
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# textures: Tex::TextureData (Texture.h:21-41)
+# ---------------------------------------------------------------------------
+
+class TextureData:
+    """Tiled, Morton-swizzled RGBA8 mip chain (Texture.cpp:119-199)."""
+
+    def __init__(self):
+        self.m_texels = np.zeros(0, np.uint8)
+        self.m_mipOffsets = np.zeros(14, np.uint32)
+        self.m_widthLog2 = 0
+        self.m_heightLog2 = 0
+        self.m_numMips = 0
+        self.m_bytesPerPixel = 0
+
+    def CreateFromRGBA8(self, texels, width, height, calcMips=False):
+        lib = load_library()
+        src = np.ascontiguousarray(texels, dtype=np.uint8).reshape(-1)
+        if src.size != width * height * 4:
+            raise ValueError("texel array does not match width*height*4")
+        nbytes = ctypes.c_uint64()
+        offs = (ctypes.c_uint32 * 14)()
+        wl, hl, nm = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib.srg_texture_build_rgba8(_ptr(src), width, height, int(calcMips), None, ctypes.byref(nbytes), offs,
+                                           ctypes.byref(wl), ctypes.byref(hl), ctypes.byref(nm)), "CreateFromRGBA8")
+        out = np.zeros(nbytes.value, np.uint8)
+        _check(lib.srg_texture_build_rgba8(_ptr(src), width, height, int(calcMips), _ptr(out), ctypes.byref(nbytes),
+                                           offs, ctypes.byref(wl), ctypes.byref(hl), ctypes.byref(nm)),
+               "CreateFromRGBA8")
+        self.m_texels = out
+        self.m_mipOffsets = np.array(list(offs), np.uint32)
+        self.m_widthLog2, self.m_heightLog2, self.m_numMips = wl.value, hl.value, nm.value
+        self.m_bytesPerPixel = 4
+        return self
+
+    def Clear(self):
+        self.m_texels = np.zeros(0, np.uint8)
+
+
+# ---------------------------------------------------------------------------
+# shaders: Viewer/Shaders.h. Host shader callbacks cannot run on the device;
+# these names map to the device implementations.
+# ---------------------------------------------------------------------------
+
+class _Shader:
+    def __init__(self, name, sid):
+        self.name, self.id = name, sid
+
+    def __repr__(self):
+        return self.name
+
+
+UnlitDiffuseShader = _Shader("UnlitDiffuseShader", SHADER_UNLIT_DIFFUSE)
+VisualizeNormalsShader = _Shader("VisualizeNormalsShader", SHADER_VISUALIZE_NORMALS)
+VisualizeUVsShader = _Shader("VisualizeUVsShader", SHADER_VISUALIZE_UVS)
+
+
+# ---------------------------------------------------------------------------
+# frame API: RenderContext / FrameBuffer / DrawCall (Renderer.h)
+# ---------------------------------------------------------------------------
+
+class GenericDrawBuffer:
+    def __init__(self):
+        self.m_ptr = None
+        self.m_num = 0
+        self.m_stride = 0
+
+
+class DrawCall:
+    """Per-draw state (Renderer.h:119-150). m_ptr holds a numpy array."""
+
+    UV_OFFSET_INVALID = 0xFFFFFFFF
+
+    def __init__(self):
+        self.m_pixelShader = None
+        self.m_pixelUniforms = None
+        self.m_indexBuffer = GenericDrawBuffer()
+        self.m_positionBuffer = GenericDrawBuffer()
+        self.m_attributeBuffer = GenericDrawBuffer()
+        self.m_uvOffset = 0
+        self.m_frameBuffer = None     # (FrameBuffer, plane)
+        self.m_mvp = np.eye(4, dtype=np.float32)  # column-major as m_mvp[col][row]
+        self.m_drawCallIdx = 0
+
+    def SetPixelShader(self, fn, uniforms):
+        self.m_pixelShader, self.m_pixelUniforms = fn, uniforms
+        return self
+
+    def SetIndexBuffer(self, buf, stride, num):
+        self.m_indexBuffer.m_ptr, self.m_indexBuffer.m_stride, self.m_indexBuffer.m_num = buf, stride, num
+        return self
+
+    def SetPositionBuffer(self, buf, stride, num):
+        self.m_positionBuffer.m_ptr, self.m_positionBuffer.m_stride, self.m_positionBuffer.m_num = buf, stride, num
+        return self
+
+    def SetAttributeBuffer(self, buf, stride, num, uvOffset=0):
+        self.m_uvOffset = uvOffset
+        b = self.m_attributeBuffer
+        b.m_ptr, b.m_stride, b.m_num = buf, stride, num
+        return self
+
+    def SetFrameBuffer(self, fb):
+        self.m_frameBuffer = (fb, fb.m_writePlane)
+        return self
+
+    def SetMVP(self, mvp):
+        """mvp: 16 floats column-major, or a (4, 4) array indexed [col][row]."""
+        self.m_mvp = np.asarray(mvp, dtype=np.float32).reshape(4, 4)
+        return self
+
+
+class FrameBuffer:
+    """Double-buffered tiled RGBA8 + fp32 depth (Renderer.h:75-108)."""
+
+    def __init__(self, ctx, width, height, colour=True, depth=True):
+        self._ctx = ctx
+        self.m_width, self.m_height = width, height
+        self.tiles_x, self.tiles_y = (width + 63) // 64, (height + 63) // 64
+        h = ctypes.c_void_p()
+        flags = (FB_COLOUR if colour else 0) | (FB_DEPTH if depth else 0)
+        _check(ctx._lib.srg_fb_create(ctx._ctx, width, height, flags, ctypes.byref(h)), "FrameBuffer")
+        self._h = h
+
+    @property
+    def m_writePlane(self):
+        return self._ctx._lib.srg_fb_write_plane(self._h)
+
+    def SwapPlanes(self):
+        self._ctx._lib.srg_fb_swap_planes(self._h)
+
+    def read_tiles(self, plane=None, primid=False):
+        """Tiled (colour u32, depth f32[, primitive order u32]) of a plane, shape (tiles, 64, 64)."""
+        plane = self.m_writePlane if plane is None else plane
+        n = self.tiles_x * self.tiles_y * 4096
+        col = np.empty(n, np.uint32)
+        dep = np.empty(n, np.float32)
+        pid = np.empty(n, np.uint32) if primid else None
+        _check(self._ctx._lib.srg_fb_read_tiles(self._ctx._ctx, self._h, plane, _ptr(col), _ptr(dep), _ptr(pid)),
+               "read_tiles")
+        shp = (self.tiles_x * self.tiles_y, 64, 64)
+        out = (col.reshape(shp), dep.reshape(shp))
+        return out + (pid.reshape(shp),) if primid else out
+
+    def write_tiles(self, colour=None, depth=None, plane=None):
+        plane = self.m_writePlane if plane is None else plane
+        c = None if colour is None else np.ascontiguousarray(colour, np.uint32).reshape(-1)
+        d = None if depth is None else np.ascontiguousarray(depth, np.float32).reshape(-1)
+        _check(self._ctx._lib.srg_fb_write_tiles(self._ctx._ctx, self._h, plane, _ptr(c), _ptr(d)), "write_tiles")
+
+    def destroy(self):
+        if self._h:
+            self._ctx._lib.srg_fb_destroy(self._ctx._ctx, self._h)
+            self._h = None
+
+
+class RenderContext:
+    """RenderContext (Renderer.h:153-177) over libsrgpu.
+
+    Host arrays passed in DrawCalls are uploaded once and cached by
+    (address, size, dtype); call invalidate(array) after changing one.
+    """
+
+    def __init__(self, device=0, flags=0, tile_order=ORDER_NATURAL, max_prims=0, max_entries=0, wg_per_cu=0,
+                 lib_path=None):
+        self._lib = load_library(lib_path)
+        cfg = SrgConfig()
+        cfg.struct_size = ctypes.sizeof(SrgConfig)
+        cfg.device, cfg.flags, cfg.tile_order = device, flags, tile_order
+        cfg.max_prims, cfg.max_entries, cfg.wg_per_cu = max_prims, max_entries, wg_per_cu
+        h = ctypes.c_void_p()
+        _check(self._lib.srg_create(ctypes.byref(cfg), ctypes.byref(h)), "RenderContext")
+        self._ctx = h
+        self._buffers = {}
+        self._textures = {}
+        self._keep = []
+        self._draws = []
+
+    # -- resources ---------------------------------------------------------
+    def buffer_handle(self, arr):
+        arr = np.ascontiguousarray(arr)
+        key = (arr.ctypes.data, arr.nbytes, arr.dtype.str)
+        h = self._buffers.get(key)
+        if h is None:
+            out = ctypes.c_uint64()
+            _check(self._lib.srg_register_buffer(self._ctx, _ptr(arr), arr.nbytes, ctypes.byref(out)), "register_buffer")
+            h = out.value
+            self._buffers[key] = h
+            self._keep.append(arr)
+        return h
+
+    def invalidate(self, arr):
+        key = (arr.ctypes.data, arr.nbytes, arr.dtype.str)
+        h = self._buffers.pop(key, None)
+        if h is not None:
+            self._lib.srg_release_buffer(self._ctx, h)
+
+    def texture_handle(self, tex):
+        if tex is None:
+            return 0
+        key = id(tex)
+        h = self._textures.get(key)
+        if h is None:
+            out = ctypes.c_uint64()
+            offs = (ctypes.c_uint32 * 14)(*[int(x) for x in tex.m_mipOffsets])
+            data = np.ascontiguousarray(tex.m_texels, np.uint8)
+            _check(self._lib.srg_register_texture(self._ctx, _ptr(data), data.nbytes, offs, tex.m_widthLog2,
+                                                  tex.m_heightLog2, tex.m_numMips, ctypes.byref(out)),
+                   "register_texture")
+            h = out.value
+            self._textures[key] = h
+            self._keep.append(tex)
+        return h
+
+    # -- frame -------------------------------------------------------------
+    def BeginFrame(self):
+        self._draws = []
+        _check(self._lib.srg_begin_frame(self._ctx), "BeginFrame")
+
+    def ClearFrameBuffer(self, fb, color=0, clearColour=True, clearDepth=True):
+        _check(self._lib.srg_clear(self._ctx, fb._h, color & 0xFFFFFFFF, int(clearColour), int(clearDepth)),
+               "ClearFrameBuffer")
+
+    def DrawIndexed(self, call):
+        if call.m_indexBuffer.m_ptr is None:
+            raise SrgError(-1, "DrawIndexed: no index buffer bound")
+        d = SrgDraw()
+        shader = call.m_pixelShader if call.m_pixelShader is not None else VisualizeNormalsShader
+        d.shader = shader.id
+        d.texture = self.texture_handle(call.m_pixelUniforms) if shader.id == SHADER_UNLIT_DIFFUSE else 0
+        if call.m_pixelUniforms is not None and shader.id == SHADER_UNLIT_DIFFUSE and \
+                call.m_pixelUniforms.m_texels.size == 0:
+            d.texture = 0
+        for name, buf in (("index", call.m_indexBuffer), ("position", call.m_positionBuffer),
+                          ("attribute", call.m_attributeBuffer)):
+            setattr(d, name + "_buffer", self.buffer_handle(buf.m_ptr))
+            setattr(d, name + "_offset", 0)
+            setattr(d, name + "_stride", buf.m_stride)
+            setattr(d, name + "_num", buf.m_num)
+        d.uv_offset = call.m_uvOffset
+        fb, plane = call.m_frameBuffer
+        d.fb, d.plane = fb._h, plane
+        d.mvp[:] = [float(x) for x in np.asarray(call.m_mvp, np.float32).reshape(-1)]
+        _check(self._lib.srg_draw_indexed(self._ctx, ctypes.byref(d)), "DrawIndexed")
+        call.m_drawCallIdx = len(self._draws)
+        self._draws.append(call)
+
+    def EndFrame(self):
+        _check(self._lib.srg_end_frame(self._ctx), "EndFrame")
+
+    def Blit(self, fb, linear=None, onFinish=None, user=None):
+        out = linear if linear is not None else np.empty((fb.m_height, fb.m_width), np.uint32)
+        if out.nbytes < fb.m_width * fb.m_height * 4 or not out.flags.c_contiguous:
+            raise ValueError("Blit target too small or not contiguous")
+        _check(self._lib.srg_blit(self._ctx, fb._h, _ptr(out), None, None), "Blit")
+        if onFinish is not None:
+            onFinish(user)
+        return out
+
+    def frame_stats(self):
+        s = SrgFrameStats()
+        _check(self._lib.srg_get_frame_stats(self._ctx, ctypes.byref(s)), "frame_stats")
+        return {k: getattr(s, k) for k, _ in SrgFrameStats._fields_}
+
+    def Shutdown(self):
+        if self._ctx:
+            self._lib.srg_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.Shutdown()
+        except Exception:
+            pass

@@ -1,0 +1,124 @@
+"""Seeded synthetic scenes for the benchmark configurations and tests.
+
+Wraps libsrscene.so (csrc/scene.cpp). Meshes use SoftRast's Obj::Vertex
+layout (pos3, norm3, uv2 = 32 bytes, Viewer/Obj.h:16-21), so one array is
+bound as both the position and the attribute buffer with uv offset 6, as
+Viewer/Scene.cpp:43-47 does. Matrices are column-major (clip = M * p).
+
+Seeds: master seeds 0x5EED0001.. per configuration, expanded by SplitMix64
+into PCG32 streams.
+"""
+import ctypes
+import math
+import pathlib
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_PKG = pathlib.Path(__file__).resolve().parent
+_lib = None
+
+VERTEX_DTYPE = np.dtype([("pos", np.float32, 3), ("nrm", np.float32, 3), ("uv", np.float32, 2)])
+SEED_C1, SEED_C2, SEED_C3, SEED_C4, SEED_C5 = 0x5EED0001, 0x5EED0002, 0x5EED0003, 0x5EED0004, 0x5EED0005
+REVERSE_Z_NEAR, REVERSE_Z_FAR = 10000.0, 0.1  # Viewer/Scene.cpp:20-22
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        p = _PKG / "libsrscene.so"
+        if not p.exists():
+            raise FileNotFoundError(f"{p} is not built; run softrast_amd/build.py")
+        lib = ctypes.CDLL(str(p))
+        d, f, u32, u64, vp = ctypes.c_double, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p
+        lib.srs_camera_mvp.argtypes = [ctypes.POINTER(d), ctypes.POINTER(d), d, d, d, d, ctypes.POINTER(f)]
+        lib.srs_camera_mvp.restype = None
+        lib.srs_value_noise_rgba8.argtypes = [u64, u32, u32, vp]
+        lib.srs_value_noise_rgba8.restype = None
+        lib.srs_cube_grid.argtypes = [u64, u32, u32, vp, vp]
+        lib.srs_cube_grid.restype = u32
+        lib.srs_spheres.argtypes = [u64, u32, u32, u32, d, d, d, d, d, u32, u32, d, vp, vp, ctypes.POINTER(d)]
+        lib.srs_spheres.restype = u32
+        _lib = lib
+    return _lib
+
+
+def camera_mvp(eye, target, fov_y_deg, aspect, znear=REVERSE_Z_NEAR, zfar=REVERSE_Z_FAR):
+    lib = _load()
+    e = (ctypes.c_double * 3)(*eye)
+    t = (ctypes.c_double * 3)(*target)
+    out = (ctypes.c_float * 16)()
+    lib.srs_camera_mvp(e, t, math.radians(fov_y_deg), aspect, znear, zfar, out)
+    return np.array(list(out), np.float32)
+
+
+def value_noise_rgba8(seed, width, height):
+    out = np.empty((height, width, 4), np.uint8)
+    _load().srs_value_noise_rgba8(seed, width, height, out.ctypes.data)
+    return out
+
+
+@dataclass
+class Draw:
+    vertices: np.ndarray          # VERTEX_DTYPE
+    indices: np.ndarray           # uint16 / uint32
+    texture: int                  # index into Scene.textures, -1 = none
+    shader: int = 0
+    uv_offset: int = 6
+
+
+@dataclass
+class Scene:
+    width: int
+    height: int
+    mvp: np.ndarray
+    draws: list
+    textures: list = field(default_factory=list)   # list of (linear RGBA8 (h,w,4))
+    info: dict = field(default_factory=dict)
+
+    @property
+    def num_tris(self):
+        return sum(len(d.indices) // 3 for d in self.draws)
+
+
+def cube_grid_scene(seed=SEED_C1, gx=32, gz=32, width=1280, height=720, tex_size=512):
+    """1280x720, 32x32 unit cubes (12,288 tris), one 512^2 value-noise texture."""
+    lib = _load()
+    nv = lib.srs_cube_grid(seed, gx, gz, None, None)
+    v = np.empty(nv, VERTEX_DTYPE)
+    i = np.empty(gx * gz * 36, np.uint16)
+    lib.srs_cube_grid(seed, gx, gz, v.ctypes.data, i.ctypes.data)
+    mvp = camera_mvp((0.0, 8.0, -20.0), (0.0, 0.0, 0.0), 60.0, width / height)
+    tex = value_noise_rgba8(seed ^ 0x7E47, tex_size, tex_size)
+    return Scene(width, height, mvp, [Draw(v, i, 0)], [tex], {"config": "cube_grid"})
+
+
+def spheres_scene(seed=SEED_C2, nspheres=4000, ndraws=8, width=1920, height=1080, tex_size=2048,
+                  slices=25, stacks=6, sigma=0.8, depth_complexity=4.0, zmin=20.0, zmax=80.0, fov=60.0):
+    """1920x1080, 4,000 UV spheres x 250 tris = 1M tris, 8 draws x 2048^2 textures."""
+    lib = _load()
+    aspect = width / height
+    nv = lib.srs_spheres(seed, nspheres, slices, stacks, sigma, math.radians(fov), aspect, zmin, zmax, width, height,
+                         depth_complexity, None, None, None)
+    tris_per = 2 * slices * (stacks - 1)
+    v = np.empty(nv, VERTEX_DTYPE)
+    i = np.empty(nspheres * tris_per * 3, np.uint32)
+    scale = ctypes.c_double()
+    lib.srs_spheres(seed, nspheres, slices, stacks, sigma, math.radians(fov), aspect, zmin, zmax, width, height,
+                    depth_complexity, v.ctypes.data, i.ctypes.data, ctypes.byref(scale))
+    mvp = camera_mvp((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), fov, aspect)
+    vps = (slices + 1) * (stacks + 1)
+    per_draw = (nspheres + ndraws - 1) // ndraws
+    draws = []
+    for k in range(ndraws):
+        s0, s1 = k * per_draw, min(nspheres, (k + 1) * per_draw)
+        if s0 >= s1:
+            break
+        vv = v[s0 * vps:s1 * vps].copy()
+        ii = (i[s0 * tris_per * 3:s1 * tris_per * 3] - np.uint32(s0 * vps)).astype(np.uint32)
+        draws.append(Draw(vv, ii, k))
+    texs = [value_noise_rgba8(seed ^ (0x7E470000 + k), tex_size, tex_size) for k in range(ndraws)] if tex_size else []
+    if not texs:
+        for d in draws:
+            d.texture = -1
+    return Scene(width, height, mvp, draws, texs, {"config": "spheres", "radius_scale": scale.value})

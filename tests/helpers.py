@@ -1,0 +1,139 @@
+"""Shared test scaffolding: seeded scenes, and the same frame rendered by the
+CPU oracle and by the GPU library through the C ABI."""
+import numpy as np
+
+from softrast_amd import scenes
+
+ATTR_STRIDE = 32
+
+
+def random_scene(seed, ntris, width, height, clip_heavy=False, index_stride=4, shaders=(0,), uv_offset=6,
+                 ndraws=1, tex_size=(64, 64), with_texture=True, coplanar_dups=0):
+    """Random triangles in front of a reverse-Z camera. With clip_heavy, vertices
+    extend outside the frustum (sides and the z < 0.1 near region) so many
+    triangles go through the clipper. coplanar_dups re-submits that many
+    earlier triangles unchanged (exact depth ties)."""
+    rng = np.random.default_rng(seed)
+    draws = []
+    for k in range(ndraws):
+        n = ntris // ndraws
+        centres = np.stack([rng.uniform(-12, 12, n), rng.uniform(-7, 7, n), rng.uniform(6, 40, n)], 1)
+        if clip_heavy:
+            centres[:, 0] *= 2.2
+            centres[:, 1] *= 2.2
+            centres[:, 2] = rng.uniform(-4, 30, n)
+        size = np.exp(rng.normal(-0.5, 1.0, n))[:, None, None]
+        offs = rng.normal(0, 1, (n, 3, 3)) * size
+        pos = (centres[:, None, :] + offs).reshape(-1, 3)
+        if coplanar_dups and k == 0:
+            pick = rng.integers(0, n, coplanar_dups)
+            pos = np.concatenate([pos, pos.reshape(-1, 3, 3)[pick].reshape(-1, 3)])
+        nv = len(pos)
+        v = np.zeros((nv, 8), np.float32)
+        v[:, 0:3] = pos
+        nrm = rng.normal(0, 1, (nv, 3))
+        v[:, 3:6] = nrm / np.linalg.norm(nrm, axis=1, keepdims=True)
+        v[:, 6:8] = rng.uniform(-3, 3, (nv, 2))
+        if uv_offset != 6:
+            v[:, uv_offset:uv_offset + 2] = rng.uniform(-2, 2, (nv, 2))
+        # shuffle vertex storage so indices are not trivial
+        perm = rng.permutation(nv)
+        inv = np.empty(nv, np.int64)
+        inv[perm] = np.arange(nv)
+        v = v[perm]
+        idx = inv[np.arange(nv)]
+        # both windings so about half the triangles face the camera
+        t = idx.reshape(-1, 3)
+        flip = rng.random(len(t)) < 0.5
+        t[flip] = t[flip][:, ::-1]
+        idx = t.reshape(-1)
+        if index_stride == 1:
+            assert nv <= 256
+        dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[index_stride]
+        draws.append(scenes.Draw(v.view(scenes.VERTEX_DTYPE).reshape(-1), idx.astype(dt),
+                                 0 if with_texture else -1, shaders[k % len(shaders)], uv_offset))
+    mvp = scenes.camera_mvp((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 60.0, width / height)
+    texs = []
+    if with_texture:
+        tw, th = tex_size
+        texs = [rng.integers(0, 256, (th, tw, 4), dtype=np.uint8)]
+    return scenes.Scene(width, height, mvp, draws, texs, {"config": "random"})
+
+
+def build_textures(scene):
+    from softrast_amd import TextureData
+    out = []
+    for t in scene.textures:
+        h, w = t.shape[:2]
+        out.append(TextureData().CreateFromRGBA8(t, w, h, True))
+    return out
+
+
+def oracle_render(scene, texdata, nthreads=1, clear=0, plane_init=None, fast=False, tile_entries=False):
+    """Render one frame of `scene` with the oracle; returns (plane, stats)."""
+    from oracle.oracle import Oracle, Plane, make_texture
+    o = Oracle(nthreads=nthreads, fast=fast)
+    p = Plane(scene.width, scene.height)
+    if plane_init is not None:
+        p.colour[:] = plane_init[0].reshape(-1)
+        p.depth[:] = plane_init[1].reshape(-1)
+    if clear is not None:
+        o.clear(p, clear)
+    stex = [make_texture(t) for t in texdata]
+    draws = []
+    for d in scene.draws:
+        vf = d.vertices.view(np.float32).reshape(-1, 8)
+        draws.append(dict(shader=d.shader, tex=stex[d.texture][0] if d.texture >= 0 else None, idx=d.indices,
+                          idx_stride=d.indices.dtype.itemsize, pos=vf, pos_stride=ATTR_STRIDE, pos_num=len(vf),
+                          attr=vf, attr_stride=ATTR_STRIDE, attr_num=len(vf), uv_offset=d.uv_offset, plane=0,
+                          mvp=scene.mvp))
+    stats = o.render(draws, [p], tile_entries=tile_entries)
+    o.close()
+    return p, stats
+
+
+def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, **ctx_kw):
+    """Render one frame on the GPU through the C ABI; returns (colour, depth, primid, stats)."""
+    import softrast_amd as sr
+    if flags is None:
+        flags = sr.CFG_KEEP_PRIMID | sr.CFG_STATS
+    own = ctx is None
+    if own:
+        ctx = sr.RenderContext(flags=flags, **ctx_kw)
+    fb = sr.FrameBuffer(ctx, scene.width, scene.height)
+    if plane_init is not None:
+        fb.write_tiles(plane_init[0], plane_init[1])
+    ctx.BeginFrame()
+    if clear is not None:
+        ctx.ClearFrameBuffer(fb, clear)
+    for d in scene.draws:
+        call = sr.DrawCall()
+        call.SetFrameBuffer(fb).SetMVP(scene.mvp)
+        call.SetIndexBuffer(d.indices, d.indices.dtype.itemsize, len(d.indices))
+        call.SetPositionBuffer(d.vertices, ATTR_STRIDE, len(d.vertices))
+        call.SetAttributeBuffer(d.vertices, ATTR_STRIDE, len(d.vertices), d.uv_offset)
+        shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader}[d.shader]
+        call.SetPixelShader(shader, texdata[d.texture] if d.texture >= 0 else None)
+        ctx.DrawIndexed(call)
+    ctx.EndFrame()
+    stats = ctx.frame_stats()
+    res = fb.read_tiles(primid=True) if flags & sr.CFG_KEEP_PRIMID else fb.read_tiles() + (None,)
+    fb.destroy()
+    if own:
+        ctx.Shutdown()
+    return res[0].reshape(-1), res[1].reshape(-1), (res[2].reshape(-1) if res[2] is not None else None), stats
+
+
+def compare(oracle_plane, colour, depth, primid=None):
+    """Differing-pixel counts: coverage (primid != none), primitive id, depth bits, RGBA8 (exact / >1 LSB)."""
+    out = {}
+    if primid is not None:
+        out["coverage"] = int(np.count_nonzero((oracle_plane.primid != 0xFFFFFFFF) != (primid != 0xFFFFFFFF)))
+        out["primid"] = int(np.count_nonzero(oracle_plane.primid != primid))
+    out["depth"] = int(np.count_nonzero(oracle_plane.depth.view(np.uint32) != depth.view(np.uint32)))
+    a = oracle_plane.colour.view(np.uint8).reshape(-1, 4).astype(np.int16)
+    b = colour.view(np.uint8).reshape(-1, 4).astype(np.int16)
+    diff = np.abs(a - b).max(axis=1)
+    out["colour"] = int(np.count_nonzero(diff))
+    out["colour_gt1"] = int(np.count_nonzero(diff > 1))
+    return out
