@@ -68,7 +68,9 @@ typedef struct srg_config {
     uint32_t max_prims;     /* initial primitive-record capacity, 0 = sized per frame */
     uint32_t max_entries;   /* initial tile-entry capacity, 0 = sized per frame */
     uint32_t wg_per_cu;     /* persistent back-end workgroups per CU, 0 = default */
-    uint32_t reserved[9];
+    uint32_t bin_size;      /* back-end work bin edge: 64, 32 or 16 pixels, 0 = 32. The
+                               reference's 64x64 bin semantics are kept for every size. */
+    uint32_t reserved[8];
 } srg_config;
 
 /* Mirrors DrawCall (Renderer.h:119-150). Buffers are registered handles plus
@@ -91,8 +93,10 @@ typedef struct srg_draw {
 
 typedef struct srg_frame_stats {
     uint64_t tris_in;         /* triangles submitted */
-    uint64_t prims;           /* primitive records after clipping (incl. culled fans) */
-    uint64_t bin_entries;     /* triangle-tile entries */
+    uint64_t tris_clipped;    /* triangles routed through the clipper */
+    uint64_t prims;           /* primitives surviving clip and cull (SRG_CFG_STATS) */
+    uint64_t bin_entries;     /* triangle-64x64-tile entries as the reference bins them (SRG_CFG_STATS) */
+    uint64_t work_entries;    /* entries in the back end's work bins */
     uint64_t frags_raster;    /* covered pixel samples before the depth test (SRG_CFG_STATS) */
     uint32_t tiles_queued;    /* tiles processed by the back end */
     uint32_t retries;         /* frame re-runs after arena growth */
@@ -142,6 +146,13 @@ int srg_end_frame(srg_ctx* ctx);
  * swaps the planes, then calls on_finish(user) (Renderer.cpp:350-372). */
 int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear_rgba8, void (*on_finish)(void*), void* user);
 int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out);
+/* Per work-bin data of the last frame, bins numbered view by view in the
+ * order views were first drawn to, row-major inside a view: entry counts and,
+ * with SRG_CFG_TILE_COSTS, the back end's measured cost (s_memtime ticks).
+ * Either array may be NULL; *nbins receives the bin count. No counterpart in
+ * the reference: its per-tile scope timing lives in microprofile
+ * (Rasterizer.cpp:527). */
+int srg_get_bin_stats(srg_ctx* ctx, uint32_t* counts, uint64_t* costs, uint32_t capacity, uint32_t* nbins);
 
 /* ---- host helper: TextureData::CreateFromRGBA8 (Texture.cpp:119-199) ----
  * Tiles a power-of-two RGBA8 image (>= 32 in both dimensions) into 32x32

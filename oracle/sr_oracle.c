@@ -641,7 +641,7 @@ static void bin_packet(sro_ctx* ctx, thread_state* ts, uint32_t packet)
         const float* ap[3];
         for (int k = 0; k < 3; ++k) ap[k] = (const float*)(attr + (size_t)idx[k] * d->attr_stride);
         if (mor == 0) {
-            bin_tri(ctx, ts, di, packet, v[0], v[1], v[2], ap[0], ap[1], ap[2], primbase);
+            bin_tri(ctx, ts, di, packet, v[0], v[1], v[2], ap[0], ap[1], ap[2], primbase | 7u);
             continue;
         }
         if (c0 & c1 & c2) continue;

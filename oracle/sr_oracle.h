@@ -61,8 +61,9 @@ typedef struct sro_draw {
  * fp32 depth, row-major inside a tile. All arrays are caller-owned and hold
  * tiles_x * tiles_y * 4096 elements. primid (may be NULL) receives, for the
  * pixels written during sro_render, the primitive order key of the final
- * fragment, ((first_tri_of_draw + tri) << 3) | fan_index; untouched pixels are
- * set to 0xFFFFFFFF. */
+ * fragment, ((first_tri_of_draw + tri) << 3) | fan, where fan is the clip-fan
+ * index (0..6) of a clipped triangle and 7 for an unclipped one; untouched
+ * pixels are set to 0xFFFFFFFF. */
 typedef struct sro_plane {
     uint32_t width, height, tiles_x, tiles_y;
     uint32_t* colour;

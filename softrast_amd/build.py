@@ -58,9 +58,17 @@ def build_scene():
     return out
 
 
+def build_copy_calib():
+    """Stand-alone copy kernel of known size (counter calibration for profiles)."""
+    out = PKG / "copy_calib"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-o", str(out), str(CSRC / "copy_calib.hip")])
+    return out
+
+
 def build_all():
     build_scene()
     build_gpu()
+    build_copy_calib()
 
 
 if __name__ == "__main__":

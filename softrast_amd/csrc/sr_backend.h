@@ -1,0 +1,364 @@
+// sr_backend.h -- back end: per-bin coverage, early depth and resolve,
+// then interpolation and shading of the visible fragment of each pixel.
+//
+// k_backend<S> is persistent: each workgroup (4 waves) takes S x S bins from
+// the frame's queue. A 64-bit visibility key per pixel lives in LDS. Waves
+// stride over the bin's entries 64 at a time, each lane loading one
+// primitive record so the loads overlap, then walk the entries one by one
+// with the record broadcast from its lane; an 8x8 block is evaluated with
+// one pixel per lane (RasterizeTrisInBin_OutputFragments and
+// ComputeBlockMask8x8, Rasterizer.cpp:97-304). Then each lane resolves its
+// pixels (the key names the fragment the reference would have shaded last)
+// and shades them (ComputeInterpolantsDrawCallImpl :306-419, Shaders.h,
+// Texture.cpp), writing RGBA8 and depth once.
+#pragma once
+#include "sr_frontend.h"
+
+namespace srg {
+
+constexpr uint32_t kBackendThreads = 256;
+constexpr uint32_t kBackendWaves = kBackendThreads / 64;
+constexpr uint32_t kSmallBlocks = 4;  // primitives touching at most this many 8x8 blocks of a bin go lane-private
+
+// The reference keeps a fragment iff z > 0 and z > D (ordered compare) and
+// shades every kept fragment in submission order, so the final pixel is the
+// earliest-submitted fragment of largest depth. Key = depth bits << 32 |
+// (0xFFFFFFFE - order); an atomic max over the keys selects it. Initial key
+// from the depth already in the plane: D <= 0 -> 0, D NaN -> all ones
+// (nothing passes), D > 0 -> D's bits above every order field.
+__device__ __forceinline__ unsigned long long init_key(float d0)
+{
+    if (d0 != d0) return ~0ull;
+    if (!(d0 > 0.0f)) return 0ull;
+    return ((unsigned long long)fbits(d0) << 32) | 0xFFFFFFFFull;
+}
+
+__device__ __forceinline__ uint32_t record_of_order(const FrameDev& F, uint32_t order)
+{
+    const uint32_t g = order >> 3, fan = order & 7u;
+    return fan == kFanUnclipped ? g : F.ntris + F.clip_base[g] + fan;
+}
+
+// ComputeInterpolants + pixel shader for the visible fragment of one pixel
+// (x, y tile-relative) (Rasterizer.cpp:356-400, Viewer/Shaders.h:71-130).
+__device__ __forceinline__ uint32_t shade_pixel(const FrameDev& F, uint32_t rec, int32_t X0, int32_t Y0, uint32_t px,
+                                                uint32_t py)
+{
+    const uint32_t d = F.rec_meta[rec];
+    const DrawDev& D = F.draws[d];
+    const float4* Sp = F.rec_shade + (size_t)rec * 4;
+    const float4 s0 = Sp[0], s1 = Sp[1], s2 = Sp[2], s3 = Sp[3];
+    const uint4 q2 = F.rec_raster[(size_t)rec * 4 + 2];
+    const uint4 q3 = F.rec_raster[(size_t)rec * 4 + 3];
+    // plane constants re-based to the tile origin (Binning.cpp:435-452)
+    const float sx = (float)X0 - ffrom(q2.w), sy = (float)Y0 - ffrom(q3.x);
+    const float wdx = s0.x, wdy = s0.y;
+    const float wc = wdx * sx + wdy * sy + s0.z;
+    const float pdx[kMaxSlots] = { s0.w, s1.z, s2.y, s3.x };
+    const float pdy[kMaxSlots] = { s1.x, s1.w, s2.z, s3.y };
+    const float pf0[kMaxSlots] = { s1.y, s2.x, s2.w, s3.z };
+    const float X = (float)px, Y = (float)py;
+    const float iq = 1.0f / __builtin_fmaf(X, wdx, __builtin_fmaf(Y, wdy, wc));
+    float pc[kMaxSlots], var[kMaxSlots];
+    const uint32_t ns = D.nslots;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxSlots; ++k) {
+        pc[k] = pdx[k] * sx + pdy[k] * sy + pf0[k];
+        var[k] = k < ns ? iq * __builtin_fmaf(pdy[k], Y, __builtin_fmaf(pdx[k], X, pc[k])) : 0.0f;
+    }
+    switch (D.shader) {
+    case SRG_SHADER_VISUALIZE_NORMALS:
+        return pack_rgba(__builtin_fmaf(var[0], 0.5f, 0.5f), __builtin_fmaf(var[1], 0.5f, 0.5f),
+                         __builtin_fmaf(var[2], 0.5f, 0.5f), 1.0f);
+    case SRG_SHADER_VISUALIZE_UVS:
+        return pack_rgba(var[0], var[1], 0.0f, 0.0f);
+    default: break;
+    }
+    if (D.tex == kNoTex) return 0xFFFFFFFFu;
+    const TexDev& T = F.texs[D.tex];
+    if (T.empty) return 0xFFFFFFFFu;
+    // neighbour 1/w by RCPPS, forward differences (Rasterizer.cpp:373-400)
+    const float X1 = 1.0f + X, Y1 = 1.0f + Y;
+    const float i10 = rcpps(__builtin_fmaf(wdx, X1, __builtin_fmaf(wdy, Y, wc)));
+    const float i01 = rcpps(__builtin_fmaf(wdx, X, __builtin_fmaf(wdy, Y1, wc)));
+    float der[4];
+    const bool hi = D.deriv_slot != 0;
+#pragma unroll
+    for (uint32_t uv = 0; uv < 2; ++uv) {
+        const float tdx = hi ? pdx[2 + uv] : pdx[uv], tdy = hi ? pdy[2 + uv] : pdy[uv];
+        const float tc = hi ? pc[2 + uv] : pc[uv], tv = hi ? var[2 + uv] : var[uv];
+        const float t10 = i10 * __builtin_fmaf(tdx, X1, __builtin_fmaf(tdy, Y, tc));
+        const float t01 = i01 * __builtin_fmaf(tdx, X, __builtin_fmaf(tdy, Y1, tc));
+        der[2 * uv] = t10 - tv;
+        der[2 * uv + 1] = t01 - tv;
+    }
+    return sample_unlit(T, var[0], var[1], der[0], der[1], der[2], der[3]);
+}
+
+__device__ __forceinline__ uint32_t view_of_bin(const FrameDev& F, uint32_t gb)
+{
+    uint32_t lo = 0, hi = F.nviews;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (F.views[mid].bin_base <= gb) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane)
+{
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
+}
+
+// 64-wide block reject / full-accept test of RasterizeTrisInBin_OutputFragments
+// (Rasterizer.cpp:230-272): corners {xb, xb+64} x {yb, yb+64} of the block,
+// tile-relative. Returns 0 = reject, 1 = test pixels, 2 = every pixel covered.
+__device__ __forceinline__ uint32_t block_class(const uint32_t ct[3], const uint32_t dxu[3], const uint32_t dyu[3],
+                                                uint32_t xb, uint32_t yb)
+{
+    const uint32_t Xa = xb, Xb = xb + kBin, Ya = yb, Yb = yb + kBin;
+    bool reject = false, full = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t m = ((int32_t)(ct[k] + dyu[k] * Xa + dxu[k] * Ya) > 0 ? 1u : 0u) |
+                           ((int32_t)(ct[k] + dyu[k] * Xa + dxu[k] * Yb) > 0 ? 2u : 0u) |
+                           ((int32_t)(ct[k] + dyu[k] * Xb + dxu[k] * Ya) > 0 ? 4u : 0u) |
+                           ((int32_t)(ct[k] + dyu[k] * Xb + dxu[k] * Yb) > 0 ? 8u : 0u);
+        reject = reject || m == 0;
+        full = full && m == 0xFu;
+    }
+    return reject ? 0u : (full ? 2u : 1u);
+}
+
+// One primitive's tile-relative setup (Binning.cpp:423-451 re-basing) and its
+// block range inside the work bin at (sx0, sy0).
+struct EntryRaster {
+    uint32_t ct[3], dxu[3], dyu[3];
+    uint32_t xs, xe, ys, ye;
+    float zdx, zdy, zc0;
+    unsigned long long low;
+};
+
+__device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2, uint4 q3, uint32_t order, int32_t X0,
+                                                    int32_t Y0, uint32_t sx0, uint32_t sy0, uint32_t S)
+{
+    EntryRaster e;
+    e.dxu[0] = q0.w; e.dxu[1] = q1.x; e.dxu[2] = q1.y;
+    e.dyu[0] = q1.z; e.dyu[1] = q1.w; e.dyu[2] = q2.x;
+    const uint32_t c[3] = { q0.x, q0.y, q0.z };
+#pragma unroll
+    for (int k = 0; k < 3; ++k) e.ct[k] = c[k] + (e.dxu[k] * (uint32_t)Y0 + e.dyu[k] * (uint32_t)X0);
+    const int32_t xmin = q2.y & 0xFFFF, xmax = q2.y >> 16, ymin = q2.z & 0xFFFF, ymax = q2.z >> 16;
+    const uint32_t bminx = (uint32_t)clampi(xmin - X0, 0, 64), bmaxx = (uint32_t)clampi(xmax - X0, 0, 64);
+    const uint32_t bminy = (uint32_t)clampi(ymin - Y0, 0, 64), bmaxy = (uint32_t)clampi(ymax - Y0, 0, 64);
+    e.xs = max(bminx & ~7u, sx0);
+    e.xe = min(bmaxx, sx0 + S);
+    e.ys = max(bminy & ~7u, sy0);
+    e.ye = min(bmaxy, sy0 + S);
+    const float r0x = ffrom(q2.w), r0y = ffrom(q3.x);
+    e.zdx = ffrom(q3.y);
+    e.zdy = ffrom(q3.z);
+    const float sxf = (float)X0 - r0x, syf = (float)Y0 - r0y;
+    e.zc0 = e.zdx * sxf + e.zdy * syf + ffrom(q3.w);
+    e.low = 0xFFFFFFFEull - order;
+    return e;
+}
+
+// Lane-private rasterisation of one small primitive: its blocks one after
+// another, 8 rows x 8 columns per block with the reference's arithmetic
+// (ComputeBlockMask8x8, Rasterizer.cpp:134-192): edge values stepped by dx per
+// row, depth per column as fma(yb, dy, fma(l, dx, c0)) + xb*dx then += dy per row.
+template <uint32_t S>
+__device__ __forceinline__ void raster_small(unsigned long long* keys, const EntryRaster& e, uint32_t sx0, uint32_t sy0,
+                                             unsigned long long& covered_count)
+{
+    float ztop[8];
+#pragma unroll
+    for (uint32_t l = 0; l < 8; ++l) ztop[l] = __builtin_fmaf((float)l, e.zdx, e.zc0);
+    for (uint32_t yb = e.ys; yb < e.ye; yb += 8)
+        for (uint32_t xb = e.xs; xb < e.xe; xb += 8) {
+            const uint32_t cls = block_class(e.ct, e.dxu, e.dyu, xb, yb);
+            if (cls == 0u) continue;
+            const bool full = cls == 2u;
+            uint32_t er[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) er[k] = e.ct[k] + e.dyu[k] * xb + e.dxu[k] * yb;
+            float z[8];
+            const float xoff = (float)xb * e.zdx;
+#pragma unroll
+            for (uint32_t l = 0; l < 8; ++l) z[l] = __builtin_fmaf((float)yb, e.zdy, ztop[l]) + xoff;
+            unsigned long long* row = keys + (yb - sy0) * S + (xb - sx0);
+#pragma unroll
+            for (uint32_t r = 0; r < 8; ++r) {
+#pragma unroll
+                for (uint32_t l = 0; l < 8; ++l) {
+                    const bool cov = full || (int32_t)((er[0] + e.dyu[0] * l) | (er[1] + e.dyu[1] * l) |
+                                                       (er[2] + e.dyu[2] * l)) >= 0;
+                    covered_count += cov ? 1u : 0u;
+                    if (cov && z[l] > 0.0f) atomicMax(&row[l], ((unsigned long long)fbits(z[l]) << 32) | e.low);
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) er[k] += e.dxu[k];
+#pragma unroll
+                for (uint32_t l = 0; l < 8; ++l) z[l] = z[l] + e.zdy;
+                row += S;
+            }
+        }
+}
+
+template <uint32_t S>
+__global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
+{
+    constexpr uint32_t K = kBin / S;
+    constexpr uint32_t NPIX = S * S;
+    constexpr uint32_t PPT = NPIX / kBackendThreads;
+    constexpr uint32_t BATCH = PPT < 4 ? PPT : 4;
+    static_assert(NPIX % kBackendThreads == 0, "bin must hold a whole number of pixels per thread");
+    __shared__ unsigned long long keys[NPIX];
+    __shared__ uint32_t s_q;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u;
+    const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
+    const uint32_t l = lane & 7u, rr = lane >> 3;
+    unsigned long long frags = 0;
+    if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
+    const uint32_t qlen = F.counters[C_QLEN];
+
+    for (;;) {
+        if (tid == 0) s_q = atomicAdd(&F.counters[C_QHEAD], 1u);
+        __syncthreads();
+        const uint32_t qi = s_q;
+        __syncthreads();
+        if (qi >= qlen) break;
+        const uint64_t t_start = (F.flags & FF_TILE_COSTS) ? clock64() : 0;
+        const uint32_t gb = F.queue[qi];
+        const ViewDev V = F.views[view_of_bin(F, gb)];
+        const uint32_t lb = gb - V.bin_base;
+        const uint32_t sbx = lb % V.bins_x, sby = lb / V.bins_x;
+        const uint32_t tx = sbx / K, ty = sby / K;
+        const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S;
+        const int32_t X0 = (int32_t)(tx * kBin), Y0 = (int32_t)(ty * kBin);
+        const size_t tofs = (size_t)(ty * V.tiles_x + tx) * kTilePixels;
+        const bool clear_d = (V.flags & VF_CLEAR_DEPTH) != 0;
+        const bool clear_c = (V.flags & VF_CLEAR_COLOUR) != 0;
+
+        for (uint32_t p = tid; p < NPIX; p += kBackendThreads) {
+            const uint32_t x = sx0 + p % S, y = sy0 + p / S;
+            keys[p] = clear_d ? 0ull : init_key(V.depth[tofs + y * kBin + x]);
+        }
+        __syncthreads();
+
+        const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
+        const bool stats = (F.flags & FF_STATS) != 0;
+        for (uint32_t b0 = 0; b0 + wave < cnt; b0 += kBackendThreads) {
+            // lane j of wave w holds entry b0 + w + 4j
+            const uint32_t my = b0 + wave + kBackendWaves * lane;
+            const bool has = my < cnt;
+            uint4 q0 = make_uint4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
+            uint32_t order = 0;
+            if (has) {
+                const uint32_t r = F.entries[beg + my];
+                const uint4* R = F.rec_raster + (size_t)r * 4;
+                q0 = R[0]; q1 = R[1]; q2 = R[2]; q3 = R[3];
+                order = r < F.ntris ? ((r << 3) | kFanUnclipped) : F.clip_order[r - F.ntris];
+            }
+            const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S);
+            const uint32_t nbx = e.xe > e.xs ? (e.xe - e.xs + 7u) >> 3 : 0u;
+            const uint32_t nby = e.ye > e.ys ? (e.ye - e.ys + 7u) >> 3 : 0u;
+            const bool small = has && nbx * nby <= kSmallBlocks;
+            // small primitives: one per lane
+            if (small) {
+                if (stats) raster_small<S>(keys, e, sx0, sy0, frags);
+                else {
+                    unsigned long long unused = 0;
+                    raster_small<S>(keys, e, sx0, sy0, unused);
+                }
+            }
+            // large primitives: the whole wave on one 8x8 block at a time, one pixel per lane
+            unsigned long long lm = __ballot(has && !small);
+            while (lm) {
+                const uint32_t j = (uint32_t)(__ffsll((long long)lm) - 1);
+                lm &= lm - 1;
+                uint32_t ct[3], dxu[3], dyu[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    ct[k] = bcast(e.ct[k], j);
+                    dxu[k] = bcast(e.dxu[k], j);
+                    dyu[k] = bcast(e.dyu[k], j);
+                }
+                const uint32_t xs = bcast(e.xs, j), xe = bcast(e.xe, j), ys = bcast(e.ys, j), ye = bcast(e.ye, j);
+                const float zdx = ffrom(bcast(fbits(e.zdx), j)), zdy = ffrom(bcast(fbits(e.zdy), j));
+                const float zc0 = ffrom(bcast(fbits(e.zc0), j));
+                const unsigned long long low = 0xFFFFFFFEull - bcast(0xFFFFFFFEu - (uint32_t)e.low, j);
+                const float ztop = __builtin_fmaf((float)l, zdx, zc0);
+                for (uint32_t yb = ys; yb < ye; yb += 8)
+                    for (uint32_t xb = xs; xb < xe; xb += 8) {
+                        const uint32_t cls = block_class(ct, dxu, dyu, xb, yb);
+                        if (cls == 0u) continue;
+                        const uint32_t x = xb + l, y = yb + rr;
+                        bool covered = cls == 2u;
+                        if (!covered) {
+                            const uint32_t e0 = ct[0] + dyu[0] * x + dxu[0] * y;
+                            const uint32_t e1 = ct[1] + dyu[1] * x + dxu[1] * y;
+                            const uint32_t e2 = ct[2] + dyu[2] * x + dxu[2] * y;
+                            covered = (int32_t)(e0 | e1 | e2) >= 0;
+                        }
+                        // depth: row 0 of the block, then += dy per row (ComputeBlockMask8x8)
+                        float z = __builtin_fmaf((float)yb, zdy, ztop) + (float)xb * zdx;
+#pragma unroll
+                        for (uint32_t i = 0; i < 7; ++i)
+                            if (i < rr) z = z + zdy;
+                        if (stats) frags += covered ? 1u : 0u;
+                        if (covered && z > 0.0f)
+                            atomicMax(&keys[(y - sy0) * S + (x - sx0)], ((unsigned long long)fbits(z) << 32) | low);
+                    }
+            }
+        }
+        __syncthreads();
+
+        for (uint32_t p0 = 0; p0 < PPT; p0 += BATCH) {
+            uint32_t rec[BATCH], col[BATCH];
+            unsigned long long kk[BATCH];
+            bool win[BATCH];
+#pragma unroll
+            for (uint32_t i = 0; i < BATCH; ++i) {
+                const uint32_t p = tid + (p0 + i) * kBackendThreads;
+                kk[i] = keys[p];
+                const uint32_t lo = (uint32_t)kk[i];
+                win[i] = lo != 0xFFFFFFFFu && kk[i] != 0ull;
+                rec[i] = win[i] ? record_of_order(F, 0xFFFFFFFEu - lo) : 0u;
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < BATCH; ++i) {
+                const uint32_t p = tid + (p0 + i) * kBackendThreads;
+                col[i] = (win[i] && (V.flags & VF_HAS_COLOUR)) ? shade_pixel(F, rec[i], X0, Y0, sx0 + p % S, sy0 + p / S)
+                                                               : V.clear_colour;
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < BATCH; ++i) {
+                const uint32_t p = tid + (p0 + i) * kBackendThreads;
+                const size_t o = tofs + (sy0 + p / S) * kBin + sx0 + p % S;
+                if (win[i]) {
+                    if (V.flags & VF_HAS_COLOUR) V.colour[o] = col[i];
+                    V.depth[o] = ffrom((uint32_t)(kk[i] >> 32));
+                    if (V.primid) V.primid[o] = 0xFFFFFFFEu - (uint32_t)kk[i];
+                } else {
+                    if (clear_c) V.colour[o] = col[i];
+                    if (clear_d) V.depth[o] = 0.0f;
+                }
+            }
+        }
+        if ((F.flags & FF_TILE_COSTS) && tid == 0) F.bin_cost[gb] = clock64() - t_start;
+        __syncthreads();
+    }
+    if (F.flags & FF_STATS) {
+        // per-lane counts -> wave total -> one atomic per wave
+        uint32_t lo = (uint32_t)frags, hi = (uint32_t)(frags >> 32);
+        lo = wave_sum(lo);
+        hi = wave_sum(hi);
+        const unsigned long long total = ((unsigned long long)hi << 32) + lo;
+        if (lane == 0 && total) atomicAdd((unsigned long long*)&F.counters[C_FRAGS], total);
+    }
+}
+
+}  // namespace srg

@@ -1,0 +1,156 @@
+// sr_frame.h -- per-frame device tables shared by the front-end and back-end
+// kernels, and the wave-level helpers they use.
+#pragma once
+#include "sr_device.h"
+
+namespace srg {
+
+// One entry of the frame's draw table: a DrawCall (Renderer.h:119-150)
+// resolved to device pointers, plus the attribute slots its shader reads.
+struct DrawDev {
+    const uint8_t* idx;
+    const uint8_t* pos;
+    const uint8_t* attr;
+    uint32_t idx_stride, pos_stride, attr_stride, ntris;
+    uint32_t pos_num, attr_num, tri_base, view;
+    uint32_t shader, tex, nslots, deriv_slot;  // deriv_slot: first slot of (u, v) at uv_offset
+    uint32_t slot_attr[kMaxSlots];
+    float mvp[16];
+};
+
+enum : uint32_t {
+    VF_CLEAR_COLOUR = 1u,
+    VF_CLEAR_DEPTH = 2u,
+    VF_HAS_COLOUR = 4u,
+};
+
+// A framebuffer plane targeted this frame. Work items ("bins") are
+// S x S squares, S = 64 / bin_k, inside the reference's 64x64 tiles.
+struct ViewDev {
+    uint32_t* colour;
+    float* depth;
+    uint32_t* primid;
+    uint32_t width, height, tiles_x, tiles_y;
+    uint32_t bins_x, bins_y, bin_base, flags;
+    uint32_t clear_colour, pad0, pad1, pad2;
+};
+
+enum : uint32_t {
+    C_CLIPQ = 0,       // clipped triangles queued for the slow path
+    C_CLIPREC = 1,     // records allocated by the slow path
+    C_FLAGS = 2,       // error / overflow bits
+    C_QLEN = 3,        // back-end queue length
+    C_QHEAD = 4,       // back-end queue head
+    C_ENTRIES = 5,     // bin entries
+    C_TILE_ENTRIES = 6,// 64x64-tile entries as the reference counts them (stats)
+    C_PRIMS = 7,       // primitives that survived the cull (stats)
+    C_FRAGS = 8,       // 64-bit rasterised fragment count (words 8, 9)
+    C_NCOUNTERS = 16,
+};
+
+enum : uint32_t {
+    FLAG_CLIP_OVERFLOW = 1u,
+    FLAG_ENTRY_OVERFLOW = 2u,
+    FLAG_INDEX_RANGE = 4u,
+};
+
+enum : uint32_t { FF_STATS = 1u, FF_PRIMID = 2u, FF_TILE_COSTS = 4u };
+
+constexpr uint32_t kFanUnclipped = 7u;  // fan field of an unclipped triangle's order key
+
+// Primitive records. Index space: [0, ntris) one slot per submitted triangle
+// (its unclipped primitive), [ntris, ntris + clip_cap) fan triangles of
+// clipped triangles. Order key of a primitive: (global triangle << 3) | fan,
+// fan = 7 for an unclipped triangle, 0..6 for fan triangles.
+struct FrameDev {
+    const DrawDev* draws;
+    const ViewDev* views;
+    const TexDev* texs;
+    const uint32_t* block_draw;
+    const uint32_t* block_tri0;
+    uint32_t ndraws, nviews, total_bins, ntris;
+    uint32_t bin_k, bin_size, clip_cap, entry_cap;
+    uint4* rec_raster;     // 4 x uint4 per record: edges, bbox, r0, z plane
+    float4* rec_shade;     // 4 x float4 per record: 1/w plane and attribute planes
+    uint32_t* rec_meta;    // draw index; 0xFFFFFFFF = no primitive
+    uint32_t* clip_queue;  // global triangle ids routed to the clipper
+    uint32_t* clip_base;   // per global triangle: first fan record (relative to ntris)
+    uint32_t* clip_order;  // per fan record: order key
+    uint32_t* bin_count;
+    uint32_t* bin_start;   // total_bins + 1
+    uint32_t* bin_cursor;
+    uint32_t* entries;
+    uint32_t* queue;
+    uint64_t* bin_cost;    // optional per bin (s_memtime ticks)
+    uint32_t* counters;
+    uint32_t flags;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long lanemask_lt()
+{
+    const uint32_t l = lane_id();
+    return l ? (~0ull >> (64u - l)) : 0ull;
+}
+
+// Wave-aggregated counter increment: every active lane adds 1 to
+// counters[key] and receives its slot; lanes with equal keys share one
+// atomic. Must be called by all lanes of the wave (uniform control flow).
+__device__ __forceinline__ uint32_t wave_add_slot(uint32_t* counters, uint32_t key, bool active)
+{
+    unsigned long long pending = __ballot(active);
+    uint32_t slot = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader, 64);
+        const unsigned long long same = __ballot(active && key == lkey) & pending;
+        uint32_t base = 0;
+        if (lane_id() == (uint32_t)leader) base = atomicAdd(&counters[lkey], (uint32_t)__popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane_id()) & 1ull) slot = base + (uint32_t)__popcll(same & lanemask_lt());
+        pending &= ~same;
+    }
+    return slot;
+}
+
+// Wave-aggregated increment without slots: lanes with equal keys share one
+// non-returning atomic add. Uniform control flow required.
+__device__ __forceinline__ void wave_count(uint32_t* counters, uint32_t key, bool active)
+{
+    unsigned long long pending = __ballot(active);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader, 64);
+        const unsigned long long same = __ballot(active && key == lkey) & pending;
+        if (lane_id() == (uint32_t)leader) atomicAdd(&counters[lkey], (uint32_t)__popcll(same));
+        pending &= ~same;
+    }
+}
+
+// Sum over the wave (result in every lane).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Wave-aggregated add of per-lane amounts to one counter; returns each lane's
+// exclusive offset. Uniform control flow required.
+__device__ __forceinline__ uint32_t wave_add_amount(uint32_t* counter, uint32_t amount)
+{
+    uint32_t x = amount;
+    const uint32_t l = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (l >= (uint32_t)o) x += y;
+    }
+    const uint32_t total = __shfl(x, 63, 64);
+    uint32_t base = 0;
+    if (l == 0 && total) base = atomicAdd(counter, total);
+    return __shfl(base, 0, 64) + x - amount;
+}
+
+}  // namespace srg

@@ -1,0 +1,532 @@
+// sr_frontend.h -- front end: fetch, transform, clip, triangle setup and
+// binning (SoftRast/Binning.cpp), as four kernels:
+//   k_setup  one lane per submitted triangle; unclipped survivors are set up
+//            and written to their own record slot, clipped triangles are
+//            queued for k_clip; bin entries are counted
+//   k_clip   slow path over the queued triangles: Sutherland-Hodgman clip,
+//            fan, setup of each fan triangle, bin counts
+//   k_scan   exclusive scan of bin counts + the back end's work queue
+//   k_emit   one lane per record: writes the record index into each bin list
+#pragma once
+#include "sr_frame.h"
+
+namespace srg {
+
+struct Prim {
+    int32_t c[3], dx[3], dy[3];
+    int32_t xmin, xmax, ymin, ymax;
+    float r0x, r0y, zdx, zdy, zf0, wdx, wdy, wf0;
+    float adx[kMaxSlots], ady[kMaxSlots], af0[kMaxSlots];
+};
+
+// kt::Mat4 * kt::Vec4 (Binning.cpp:477-479), order ((c0*x + c1*y) + c2*z) + c3*w,
+// w = 1, every operation rounded.
+__device__ __forceinline__ float4 transform(const float* m, float x, float y, float z)
+{
+    const float w = 1.0f;
+    float4 r;
+    r.x = ((m[0] * x + m[4] * y) + m[8] * z) + m[12] * w;
+    r.y = ((m[1] * x + m[5] * y) + m[9] * z) + m[13] * w;
+    r.z = ((m[2] * x + m[6] * y) + m[10] * z) + m[14] * w;
+    r.w = ((m[3] * x + m[7] * y) + m[11] * z) + m[15] * w;
+    return r;
+}
+
+// ComputeClipMask (Binning.cpp:56-68).
+__device__ __forceinline__ uint32_t clip_codes(float4 v)
+{
+    uint32_t m = 0;
+    m |= (v.x + v.w < 0.0f) ? 0x01u : 0u;
+    m |= (v.x - v.w > 0.0f) ? 0x02u : 0u;
+    m |= (v.y + v.w < 0.0f) ? 0x04u : 0u;
+    m |= (v.y - v.w > 0.0f) ? 0x08u : 0u;
+    m |= (v.z < 0.0f) ? 0x10u : 0u;
+    m |= (v.z - v.w > 0.0f) ? 0x20u : 0u;
+    return m;
+}
+
+// SetupEdge (Binning.cpp:242-259): fill-rule bias, >> 8 truncated to 32 bits.
+__device__ __forceinline__ void setup_edge(Prim& s, int k, int32_t ax, int32_t ay, int32_t bx, int32_t by)
+{
+    const int32_t dy = (int32_t)((uint32_t)by - (uint32_t)ay);
+    const int32_t dx = (int32_t)((uint32_t)ax - (uint32_t)bx);
+    int64_t c = (int64_t)ay * (int64_t)(int32_t)((uint32_t)bx - (uint32_t)ax) -
+                (int64_t)ax * (int64_t)(int32_t)((uint32_t)by - (uint32_t)ay);
+    if (dy < 0 || (dy == 0 && dx > 0)) c += 256;
+    s.dy[k] = dy;
+    s.dx[k] = dx;
+    s.c[k] = (int32_t)(uint32_t)(uint64_t)(c >> 8);
+}
+
+// SetupPlane (Binning.cpp:261-277).
+__device__ __forceinline__ void setup_plane(float cterm, float d10x, float d10y, float d20x, float d20y, float f10,
+                                            float f20, float& odx, float& ody)
+{
+    const float A = d10y * f20 - f10 * d20y;
+    const float B = d20x * f10 - d10x * f20;
+    odx = -A / cterm;
+    ody = -B / cterm;
+}
+
+__device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Viewport, snap, cull, bbox, edges and planes (Binning.cpp:291-350) for the
+// kept attribute slots. Returns false if the triangle is culled.
+__device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const float a[3][kMaxSlots], uint32_t nslots,
+                                           uint32_t width, uint32_t height)
+{
+    const float hw = (float)width * 0.5f, hh = (float)height * 0.5f;
+    const float iw0 = 1.0f / v[0].w, iw1 = 1.0f / v[1].w, iw2 = 1.0f / v[2].w;
+    const float r0x = iw0 * v[0].x * hw + hw, r0y = iw0 * v[0].y * -hh + hh;
+    const float r1x = iw1 * v[1].x * hw + hw, r1y = iw1 * v[1].y * -hh + hh;
+    const float r2x = iw2 * v[2].x * hw + hw, r2y = iw2 * v[2].y * -hh + hh;
+    const int32_t f0x = cvtt_i32(r0x * 256.0f + 0.5f), f0y = cvtt_i32(r0y * 256.0f + 0.5f);
+    const int32_t f1x = cvtt_i32(r1x * 256.0f + 0.5f), f1y = cvtt_i32(r1y * 256.0f + 0.5f);
+    const int32_t f2x = cvtt_i32(r2x * 256.0f + 0.5f), f2y = cvtt_i32(r2y * 256.0f + 0.5f);
+
+    int64_t area = (int64_t)(int32_t)((uint32_t)f2x - (uint32_t)f0x) * (int64_t)(int32_t)((uint32_t)f1y - (uint32_t)f0y) -
+                   (int64_t)(int32_t)((uint32_t)f2y - (uint32_t)f0y) * (int64_t)(int32_t)((uint32_t)f1x - (uint32_t)f0x);
+    area >>= 8;
+    if (area <= 0) return false;
+
+    const int32_t mnx = min(min(f0x, f1x), f2x), mny = min(min(f0y, f1y), f2y);
+    const int32_t mxx = max(max(f0x, f1x), f2x), mxy = max(max(f0y, f1y), f2y);
+    s.xmin = (uint16_t)clampi((int32_t)((uint32_t)mnx + 255u) >> 8, 0, (int32_t)width - 1);
+    s.ymin = (uint16_t)clampi((int32_t)((uint32_t)mny + 255u) >> 8, 0, (int32_t)height - 1);
+    s.xmax = (uint16_t)clampi((int32_t)((uint32_t)mxx + 255u) >> 8, 0, (int32_t)width - 1);
+    s.ymax = (uint16_t)clampi((int32_t)((uint32_t)mxy + 255u) >> 8, 0, (int32_t)height - 1);
+
+    setup_edge(s, 0, f0x, f0y, f1x, f1y);
+    setup_edge(s, 1, f1x, f1y, f2x, f2y);
+    setup_edge(s, 2, f2x, f2y, f0x, f0y);
+
+    const float d10x = r1x - r0x, d10y = r1y - r0y, d20x = r2x - r0x, d20y = r2y - r0y;
+    const float cterm = d10x * d20y - d10y * d20x;
+    setup_plane(cterm, d10x, d10y, d20x, d20y, v[1].z * iw1 - v[0].z * iw0, v[2].z * iw2 - v[0].z * iw0, s.zdx, s.zdy);
+    setup_plane(cterm, d10x, d10y, d20x, d20y, iw1 - iw0, iw2 - iw0, s.wdx, s.wdy);
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxSlots; ++k)
+        if (k < nslots)
+            setup_plane(cterm, d10x, d10y, d20x, d20y, a[1][k] * iw1 - a[0][k] * iw0, a[2][k] * iw2 - a[0][k] * iw0,
+                        s.adx[k], s.ady[k]);
+    s.r0x = r0x;
+    s.r0y = r0y;
+    s.zf0 = v[0].z * iw0;
+    s.wf0 = iw0;
+#pragma unroll
+    for (uint32_t k = 0; k < kMaxSlots; ++k) s.af0[k] = k < nslots ? a[0][k] * iw0 : 0.0f;
+    return true;
+}
+
+// 64x64 bin corner test (Binning.cpp:376-409): false if for some edge no
+// corner of the bin is strictly inside.
+__device__ __forceinline__ bool tile_accepts(const int32_t c[3], const int32_t dx[3], const int32_t dy[3], uint32_t X0,
+                                             uint32_t Y0)
+{
+    const uint32_t X1 = X0 + kBin, Y1 = Y0 + kBin;
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t cc = (uint32_t)c[k], ex = (uint32_t)dx[k], ey = (uint32_t)dy[k];
+        const bool any = ((int32_t)(cc + ey * X0 + ex * Y0) > 0) | ((int32_t)(cc + ey * X0 + ex * Y1) > 0) |
+                         ((int32_t)(cc + ey * X1 + ex * Y0) > 0) | ((int32_t)(cc + ey * X1 + ex * Y1) > 0);
+        ok = ok && any;
+    }
+    return ok;
+}
+
+// The bins a primitive lands in. The reference appends the triangle to every
+// 64x64 bin of its bbox that passes the corner test -- skipped when the bbox
+// spans at most two bin rows, because numXbins is computed from Y
+// (Binning.cpp:352-370) -- and later walks 8x8 blocks from (bMin & ~7) up to
+// (exclusively) bMax inside each bin (Rasterizer.cpp:203-229). Globally those
+// blocks start at xmin & ~7 and end before xmax. A work bin of size S keeps
+// the primitive iff one of those blocks lies in it and its 64x64 tile passed.
+struct BinWalk {
+    uint32_t sbx0, sby0, nx, ncand, k, view_bins_x, bin_base;
+    bool check;
+};
+
+__device__ __forceinline__ BinWalk bin_walk(const ViewDev& V, uint32_t bin_k, uint32_t bin_size, int32_t xmin,
+                                            int32_t xmax, int32_t ymin, int32_t ymax)
+{
+    BinWalk w;
+    const uint32_t xlo = (uint32_t)xmin & ~7u, ylo = (uint32_t)ymin & ~7u;
+    w.k = bin_k;
+    w.view_bins_x = V.bins_x;
+    w.bin_base = V.bin_base;
+    w.check = !((((uint32_t)ymax >> 6) - ((uint32_t)ymin >> 6) + 1u) <= 2u);
+    if ((uint32_t)xmax <= xlo || (uint32_t)ymax <= ylo) {
+        w.sbx0 = w.sby0 = w.nx = w.ncand = 0;
+        return w;
+    }
+    w.sbx0 = xlo / bin_size;
+    w.sby0 = ylo / bin_size;
+    w.nx = ((uint32_t)xmax - 1u) / bin_size - w.sbx0 + 1u;
+    const uint32_t ny = ((uint32_t)ymax - 1u) / bin_size - w.sby0 + 1u;
+    w.ncand = w.nx * ny;
+    return w;
+}
+
+// Candidate i of the walk: returns the global bin index, or ~0 if its tile
+// fails the corner test.
+__device__ __forceinline__ uint32_t bin_candidate(const BinWalk& w, uint32_t i, const int32_t c[3], const int32_t dx[3],
+                                                  const int32_t dy[3])
+{
+    const uint32_t sbx = w.sbx0 + i % w.nx, sby = w.sby0 + i / w.nx;
+    const uint32_t tx = sbx / w.k, ty = sby / w.k;
+    if (w.check && !tile_accepts(c, dx, dy, tx * kBin, ty * kBin)) return 0xFFFFFFFFu;
+    return w.bin_base + sby * w.view_bins_x + sbx;
+}
+
+// Number of 64x64 tiles the reference bins the primitive into (statistics).
+__device__ __forceinline__ uint32_t reference_tile_entries(const Prim& s)
+{
+    const uint32_t bymin = (uint32_t)s.ymin >> 6, bymax = (uint32_t)s.ymax >> 6;
+    const uint32_t bxmin = (uint32_t)s.xmin >> 6, bxmax = (uint32_t)s.xmax >> 6;
+    const bool check = !(bymax - bymin + 1u <= 2u);
+    uint32_t n = 0;
+    for (uint32_t by = bymin; by <= bymax; ++by)
+        for (uint32_t bx = bxmin; bx <= bxmax; ++bx) n += (!check || tile_accepts(s.c, s.dx, s.dy, bx * kBin, by * kBin)) ? 1u : 0u;
+    return n;
+}
+
+// Counts (emit = false) or appends (emit = true) a primitive's bin entries,
+// wave-aggregated. Uniform control flow required; `active` lanes carry a
+// primitive described by (c, dx, dy, bbox) in view V.
+__device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, bool active, const int32_t c[3],
+                                          const int32_t dx[3], const int32_t dy[3], int32_t xmin, int32_t xmax,
+                                          int32_t ymin, int32_t ymax, bool emit, uint32_t rec)
+{
+    BinWalk w;
+    if (active) w = bin_walk(V, F.bin_k, F.bin_size, xmin, xmax, ymin, ymax);
+    else w.ncand = 0;
+    for (uint32_t i = 0; __ballot(i < w.ncand) != 0ull; ++i) {
+        uint32_t key = 0xFFFFFFFFu;
+        if (i < w.ncand) key = bin_candidate(w, i, c, dx, dy);
+        const bool act = key != 0xFFFFFFFFu;
+        if (!emit) {
+            wave_count(F.bin_count, act ? key : 0u, act);
+        } else {
+            const uint32_t slot = wave_add_slot(F.bin_cursor, act ? key : 0u, act);
+            if (act) F.entries[F.bin_start[key] + slot] = rec;
+        }
+    }
+}
+
+__device__ __forceinline__ void write_record(const FrameDev& F, uint32_t r, const Prim& s, uint32_t draw)
+{
+    uint4* R = F.rec_raster + (size_t)r * 4;
+    R[0] = make_uint4((uint32_t)s.c[0], (uint32_t)s.c[1], (uint32_t)s.c[2], (uint32_t)s.dx[0]);
+    R[1] = make_uint4((uint32_t)s.dx[1], (uint32_t)s.dx[2], (uint32_t)s.dy[0], (uint32_t)s.dy[1]);
+    R[2] = make_uint4((uint32_t)s.dy[2], (uint32_t)s.xmin | ((uint32_t)s.xmax << 16),
+                      (uint32_t)s.ymin | ((uint32_t)s.ymax << 16), fbits(s.r0x));
+    R[3] = make_uint4(fbits(s.r0y), fbits(s.zdx), fbits(s.zdy), fbits(s.zf0));
+    float4* S = F.rec_shade + (size_t)r * 4;
+    S[0] = make_float4(s.wdx, s.wdy, s.wf0, s.adx[0]);
+    S[1] = make_float4(s.ady[0], s.af0[0], s.adx[1], s.ady[1]);
+    S[2] = make_float4(s.af0[1], s.adx[2], s.ady[2], s.af0[2]);
+    S[3] = make_float4(s.adx[3], s.ady[3], s.af0[3], 0.0f);
+    F.rec_meta[r] = draw;
+}
+
+// FetchIndices / FetchPositions / attribute fetch + MVP (Binning.cpp:167-221, 474-479).
+// Returns false (and flags the frame) if an index is out of range.
+__device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev& D, uint32_t tri, float4 v[3],
+                                               float a[3][kMaxSlots])
+{
+    uint32_t idx[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const size_t i = (size_t)tri * 3u + (size_t)k;
+        idx[k] = D.idx_stride == 2 ? (uint32_t)((const uint16_t*)D.idx)[i]
+               : D.idx_stride == 4 ? ((const uint32_t*)D.idx)[i]
+                                   : (uint32_t)D.idx[i];
+    }
+    const bool ok = idx[0] < D.pos_num && idx[1] < D.pos_num && idx[2] < D.pos_num && idx[0] < D.attr_num &&
+                    idx[1] < D.attr_num && idx[2] < D.attr_num;
+    if (!ok) {
+        atomicOr(&F.counters[C_FLAGS], FLAG_INDEX_RANGE);
+        return false;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float* p = (const float*)(D.pos + (size_t)idx[k] * D.pos_stride);
+        v[k] = transform(D.mvp, p[0], p[1], p[2]);
+        const float* at = (const float*)(D.attr + (size_t)idx[k] * D.attr_stride);
+#pragma unroll
+        for (uint32_t s = 0; s < kMaxSlots; ++s) a[k][s] = s < D.nslots ? at[D.slot_attr[s]] : 0.0f;
+    }
+    return true;
+}
+
+constexpr uint32_t kSetupThreads = 256;
+
+// Per-workgroup statistics accumulator: one global atomic per workgroup.
+struct BlockStats {
+    uint32_t tile_entries, prims;
+};
+
+__device__ __forceinline__ void block_stats_add(BlockStats* bs, uint32_t te, uint32_t pr)
+{
+    te = wave_sum(te);
+    pr = wave_sum(pr);
+    if (lane_id() == 0) {
+        if (te) atomicAdd(&bs->tile_entries, te);
+        if (pr) atomicAdd(&bs->prims, pr);
+    }
+}
+
+__device__ __forceinline__ void block_stats_flush(const FrameDev& F, BlockStats* bs)
+{
+    if (threadIdx.x == 0) {
+        if (bs->tile_entries) atomicAdd(&F.counters[C_TILE_ENTRIES], bs->tile_entries);
+        if (bs->prims) atomicAdd(&F.counters[C_PRIMS], bs->prims);
+    }
+}
+
+__global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
+{
+    __shared__ BlockStats bs;
+    const bool stats = (F.flags & FF_STATS) != 0;
+    if (stats && threadIdx.x == 0) bs = BlockStats{ 0, 0 };
+    if (stats) __syncthreads();
+    const uint32_t d = F.block_draw[blockIdx.x];
+    const DrawDev& D = F.draws[d];
+    const ViewDev& V = F.views[D.view];
+    const uint32_t tri = F.block_tri0[blockIdx.x] + threadIdx.x;
+    const bool valid = tri < D.ntris;
+    const uint32_t g = D.tri_base + tri;
+
+    float4 v[3];
+    float a[3][kMaxSlots];
+    Prim s;
+    bool survive = false, clipped = false;
+    if (valid && fetch_triangle(F, D, tri, v, a)) {
+        const uint32_t c0 = clip_codes(v[0]), c1 = clip_codes(v[1]), c2 = clip_codes(v[2]);
+        const uint32_t mor = c0 | c1 | c2;
+        if (mor == 0) survive = setup_prim(s, v, a, D.nslots, V.width, V.height);
+        else clipped = (c0 & c1 & c2) == 0;
+    }
+    if (valid) {
+        if (survive) write_record(F, g, s, d);
+        else F.rec_meta[g] = 0xFFFFFFFFu;
+    }
+    const uint32_t q = wave_add_amount(&F.counters[C_CLIPQ], clipped ? 1u : 0u);
+    if (clipped) F.clip_queue[q] = g;  // at most one slot per triangle: never overflows
+
+    if (stats) block_stats_add(&bs, survive ? reference_tile_entries(s) : 0u, survive ? 1u : 0u);
+    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0);
+    if (stats) {
+        __syncthreads();
+        block_stats_flush(F, &bs);
+    }
+}
+
+// kt::Dot / kt::Lerp with the operation order the oracle defines.
+__device__ __forceinline__ float clip_dot(int plane, float4 v)
+{
+    const float px = plane == 0 ? 1.0f : (plane == 1 ? -1.0f : 0.0f);
+    const float py = plane == 2 ? 1.0f : (plane == 3 ? -1.0f : 0.0f);
+    const float pz = plane == 4 ? 1.0f : (plane == 5 ? -1.0f : 0.0f);
+    const float pw = 1.0f;
+    return ((px * v.x + py * v.y) + pz * v.z) + pw * v.w;
+}
+__device__ __forceinline__ float lerp_kt(float a, float b, float t) { return a + (b - a) * t; }
+
+struct ClipPoly {
+    float4 v[2][9];
+    float a[2][9][kMaxSlots];
+};
+
+// ClipPlane (Binning.cpp:85-165) over the set planes of `mask`, lowest first,
+// stopping when the polygon empties (:517-523). Returns the vertex count; the
+// result is in buffer `pin`.
+__device__ __noinline__ uint32_t clip_polygon(ClipPoly& poly, uint32_t mask, uint32_t& pin)
+{
+    uint32_t n = 3;
+    pin = 0;
+    do {
+        const int plane = __builtin_ctz(mask);
+        mask ^= 1u << plane;
+        uint32_t nout = 0;
+        uint32_t i0 = n - 1;
+        float d0 = clip_dot(plane, poly.v[pin][i0]);
+        for (uint32_t i1 = 0; i1 < n; ++i1) {
+            const float d1 = clip_dot(plane, poly.v[pin][i1]);
+            const bool in0 = d0 >= 0.0f, in1 = d1 >= 0.0f;
+            if (in0) {
+                poly.v[pin ^ 1][nout] = poly.v[pin][i0];
+                for (uint32_t t = 0; t < kMaxSlots; ++t) poly.a[pin ^ 1][nout][t] = poly.a[pin][i0][t];
+                ++nout;
+            }
+            if (in0 != in1) {
+                uint32_t ia, ib;
+                float tt;
+                if (in1) { tt = d1 / (d1 - d0); ia = i1; ib = i0; }
+                else     { tt = d0 / (d0 - d1); ia = i0; ib = i1; }
+                const float4 pa = poly.v[pin][ia], pb = poly.v[pin][ib];
+                poly.v[pin ^ 1][nout] = make_float4(lerp_kt(pa.x, pb.x, tt), lerp_kt(pa.y, pb.y, tt),
+                                                    lerp_kt(pa.z, pb.z, tt), lerp_kt(pa.w, pb.w, tt));
+                for (uint32_t t = 0; t < kMaxSlots; ++t)
+                    poly.a[pin ^ 1][nout][t] = lerp_kt(poly.a[pin][ia][t], poly.a[pin][ib][t], tt);
+                ++nout;
+            }
+            d0 = d1;
+            i0 = i1;
+        }
+        n = nout;
+        pin ^= 1;
+    } while (mask && n);
+    return n;
+}
+
+__device__ __forceinline__ uint32_t draw_of_triangle(const FrameDev& F, uint32_t g)
+{
+    uint32_t lo = 0, hi = F.ndraws;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (F.draws[mid].tri_base <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Slow path: triangles that cross a clip plane (BinTrisEntry's clip branch,
+// Binning.cpp:502-533). Each fan triangle gets a record after the
+// per-triangle slots; its order key is (g << 3) | fan index.
+__global__ __launch_bounds__(256) void k_clip(FrameDev F)
+{
+    const uint32_t n = F.counters[C_CLIPQ];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base_i = blockIdx.x * blockDim.x; base_i < n; base_i += stride) {
+        const uint32_t qi = base_i + threadIdx.x;
+        const bool active = qi < n;
+        uint32_t g = 0, d = 0, nfan = 0, pin = 0;
+        ClipPoly poly;
+        if (active) {
+            g = F.clip_queue[qi];
+            d = draw_of_triangle(F, g);
+            const DrawDev& D = F.draws[d];
+            float4 v[3];
+            float a[3][kMaxSlots];
+            fetch_triangle(F, D, g - D.tri_base, v, a);
+            const uint32_t mor = clip_codes(v[0]) | clip_codes(v[1]) | clip_codes(v[2]);
+            for (int k = 0; k < 3; ++k) {
+                poly.v[0][k] = v[k];
+                for (uint32_t t = 0; t < kMaxSlots; ++t) poly.a[0][k][t] = a[k][t];
+            }
+            const uint32_t np = clip_polygon(poly, mor, pin);
+            nfan = np >= 3 ? np - 2 : 0;
+        }
+        const uint32_t base = wave_add_amount(&F.counters[C_CLIPREC], nfan);
+        const bool fits = base + nfan <= F.clip_cap;
+        if (nfan && !fits) atomicOr(&F.counters[C_FLAGS], FLAG_CLIP_OVERFLOW);
+        if (nfan && fits) F.clip_base[g] = base;
+        const DrawDev& D = F.draws[d];
+        const ViewDev& V = F.views[D.view];
+        for (uint32_t j = 0; __ballot(j < nfan) != 0ull; ++j) {
+            Prim s;
+            bool ok = false;
+            if (j < nfan && fits) {
+                const float4 fv[3] = { poly.v[pin][0], poly.v[pin][j + 1], poly.v[pin][j + 2] };
+                float fa[3][kMaxSlots];
+                for (uint32_t t = 0; t < kMaxSlots; ++t) {
+                    fa[0][t] = poly.a[pin][0][t];
+                    fa[1][t] = poly.a[pin][j + 1][t];
+                    fa[2][t] = poly.a[pin][j + 2][t];
+                }
+                const uint32_t r = F.ntris + base + j;
+                ok = setup_prim(s, fv, fa, D.nslots, V.width, V.height);
+                if (ok) {
+                    write_record(F, r, s, d);
+                    F.clip_order[base + j] = (g << 3) | j;
+                } else {
+                    F.rec_meta[r] = 0xFFFFFFFFu;
+                }
+            }
+            if (F.flags & FF_STATS) {
+                const uint32_t te = wave_sum(ok ? reference_tile_entries(s) : 0u);
+                const uint32_t pr = wave_sum(ok ? 1u : 0u);
+                if (lane_id() == 0 && te) atomicAdd(&F.counters[C_TILE_ENTRIES], te);
+                if (lane_id() == 0 && pr) atomicAdd(&F.counters[C_PRIMS], pr);
+            }
+            walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0);
+        }
+    }
+}
+
+// Exclusive scan of bin counts and construction of the back end's queue: every
+// non-empty bin, plus every bin of a view with a clear pending (one workgroup).
+constexpr uint32_t kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void k_scan(FrameDev F)
+{
+    __shared__ uint32_t s_cnt[kScanThreads];
+    __shared__ uint32_t s_q[kScanThreads];
+    const uint32_t T = F.total_bins;
+    const uint32_t per = (T + kScanThreads - 1) / kScanThreads;
+    const uint32_t b = threadIdx.x * per, e = min(T, b + per);
+    uint32_t v = 0;
+    while (v + 1 < F.nviews && F.views[v + 1].bin_base <= b) ++v;
+    uint32_t sum = 0, nq = 0;
+    for (uint32_t t = b, vv = v; t < e; ++t) {
+        while (vv + 1 < F.nviews && F.views[vv + 1].bin_base <= t) ++vv;
+        const uint32_t c = F.bin_count[t];
+        sum += c;
+        nq += (c || (F.views[vv].flags & (VF_CLEAR_COLOUR | VF_CLEAR_DEPTH))) ? 1u : 0u;
+    }
+    s_cnt[threadIdx.x] = sum;
+    s_q[threadIdx.x] = nq;
+    __syncthreads();
+    for (uint32_t o = 1; o < kScanThreads; o <<= 1) {
+        const uint32_t a1 = threadIdx.x >= o ? s_cnt[threadIdx.x - o] : 0u;
+        const uint32_t a2 = threadIdx.x >= o ? s_q[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s_cnt[threadIdx.x] += a1;
+        s_q[threadIdx.x] += a2;
+        __syncthreads();
+    }
+    uint32_t run = s_cnt[threadIdx.x] - sum, qrun = s_q[threadIdx.x] - nq;
+    for (uint32_t t = b, vv = v; t < e; ++t) {
+        while (vv + 1 < F.nviews && F.views[vv + 1].bin_base <= t) ++vv;
+        const uint32_t c = F.bin_count[t];
+        F.bin_start[t] = run;
+        run += c;
+        if (c || (F.views[vv].flags & (VF_CLEAR_COLOUR | VF_CLEAR_DEPTH))) F.queue[qrun++] = t;
+    }
+    if (threadIdx.x == kScanThreads - 1) {
+        F.bin_start[T] = s_cnt[kScanThreads - 1];
+        F.counters[C_ENTRIES] = s_cnt[kScanThreads - 1];
+        F.counters[C_QLEN] = s_q[kScanThreads - 1];
+        if (s_cnt[kScanThreads - 1] > F.entry_cap) atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_emit(FrameDev F)
+{
+    if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
+    const uint32_t nrec = F.ntris + F.counters[C_CLIPREC];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base_r = blockIdx.x * blockDim.x; base_r < nrec; base_r += stride) {
+        const uint32_t r = base_r + threadIdx.x;
+        uint32_t d = 0xFFFFFFFFu;
+        if (r < nrec) d = F.rec_meta[r];
+        const bool active = d != 0xFFFFFFFFu;
+        int32_t c[3] = { 0, 0, 0 }, dx[3] = { 0, 0, 0 }, dy[3] = { 0, 0, 0 };
+        int32_t xmin = 0, xmax = 0, ymin = 0, ymax = 0;
+        uint32_t view = 0;
+        if (active) {
+            const uint4* R = F.rec_raster + (size_t)r * 4;
+            const uint4 q0 = R[0], q1 = R[1], q2 = R[2];
+            c[0] = (int32_t)q0.x; c[1] = (int32_t)q0.y; c[2] = (int32_t)q0.z;
+            dx[0] = (int32_t)q0.w; dx[1] = (int32_t)q1.x; dx[2] = (int32_t)q1.y;
+            dy[0] = (int32_t)q1.z; dy[1] = (int32_t)q1.w; dy[2] = (int32_t)q2.x;
+            xmin = q2.y & 0xFFFF; xmax = q2.y >> 16; ymin = q2.z & 0xFFFF; ymax = q2.z >> 16;
+            view = F.draws[d].view;
+        }
+        walk_bins(F, F.views[view], active, c, dx, dy, xmin, xmax, ymin, ymax, true, r);
+    }
+}
+
+}  // namespace srg

@@ -138,3 +138,10 @@ def test_spheres_1080p_1m_tris():
     sc = scenes.spheres_scene()
     assert sc.num_tris == 1_000_000
     _check(sc, "spheres_1m", nthreads=16)
+
+
+@pytest.mark.parametrize("bin_size", [16, 32, 64])
+def test_bin_sizes_same_image(bin_size):
+    """Work-bin size is a scheduling choice: every size reproduces the oracle."""
+    sc = helpers.random_scene(40, 4000, 517, 333, clip_heavy=True, ndraws=2, shaders=(0, 1))
+    _check(sc, f"bin_size_{bin_size}", bin_size=bin_size)
