@@ -1,0 +1,533 @@
+"""Known-answer and closed-form checks that pin the CPU oracle.
+
+The reference ships no tests or golden vectors (its only fixture is a PNG),
+so each oracle function is checked against what the mathematics or the
+hardware specification fixes, independently of the oracle's own formula:
+recorded RCPPS outputs and Intel's documented error bound, Morton as a
+bit-interleave bijection, packus saturation from the instruction
+definitions, coverage by exact integer edge functions with the reference's
+quirks encoded, depth and varyings against float64 plane evaluation, and
+texel-exact sampling against the untiled source image.
+"""
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+F = ctypes.c_float
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return O.load()
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def bits(x):
+    return int(np.float32(x).view(np.uint32))
+
+
+def from_bits(u):
+    return float(np.uint32(u).view(np.float32))
+
+
+# ---------------------------------------------------------------- rcpps
+
+def _fixture_table():
+    return [int(w, 16) for w in (ROOT / "tests/golden/rcpps_table.txt").read_text().split()]
+
+
+def test_rcpps_table_files_agree():
+    tab = _fixture_table()
+    assert len(tab) == 2048
+    for inc in ("oracle/rcpps_table.inc", "softrast_amd/csrc/rcpps_table.inc"):
+        words = [int(t.strip().rstrip("u"), 16) for t in (ROOT / inc).read_text().split("*/")[-1].split(",")
+                 if t.strip()]
+        assert words == tab, inc
+
+
+def test_rcpps_within_intel_bound(lib):
+    # Intel documents |rel err| <= 1.5 * 2^-12 for RCPPS.
+    for i in range(2048):
+        x = 1.0 + i / 2048.0
+        r = lib.sro_rcpps(F(x))
+        assert abs(r * x - 1.0) <= 1.5 * 2.0 ** -12, (i, r)
+
+
+def test_rcpps_exponent_scaling_and_specials(lib):
+    rng = np.random.default_rng(1)
+    for _ in range(2000):
+        m = rng.uniform(1.0, 2.0)
+        k = int(rng.integers(-120, 120))
+        s = -1.0 if rng.random() < 0.5 else 1.0
+        x = f32(s * m * 2.0 ** k)
+        r1 = lib.sro_rcpps(F(x))
+        r0 = lib.sro_rcpps(F(f32(s * m)))
+        assert r1 == f32(r0 * 2.0 ** -k)
+    assert lib.sro_rcpps(F(0.0)) == float("inf")
+    assert lib.sro_rcpps(F(-0.0)) == float("-inf")
+    assert lib.sro_rcpps(F(float("inf"))) == 0.0
+    assert lib.sro_rcpps(F(from_bits(0x00000100))) == float("inf")  # denormal input
+    assert lib.sro_rcpps(F(2.0 ** 127)) == 0.0                         # subnormal result flushed
+
+
+def test_rcpps_fixture_matches_host_if_intel(tmp_path):
+    info = pathlib.Path("/proc/cpuinfo").read_text() if pathlib.Path("/proc/cpuinfo").exists() else ""
+    if "GenuineIntel" not in info:
+        pytest.skip("RCPPS table was recorded on an Intel host; other vendors may differ")
+    exe = tmp_path / "gen"
+    subprocess.run(["gcc", "-O2", "-msse2", str(ROOT / "tests/golden/gen_rcpps_table.c"), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert [int(w, 16) for w in out] == _fixture_table()
+
+
+# ---------------------------------------------------------------- pack / morton
+
+@pytest.mark.parametrize("c,byte", [
+    (0.0, 0), (1.0, 255), (0.5, 128), (-0.25, 0), (1.5, 255),
+    (1.0 / 255.0, 2),    # fp32(1/255) * 255 = 1.00000006 (fused), + 0.5 rounds up
+    (128.5, 0),          # fma -> 32768: packus_epi32 keeps it, packus_epi16 reads it as negative
+    (128.49, 255),       # 32765.45 -> 32765: saturates to 255
+    (200.0, 0),          # 51000.5 -> RNE 51000 -> negative as int16
+    (300.0, 0),          # 76500.5 > 65535 -> packus_epi32 saturates to 0xFFFF -> -1 as int16
+    (float("nan"), 0), (float("inf"), 0), (float("-inf"), 0),
+    (2.5 / 255.0, 3),    # 2.5 + 0.5 = 3.0 exactly
+    (1.5 / 255.0 - 1e-9, 2),
+    (0.2 / 255.0, 1),    # 0.2 + 0.5 = 0.7 -> 1: the +0.5 before a round-to-nearest biases up half an LSB
+    (254.2 / 255.0, 255),
+])
+def test_pack_channel_saturation(lib, c, byte):
+    assert lib.sro_pack_rgba(F(c), F(0), F(0), F(0)) & 0xFF == byte
+    assert (lib.sro_pack_rgba(F(0), F(0), F(0), F(c)) >> 24) == byte
+
+
+def test_pack_rne_ties(lib):
+    # fma(c, 255, 0.5) exactly k + 0.5 rounds to even: c = 1/255 * 0 -> 0.5 -> 0
+    assert lib.sro_pack_rgba(F(0.0), F(0), F(0), F(0)) == 0
+    # c = 2/255 (not exact in fp32) stays within one LSB of 2
+    assert abs((lib.sro_pack_rgba(F(2 / 255), F(0), F(0), F(0)) & 0xFF) - 2) <= 1
+
+
+def test_morton_is_bit_interleave_bijection(lib):
+    seen = set()
+    for y in range(32):
+        for x in range(32):
+            m = lib.sro_morton5(x, y)
+            seen.add(m)
+            # de-interleave independently
+            xx = sum(((m >> (2 * b)) & 1) << b for b in range(5))
+            yy = sum(((m >> (2 * b + 1)) & 1) << b for b in range(5))
+            assert (xx, yy) == (x, y)
+    assert seen == set(range(1024))
+
+
+# ---------------------------------------------------------------- textures / sampling
+
+def _texture(w, h, seed=0, const=None):
+    from softrast_amd import TextureData
+    rng = np.random.default_rng(seed)
+    img = rng.integers(0, 256, (h, w, 4), dtype=np.uint8) if const is None else np.full((h, w, 4), const, np.uint8)
+    t = TextureData().CreateFromRGBA8(img, w, h, True)
+    st, keep = O.make_texture(t)
+    return img, t, st, keep
+
+
+def _sample(lib, st, u, v, du=(0.0, 0.0), dv=(0.0, 0.0)):
+    out = (F * 4)()
+    mip = lib.sro_sample_wrap(ctypes.byref(st), F(u), F(v), F(du[0]), F(du[1]), F(dv[0]), F(dv[1]), out)
+    return mip, np.array(list(out), np.float32)
+
+
+def test_calc_mip_known_gradients(lib):
+    _, _, st, _ = _texture(256, 256)
+    for g, want in [(0.0, 0), (0.5, 0), (1.0, 0), (1.99, 0), (2.0, 1), (4.0, 2), (8.0, 3), (255.0, 7), (256.0, 8),
+                    (1e6, 8)]:
+        assert lib.sro_calc_mip(ctypes.byref(st), F(g / 256), F(0), F(0), F(0)) == want, g
+        assert lib.sro_calc_mip(ctypes.byref(st), F(0), F(0), F(0), F(g / 256)) == want, g
+
+
+def test_calc_mip_width_height_swap_quirk(lib):
+    # Texture.cpp:217-221 scales du/dy by the height and dv/dx by the width.
+    _, _, st, _ = _texture(256, 32)  # width 256, height 32
+    assert lib.sro_calc_mip(ctypes.byref(st), F(0), F(1.0 / 32), F(0), F(0)) == 0      # 1 * 32/32
+    assert lib.sro_calc_mip(ctypes.byref(st), F(0), F(8.0 / 256), F(0), F(0)) == 0     # 8/256*32 = 1
+    assert lib.sro_calc_mip(ctypes.byref(st), F(0), F(0), F(4.0 / 32), F(0)) == 5      # 4/32*256 = 32
+
+
+def test_sample_constant_texture(lib):
+    _, _, st, _ = _texture(64, 64, const=77)
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        u, v = rng.uniform(-5, 5, 2)
+        d = rng.uniform(-0.2, 0.2, 4)
+        _, out = _sample(lib, st, u, v, (d[0], d[1]), (d[2], d[3]))
+        assert np.all(out == np.float32(77 * np.float32(1 / 255)))
+
+
+def test_sample_texel_exact_all_mips(lib):
+    img, t, st, _ = _texture(128, 64, seed=3)
+    for mip in range(t.m_numMips):
+        mw, mh = max(1, 128 >> mip), max(1, 64 >> mip)
+        # expected mip data: box average of the base footprint, rounded half up
+        fx, fy = 128 // mw, 64 // mh
+        box = img.reshape(mh, fy, mw, fx, 4).astype(np.uint64).sum(axis=(1, 3))
+        cnt = fx * fy
+        mipimg = ((box + cnt // 2) // cnt).astype(np.uint8)
+        # derivative that selects this mip: du/dx * width = 2^mip
+        dudx = (2.0 ** mip) / 128
+        for (x, y) in [(0, 0), (mw - 1, mh - 1), (mw // 2, mh // 3), (min(mw - 1, 5), min(mh - 1, 7))]:
+            got_mip, out = _sample(lib, st, x / mw, y / mh, (dudx, 0.0))
+            assert got_mip == mip
+            want = mipimg[y, x].astype(np.float32) * np.float32(1 / 255)
+            assert np.array_equal(out, want), (mip, x, y)
+
+
+def test_sample_wrap_negative_and_periodic(lib):
+    _, _, st, _ = _texture(32, 32, seed=4)
+    for k in range(0, 64, 5):
+        u = k / 64.0
+        _, a = _sample(lib, st, u, 0.25)
+        _, b = _sample(lib, st, u + 3.0, 0.25)
+        _, c = _sample(lib, st, u - 2.0, 0.25) if k else _sample(lib, st, 0.0, 0.25)
+        assert np.array_equal(a, b)
+        # negative coordinates reflect through the sign handling: -u samples 1 - frac(u)
+        _, d = _sample(lib, st, -u, 0.25)
+        _, e = _sample(lib, st, 1.0 - u, 0.25)
+        assert np.array_equal(d, e)
+
+
+def test_bilinear_midpoint(lib):
+    img, _, st, _ = _texture(32, 32, seed=5)
+    # halfway between texels 3 and 4 of row 6: weights exactly 0.5
+    _, out = _sample(lib, st, 3.5 / 32, 6 / 32)
+    a = img[6, 3].astype(np.float64) / 255
+    b = img[6, 4].astype(np.float64) / 255
+    assert np.allclose(out, 0.5 * (a + b), atol=1e-6)
+
+
+# ---------------------------------------------------------------- transform / clip
+
+def test_transform_matches_float64(lib):
+    rng = np.random.default_rng(6)
+    for _ in range(100):
+        m = rng.normal(0, 2, 16).astype(np.float32)
+        p = rng.normal(0, 10, 3).astype(np.float32)
+        out = (F * 4)()
+        lib.sro_transform(O.fptr(m), O.fptr(p), out)
+        M = m.astype(np.float64).reshape(4, 4).T  # column-major
+        want = M @ np.append(p.astype(np.float64), 1.0)
+        assert np.allclose(np.array(list(out)), want, rtol=1e-5, atol=1e-4)
+
+
+def test_clip_codes(lib):
+    cases = [((0, 0, 0.5, 1), 0), ((-2, 0, 0.5, 1), 1), ((2, 0, 0.5, 1), 2), ((0, -2, 0.5, 1), 4), ((0, 2, 0.5, 1), 8),
+             ((0, 0, -0.1, 1), 16), ((0, 0, 1.5, 1), 32), ((-2, 2, 1.5, 1), 1 | 8 | 32), ((1, -1, 1, 1), 0)]
+    for v, want in cases:
+        a = np.array(v, np.float32)
+        assert lib.sro_clip_codes(O.fptr(a)) == want, v
+
+
+def _clip(lib, verts, attribs, mask):
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    a = np.ascontiguousarray(attribs, np.float32).reshape(-1)
+    ov = np.zeros(36, np.float32)
+    oa = np.zeros(72, np.float32)
+    n = lib.sro_clip_triangle(O.fptr(v), O.fptr(a), mask, O.fptr(ov), O.fptr(oa))
+    return n, ov[:4 * n].reshape(n, 4), oa[:8 * n].reshape(n, 8)
+
+
+def test_clip_invariants(lib):
+    rng = np.random.default_rng(7)
+    planes = np.array([[1, 0, 0, 1], [-1, 0, 0, 1], [0, 1, 0, 1], [0, -1, 0, 1], [0, 0, 1, 1], [0, 0, -1, 1]], np.float64)
+    nclipped = 0
+    for _ in range(400):
+        verts = np.column_stack([rng.normal(0, 1.5, (3, 3)), rng.uniform(0.5, 2.0, 3)]).astype(np.float32)
+        attribs = rng.normal(0, 1, (3, 8)).astype(np.float32)
+        mask = 0
+        for vtx in verts:
+            mask |= lib.sro_clip_codes(O.fptr(np.ascontiguousarray(vtx)))
+        n, ov, oa = _clip(lib, verts, attribs, mask)
+        assert n <= 9
+        if mask == 0:
+            assert n == 3 and np.array_equal(ov, verts) and np.array_equal(oa, attribs)
+            continue
+        nclipped += 1
+        # every output vertex lies inside every plane the clipper was asked to use (to rounding)
+        for k in range(6):
+            if mask & (1 << k):
+                d = ov.astype(np.float64) @ planes[k]
+                assert np.all(d >= -1e-5 * np.maximum(1, np.abs(ov).max(axis=1)))
+        # output vertices are convex combinations of the inputs, attributes follow the same weights
+        if n:
+            A = np.vstack([verts.astype(np.float64).T, np.ones(3)])
+            for i in range(n):
+                w, *_ = np.linalg.lstsq(A, np.append(ov[i], 1.0), rcond=None)
+                assert w.min() > -1e-4 and abs(w.sum() - 1) < 1e-4
+                assert np.allclose(w @ attribs.astype(np.float64), oa[i], atol=2e-4)
+    assert nclipped > 100
+
+
+def test_clip_fully_outside_is_empty(lib):
+    verts = np.array([[2, 0, 0.5, 1], [3, 1, 0.5, 1], [2.5, -1, 0.5, 1]], np.float32)
+    n, _, _ = _clip(lib, verts, np.zeros((3, 8)), 2)
+    assert n == 0
+
+
+# ---------------------------------------------------------------- setup and coverage
+
+def _setup(lib, verts, attribs, W, H, nattr=8):
+    v = np.ascontiguousarray(verts, np.float32).reshape(-1)
+    a = np.ascontiguousarray(attribs, np.float32).reshape(-1)
+    ints = (ctypes.c_int32 * 19)()
+    fl = (F * 32)()
+    ok = lib.sro_setup_triangle(O.fptr(v), O.fptr(a), nattr, W, H, ints, fl)
+    return ok, list(ints), np.array(list(fl), np.float64)
+
+
+def test_planes_reproduce_vertex_values(lib):
+    rng = np.random.default_rng(8)
+    W, H = 640, 480
+    checked = 0
+    for _ in range(300):
+        verts = np.column_stack([rng.uniform(-0.9, 0.9, (3, 2)) * 3, rng.uniform(0.1, 0.9, 3) * 3,
+                                 rng.uniform(1, 3, 3)]).astype(np.float32)
+        verts[:, 0:3] /= 3
+        verts[:, 0] *= verts[:, 3]
+        verts[:, 1] *= verts[:, 3]
+        verts[:, 2] *= verts[:, 3]
+        attribs = rng.normal(0, 1, (3, 8)).astype(np.float32)
+        ok, ints, fl = _setup(lib, verts, attribs, W, H)
+        if not ok:
+            continue
+        iw = 1.0 / verts[:, 3].astype(np.float64)
+        rx = (verts[:, 0] * iw) * (W / 2) + W / 2
+        ry = (verts[:, 1] * iw) * (-H / 2) + H / 2
+        if abs((rx[1] - rx[0]) * (ry[2] - ry[0]) - (ry[1] - ry[0]) * (rx[2] - rx[0])) < 50:
+            continue
+        r0 = fl[0:2]
+        planes = [(fl[2], fl[3], fl[4], verts[:, 2] * iw), (fl[5], fl[6], fl[7], iw)]
+        for k in range(8):
+            planes.append((fl[8 + 3 * k], fl[9 + 3 * k], fl[10 + 3 * k], attribs[:, k] * iw))
+        for dx, dy, f0, vals in planes:
+            for j in range(3):
+                pred = f0 + dx * (rx[j] - r0[0]) + dy * (ry[j] - r0[1])
+                scale = max(1.0, np.abs(vals).max())
+                assert abs(pred - vals[j]) <= 1e-3 * scale, (pred, vals[j])
+        checked += 1
+    assert checked > 100
+
+
+def _exact_coverage(fp, W, H):
+    """Pixels the reference covers for one triangle in one 64x64 tile grid, from
+    exact integer edge functions with the reference's rules encoded:
+    bias: E_full >= -256 for edges with dy < 0 or (dy == 0 and dx > 0), else >= 0;
+    bbox ceil'd and clamped; 8x8 blocks from (bMin & ~7) while < bMax; a block is
+    skipped if for an edge none of the corners {xb, xb+64} x {yb, yb+64} is > 0."""
+    (x0, y0), (x1, y1), (x2, y2) = fp
+    edges = [((x0, y0), (x1, y1)), ((x1, y1), (x2, y2)), ((x2, y2), (x0, y0))]
+
+    def efull(e, X, Y):  # exact edge function at subpixel point, times 1
+        (ax, ay), (bx, by) = e
+        return (by - ay) * (X - ax) - (bx - ax) * (Y - ay)
+
+    def biased(e):
+        (ax, ay), (bx, by) = e
+        dy, dx = by - ay, ax - bx
+        return 256 if (dy < 0 or (dy == 0 and dx > 0)) else 0
+
+    def ceil_px(v):
+        return -((-v) // 256)
+
+    xmin = min(max(ceil_px(min(x0, x1, x2)), 0), W - 1)
+    xmax = min(max(ceil_px(max(x0, x1, x2)), 0), W - 1)
+    ymin = min(max(ceil_px(min(y0, y1, y2)), 0), H - 1)
+    ymax = min(max(ceil_px(max(y0, y1, y2)), 0), H - 1)
+    cov = set()
+    for ty in range(ymin >> 6, (ymax >> 6) + 1):
+        for tx in range(xmin >> 6, (xmax >> 6) + 1):
+            X0, Y0 = tx * 64, ty * 64
+            bminx, bmaxx = min(max(xmin - X0, 0), 64), min(max(xmax - X0, 0), 64)
+            bminy, bmaxy = min(max(ymin - Y0, 0), 64), min(max(ymax - Y0, 0), 64)
+            for yb in range(bminy & ~7, bmaxy, 8):
+                for xb in range(bminx & ~7, bmaxx, 8):
+                    # block corner reject on E = floor((E_full + bias) / 256) > 0
+                    rej = False
+                    for e in edges:
+                        b = biased(e)
+                        anyin = False
+                        for cx in (xb, xb + 64):
+                            for cy in (yb, yb + 64):
+                                if (efull(e, 256 * (X0 + cx), 256 * (Y0 + cy)) + b) // 256 > 0:
+                                    anyin = True
+                        rej = rej or not anyin
+                    if rej:
+                        continue
+                    for r in range(8):
+                        for l in range(8):
+                            X, Y = X0 + xb + l, Y0 + yb + r
+                            if all(efull(e, 256 * X, 256 * Y) + biased(e) >= 0 for e in edges):
+                                cov.add((X, Y))
+    return cov
+
+
+def test_coverage_brute_force_tiny_triangles(lib):
+    """Single small triangles rendered by the oracle vs exact integer coverage."""
+    rng = np.random.default_rng(9)
+    W, H = 160, 128
+    o = O.Oracle(nthreads=1)
+    tried = 0
+    for it in range(150):
+        # clip-space vertices with w = 1 inside the frustum
+        c = rng.uniform(-0.8, 0.8, 2)
+        verts = np.zeros((3, 8), np.float32)
+        verts[:, 0:2] = c + rng.normal(0, 0.03 if it % 3 else 0.2, (3, 2))
+        verts[:, 2] = rng.uniform(0.2, 0.8, 3)
+        ok, ints, _ = _setup(lib, np.column_stack([verts[:, 0:3], np.ones(3)]), np.zeros((3, 8)), W, H)
+        if not ok:
+            verts = verts[[0, 2, 1]]
+            ok, ints, _ = _setup(lib, np.column_stack([verts[:, 0:3], np.ones(3)]), np.zeros((3, 8)), W, H)
+            if not ok:
+                continue
+        fp = [(ints[13], ints[14]), (ints[15], ints[16]), (ints[17], ints[18])]
+        want = _exact_coverage(fp, W, H)
+        p = O.Plane(W, H)
+        o.clear(p)
+        idx = np.array([0, 1, 2], np.uint16)
+        o.render([dict(shader=2, tex=None, idx=idx, idx_stride=2, pos=verts, pos_stride=32, pos_num=3, attr=verts,
+                       attr_stride=32, attr_num=3, uv_offset=6, plane=0, mvp=np.eye(4, dtype=np.float32))], [p])
+        pid = p.primid.reshape(p.tiles_y, p.tiles_x, 64, 64)
+        got = set()
+        for ty, tx, yy, xx in zip(*np.nonzero(pid != 0xFFFFFFFF)):
+            got.add((int(tx) * 64 + int(xx), int(ty) * 64 + int(yy)))
+        # pixels with z <= 0 are never written; all here have z > 0
+        assert got == want, (it, sorted(got ^ want)[:10])
+        tried += 1
+    o.close()
+    assert tried > 100
+
+
+def _front(xy):
+    """Index order that makes a triangle front-facing: positive signed area in
+    raster space (y down), the test of Binning.cpp:305-311."""
+    (x0, y0), (x1, y1), (x2, y2) = [(p[0], -p[1]) for p in xy]
+    return [0, 1, 2] if (x2 - x0) * (y1 - y0) - (y2 - y0) * (x1 - x0) > 0 else [0, 2, 1]
+
+
+def test_depth_matches_float64_plane(lib):
+    W, H = 128, 128
+    verts = np.zeros((3, 8), np.float32)
+    verts[:, 0:3] = [[-0.7, -0.6, 0.2], [0.8, -0.5, 0.7], [0.1, 0.9, 0.4]]
+    o = O.Oracle(nthreads=1)
+    p = O.Plane(W, H)
+    o.clear(p)
+    o.render([dict(shader=2, tex=None, idx=np.array(_front(verts[:, 0:2]), np.uint16), idx_stride=2, pos=verts,
+                   pos_stride=32, pos_num=3, attr=verts, attr_stride=32, attr_num=3, uv_offset=6, plane=0,
+                   mvp=np.eye(4, dtype=np.float32))], [p])
+    o.close()
+    rx = (verts[:, 0].astype(np.float64) + 1) * W / 2
+    ry = (-verts[:, 1].astype(np.float64) + 1) * H / 2
+    A = np.column_stack([rx, ry, np.ones(3)])
+    coef = np.linalg.solve(A, verts[:, 2].astype(np.float64))
+    dep = p.depth.reshape(p.tiles_y, p.tiles_x, 64, 64)
+    n = 0
+    for ty, tx, yy, xx in zip(*np.nonzero(dep > 0)):
+        x, y = tx * 64 + xx, ty * 64 + yy
+        assert abs(dep[ty, tx, yy, xx] - (coef[0] * x + coef[1] * y + coef[2])) < 2e-6
+        n += 1
+    assert n > 1000
+
+
+def test_depth_test_nearest_wins_earliest_on_ties(lib):
+    """Reverse-Z: larger z wins; on exactly equal z the first submitted triangle stays."""
+    W, H = 64, 64
+    tri = np.array([[-0.9, -0.9, 0.5], [-0.9, 0.9, 0.5], [0.9, -0.9, 0.5]], np.float32)
+    verts = np.zeros((9, 8), np.float32)
+    verts[0:3, 0:3] = tri
+    verts[3:6, 0:3] = tri               # identical copy: exact depth tie
+    verts[6:9, 0:3] = tri
+    verts[6:9, 2] = 0.25                # farther (smaller reverse-Z depth)
+    f = _front(tri[:, 0:2])
+    idx = np.array([6 + f[0], 6 + f[1], 6 + f[2], f[0], f[1], f[2], 3 + f[0], 3 + f[1], 3 + f[2]], np.uint16)
+    o = O.Oracle(nthreads=1)
+    p = O.Plane(W, H)
+    o.clear(p)
+    o.render([dict(shader=2, tex=None, idx=idx, idx_stride=2, pos=verts, pos_stride=32, pos_num=9, attr=verts,
+                   attr_stride=32, attr_num=9, uv_offset=6, plane=0, mvp=np.eye(4, dtype=np.float32))], [p])
+    o.close()
+    written = p.primid[p.primid != 0xFFFFFFFF]
+    assert written.size > 500
+    assert np.all(written == ((1 << 3) | 7))  # triangle 1 (first at z = 0.5), unclipped
+
+
+def test_affine_interpolation_and_derivatives(lib):
+    """w = 1: varyings are affine in screen space; derivatives equal the gradient."""
+    W, H = 128, 128
+    verts = np.zeros((3, 8), np.float32)
+    verts[:, 0:3] = [[-0.8, -0.8, 0.5], [0.8, -0.7, 0.5], [-0.1, 0.8, 0.5]]
+    verts[:, 6] = [0.0, 1.0, 0.25]
+    verts[:, 7] = [0.0, 0.125, 1.0]
+    rx = (verts[:, 0].astype(np.float64) + 1) * W / 2
+    ry = (-verts[:, 1].astype(np.float64) + 1) * H / 2
+    A = np.column_stack([rx, ry, np.ones(3)])
+    cu = np.linalg.solve(A, verts[:, 6].astype(np.float64))
+    cv = np.linalg.solve(A, verts[:, 7].astype(np.float64))
+    o = O.Oracle(nthreads=1)
+    p = O.Plane(W, H)
+    o.clear(p)
+    o.render([dict(shader=2, tex=None, idx=np.array(_front(verts[:, 0:2]), np.uint16), idx_stride=2, pos=verts,
+                   pos_stride=32, pos_num=3, attr=verts, attr_stride=32, attr_num=3, uv_offset=6, plane=0,
+                   mvp=np.eye(4, dtype=np.float32))], [p])
+    o.close()
+    col = p.colour.reshape(p.tiles_y, p.tiles_x, 64, 64)
+    pid = p.primid.reshape(p.tiles_y, p.tiles_x, 64, 64)
+    n = 0
+    for ty, tx, yy, xx in zip(*np.nonzero(pid != 0xFFFFFFFF)):
+        x, y = tx * 64 + xx, ty * 64 + yy
+        u = cu @ [x, y, 1]
+        v = cv @ [x, y, 1]
+        c = int(col[ty, tx, yy, xx])
+        # SIMDUtil.h:92-95 converts fma(c, 255, 0.5) with round-to-nearest, so a
+        # byte is within half an LSB of c*255 + 0.5 (not of c*255)
+        assert abs((c & 0xFF) - (u * 255 + 0.5)) <= 0.5 + 1e-3
+        assert abs(((c >> 8) & 0xFF) - (v * 255 + 0.5)) <= 0.5 + 1e-3
+        n += 1
+    assert n > 2000
+
+
+# ---------------------------------------------------------------- frame-level
+
+def test_clear_and_blit(lib):
+    o = O.Oracle(nthreads=1)
+    p = O.Plane(100, 70)
+    p.depth[:] = 5.0
+    o.clear(p, 0xAABBCC12)
+    assert np.all(p.depth == 0.0) and np.all(p.colour == 0x12121212)
+    p.colour[:] = np.arange(p.colour.size, dtype=np.uint32)
+    lin = o.blit(p)
+    t = p.colour.reshape(p.tiles_y, p.tiles_x, 64, 64)
+    for (x, y) in [(0, 0), (99, 69), (64, 1), (63, 64), (70, 65)]:
+        assert lin[y, x] == t[y // 64, x // 64, y % 64, x % 64]
+    o.close()
+
+
+def test_thread_count_and_build_independence():
+    from softrast_amd import scenes
+    from tests import helpers
+    sc = scenes.cube_grid_scene(gx=12, gz=12, width=320, height=180, tex_size=64)
+    tex = helpers.build_textures(sc)
+    a, sa = helpers.oracle_render(sc, tex, nthreads=1)
+    b, sb = helpers.oracle_render(sc, tex, nthreads=4)
+    c, sc_ = helpers.oracle_render(sc, tex, nthreads=3, fast=True)
+    for q in (b, c):
+        assert np.array_equal(a.colour, q.colour)
+        assert np.array_equal(a.depth.view(np.uint32), q.depth.view(np.uint32))
+        assert np.array_equal(a.primid, q.primid)
+    assert sa == sb == sc_
