@@ -1,22 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark: SoftRast's sort-middle path on one MI355X.
 
-Workload (default): 1920x1080, 1,000,000 triangles = 4,000 seeded UV spheres
-(250 tris each, log-normal radii), 8 draw calls each bound to its own
-2048^2 RGBA8 value-noise texture with a full mip chain, UnlitDiffuse
-shading, reverse-Z, clear + draw + EndFrame per step. Inputs (vertex, index
-and texture data) are resident in HBM before timing starts; a step is one
-whole frame (clear, 8 DrawIndexed, synchronous EndFrame).
+Workload: 1920x1080, 1,000,000 triangles = 4,000 seeded UV spheres (250
+tris each, log-normal radii), 8 draw calls each bound to its own 2048^2
+RGBA8 value-noise texture with a full mip chain, UnlitDiffuse shading,
+reverse-Z. Vertex, index and texture data are resident in HBM before timing
+starts. Two paths are timed:
 
-Prints one JSON line. Also reported:
-  roofline      HBM roofline of the dominant kernel (k_backend), timed with
-                HIP events on the library's stream over the timed frames
-  cpu_baseline  the CPU oracle (a scalar C port of the reference path,
-                compiled -O3 -mavx2 -mfma) on the host cores, same frame
+  plain    one view per EndFrame: clear + 8 DrawIndexed + synchronous EndFrame
+  batched  --views V views of the same scene (camera yaw spread 3 degrees per
+           view, one framebuffer each) in one EndFrame, launched as a replayed
+           hipGraph, tile queue ordered by --tile-order
+
+A step is one EndFrame; frames/s counts views completed per second
+(value = the batched path). Also reported:
+  roofline      HBM roofline of the dominant kernel (k_backend) in the plain
+                path, timed with HIP events on the library's stream
+  cpu_baseline  the CPU oracle (a C port of the reference path, -O3
+                AVX2/FMA build, reference task model) on the host cores
   parity        differing-pixel counts of the benchmarked frame vs the oracle
 """
 import argparse
 import json
+import math
 import os
 import statistics
 import sys
@@ -36,19 +42,28 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--views", type=int, default=8, help="views per EndFrame in the batched path")
+    ap.add_argument("--tile-order", type=int, default=1, help="batched path: 0 natural, 1 tri-count, 2 learned")
+    ap.add_argument("--bin-size", type=int, default=0, help="work-bin edge 16/32/64 (0 = library default)")
     ap.add_argument("--cpu-frames", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(host cores, 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--tile-order", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true")
     return ap.parse_args()
 
 
-def build_frame(sr, ctx, fb, scene, texs):
+def view_mvp(scene, yaw_deg):
+    from softrast_amd import scenes
+    t = math.radians(yaw_deg)
+    return scenes.camera_mvp((0.0, 0.0, 0.0), (math.sin(t), 0.0, math.cos(t)), 60.0, scene.width / scene.height)
+
+
+def make_calls(sr, fb, scene, texs, mvp):
     calls = []
     for d in scene.draws:
         c = sr.DrawCall()
-        c.SetFrameBuffer(fb).SetMVP(scene.mvp)
+        c.SetFrameBuffer(fb).SetMVP(mvp)
         c.SetIndexBuffer(d.indices, 4, len(d.indices))
         c.SetPositionBuffer(d.vertices, 32, len(d.vertices))
         c.SetAttributeBuffer(d.vertices, 32, len(d.vertices), 6)
@@ -57,20 +72,39 @@ def build_frame(sr, ctx, fb, scene, texs):
     return calls
 
 
-def algorithmic_bytes(scene, covered_px):
-    """B = s_idx*3*T + 32*V + 8*W*H + 16*P_cov per frame; back-end share = 8*W*H + 16*P_cov."""
-    s_idx = scene.draws[0].indices.dtype.itemsize
-    T = scene.num_tris
-    V = sum(len(d.vertices) for d in scene.draws)
-    fe = s_idx * 3 * T + 32 * V
-    be = 8 * scene.width * scene.height + 16 * covered_px
-    return fe, be
+def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup):
+    """Returns (frames/s, ms per EndFrame, per-frame stats, ctx, framebuffers)."""
+    ctx = sr.RenderContext(flags=flags, tile_order=tile_order, bin_size=bin_size)
+    fbs, calls = [], []
+    for v in range(views):
+        fb = sr.FrameBuffer(ctx, scene.width, scene.height)
+        mvp = scene.mvp if views == 1 else view_mvp(scene, (v - (views - 1) / 2.0) * 3.0)
+        fbs.append(fb)
+        calls.append(make_calls(sr, fb, scene, texs, mvp))
+
+    def frame():
+        ctx.BeginFrame()
+        for fb, cs in zip(fbs, calls):
+            ctx.ClearFrameBuffer(fb, 0)
+            for c in cs:
+                c.m_frameBuffer = (fb, fb.m_writePlane)
+                ctx.DrawIndexed(c)
+        ctx.EndFrame()
+
+    for _ in range(warmup):  # also uploads and caches every buffer and texture
+        frame()
+    stats = []
+    t0 = time.perf_counter()  # EndFrame is synchronous: the device is idle here
+    for _ in range(steps):
+        frame()
+        stats.append(ctx.frame_stats())
+    elapsed = time.perf_counter() - t0  # and idle again here
+    return views * steps / elapsed, 1000.0 * elapsed / steps, stats, ctx, fbs
 
 
 def load_traffic(name):
-    path = os.path.join(ROOT, "profiles", name)
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             return json.load(f)
     except (OSError, ValueError):
         return None
@@ -80,7 +114,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(views shard across GPUs with no exchange; see the multi-GPU design notes)", file=sys.stderr)
+              f"(views shard across GPUs with no exchange; the multi-GPU design is described, not built)",
+              file=sys.stderr)
         sys.exit(2)
     import softrast_amd as sr
     from softrast_amd import scenes
@@ -91,68 +126,70 @@ def main():
     print(f"[bench] scene ready: {scene.num_tris} tris, {len(scene.draws)} draws, {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
 
-    ctx = sr.RenderContext(flags=sr.CFG_TIMING | sr.CFG_STATS, tile_order=args.tile_order)
-    fb = sr.FrameBuffer(ctx, scene.width, scene.height)
-    calls = build_frame(sr, ctx, fb, scene, texs)
-
-    def frame():
-        ctx.BeginFrame()
-        ctx.ClearFrameBuffer(fb, 0)
-        for c in calls:
-            c.m_frameBuffer = (fb, fb.m_writePlane)
-            ctx.DrawIndexed(c)
-        ctx.EndFrame()
-
-    for _ in range(args.warmup):  # also uploads and caches every buffer and texture
-        frame()
-    be_ms, fe_ms, dev_ms = [], [], []
-    t_start = time.perf_counter()  # EndFrame is synchronous: the device is idle here
-    for _ in range(args.steps):
-        frame()
-        st = ctx.frame_stats()
-        be_ms.append(st["ms_backend"])
-        fe_ms.append(st["ms_frontend"])
-        dev_ms.append(st["ms_total"])
-    elapsed = time.perf_counter() - t_start  # and idle again here
-    stats = ctx.frame_stats()
-    ms_per_step = 1000.0 * elapsed / args.steps
-    fps = args.steps / elapsed
-
-    colour, depth = fb.read_tiles()
+    # ---- plain path: one view per EndFrame, no graph, natural tile order ----
+    fps_plain, ms_plain, st_plain, ctx, fbs = time_path(sr, scene, texs, 1, sr.CFG_TIMING, sr.ORDER_NATURAL,
+                                                        args.bin_size, args.steps, args.warmup)
+    print(f"[bench] plain: {fps_plain:.1f} frames/s", file=sys.stderr, flush=True)
+    colour, depth = fbs[0].read_tiles()
     covered = int(np.count_nonzero(depth.reshape(-1).view(np.uint32) != 0))
-    fe_bytes, be_bytes = algorithmic_bytes(scene, covered)
-    be_avg = statistics.mean(be_ms)
-    achieved = be_bytes / (be_avg * 1e-3) / 1e9
+    be_ms = statistics.mean(s["ms_backend"] for s in st_plain)
+    fe_ms = statistics.mean(s["ms_frontend"] for s in st_plain)
+    dev_ms = statistics.mean(s["ms_total"] for s in st_plain)
+    for fb in fbs:
+        fb.destroy()
+    ctx.Shutdown()
+
+    # algorithmic bytes per frame: s_idx*3*T + 32*V + 8*W*H + 16*P_cov; the back end's share
+    # is the colour + depth written once and one bilinear RGBA8 quad per covered pixel
+    fe_bytes = 4 * 3 * scene.num_tris + 32 * sum(len(d.vertices) for d in scene.draws)
+    be_bytes = 8 * scene.width * scene.height + 16 * covered
+    achieved = be_bytes / (be_ms * 1e-3) / 1e9
     traffic = load_traffic("k_backend_spheres_1080p_1m_traffic.json")
 
+    # ---- batched path ----
+    fps_batched = ms_batched = dev_batched = None
+    if not args.no_batched:
+        fps_batched, ms_batched, st_b, bctx, bfbs = time_path(sr, scene, texs, args.views,
+                                                              sr.CFG_TIMING | sr.CFG_GRAPH, args.tile_order,
+                                                              args.bin_size, max(20, args.steps // args.views),
+                                                              args.warmup)
+        dev_batched = statistics.mean(s["ms_total"] for s in st_b)
+        for fb in bfbs:
+            fb.destroy()
+        bctx.Shutdown()
+        print(f"[bench] batched x{args.views}: {fps_batched:.1f} frames/s", file=sys.stderr, flush=True)
+
+    value = fps_batched if fps_batched else fps_plain
     out = {
         "metric": METRIC,
-        "value": round(fps, 2),
+        "value": round(value, 2),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(ms_batched if ms_batched else ms_plain, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32+int32",
         "data": "synthetic (seeded spheres 0x5EED0002, value-noise textures)",
         "config": {"workload": "spheres_1080p_1m", "width": scene.width, "height": scene.height,
-                   "triangles": scene.num_tris, "draws": len(scene.draws), "textures": "8x2048^2 RGBA8 + mips",
-                   "path": "plain single-view", "tile_order": args.tile_order},
-        "gfrags_per_s": round(stats["frags_raster"] * fps / 1e9, 3),
-        "mtris_per_s": round(scene.num_tris * fps / 1e6, 1),
-        "device_ms": {"frontend": round(statistics.mean(fe_ms), 4), "backend": round(be_avg, 4),
-                      "total": round(statistics.mean(dev_ms), 4)},
-        "frags_raster": stats["frags_raster"],
-        "bin_entries": stats["bin_entries"],
+                   "triangles_per_view": scene.num_tris, "draws_per_view": len(scene.draws),
+                   "textures": "8x2048^2 RGBA8 + mips",
+                   "path": (f"batched x{args.views} views, hipGraph, tile order {args.tile_order}"
+                            if fps_batched else "plain single-view")},
+        "plain": {"frames_per_s": round(fps_plain, 2), "ms_per_frame": round(ms_plain, 4),
+                  "device_ms": {"frontend": round(fe_ms, 4), "backend": round(be_ms, 4), "total": round(dev_ms, 4)}},
+        "mtris_per_s": round(scene.num_tris * value / 1e6, 1),
         "roofline": {"bound": "hbm", "kernel": "k_backend", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                     "algorithmic_bytes": be_bytes,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": be_bytes,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None},
         "frame_algorithmic_bytes": fe_bytes + be_bytes,
     }
+    if fps_batched:
+        out["batched"] = {"views": args.views, "frames_per_s": round(fps_batched, 2),
+                          "ms_per_launch": round(ms_batched, 4), "device_ms_per_launch": round(dev_batched, 4),
+                          "vs_plain": round(fps_batched / fps_plain, 3)}
 
     if not args.no_parity or not args.no_cpu_baseline:
         from oracle.oracle import Oracle, Plane, make_texture
@@ -168,13 +205,15 @@ def main():
         plane = Plane(scene.width, scene.height)
         if not args.no_parity:
             orc.clear(plane, 0)
-            orc.render(odraws, [plane])
+            ostats = orc.render(odraws, [plane])
+            out["frags_raster"] = ostats["frags_raster"]
+            out["frags_source"] = "oracle"
             dd = int(np.count_nonzero(plane.depth.view(np.uint32) != depth.reshape(-1).view(np.uint32)))
             dc = int(np.count_nonzero(plane.colour != colour.reshape(-1)))
-            out["parity"] = {"depth_px_diff": dd, "rgba8_px_diff": dc, "vs": "oracle"}
+            out["parity"] = {"depth_px_diff": dd, "rgba8_px_diff": dc, "vs": "oracle", "frame": "plain path"}
         if not args.no_cpu_baseline:
             times = []
-            for i in range(5 + args.cpu_frames):
+            for _ in range(5 + args.cpu_frames):
                 t = time.perf_counter()
                 orc.clear(plane, 0)
                 orc.render(odraws, [plane])
@@ -184,12 +223,23 @@ def main():
                                    "kind": "port",
                                    "sample": f"same 1080p/1M-tri frame (clear+render), median of {args.cpu_frames} "
                                              f"frames after 5 warm-up, {nthreads} threads, oracle -O3 AVX2/FMA build"}
-            out["speedup_vs_cpu"] = round(fps * med, 1)
+            out["speedup_vs_cpu"] = round(value * med, 1)
         orc.close()
 
+    if "frags_raster" not in out:  # no oracle frame: count on the GPU in an untimed frame
+        sctx = sr.RenderContext(flags=sr.CFG_STATS, bin_size=args.bin_size)
+        sfb = sr.FrameBuffer(sctx, scene.width, scene.height)
+        sctx.BeginFrame()
+        sctx.ClearFrameBuffer(sfb, 0)
+        for c in make_calls(sr, sfb, scene, texs, scene.mvp):
+            sctx.DrawIndexed(c)
+        sctx.EndFrame()
+        out["frags_raster"] = sctx.frame_stats()["frags_raster"]
+        out["frags_source"] = "gpu stats frame"
+        sfb.destroy()
+        sctx.Shutdown()
+    out["gfrags_per_s"] = round(out["frags_raster"] * value / 1e9, 3)
     print(json.dumps(out))
-    fb.destroy()
-    ctx.Shutdown()
 
 
 if __name__ == "__main__":

@@ -57,7 +57,9 @@ enum srg_config_flags {
     SRG_CFG_KEEP_PRIMID = 1u << 0, /* keep a per-pixel primitive-order plane (tests) */
     SRG_CFG_STATS = 1u << 1,       /* count rasterised fragments and binned entries */
     SRG_CFG_TIMING = 1u << 2,      /* time each kernel with HIP events */
-    SRG_CFG_TILE_COSTS = 1u << 3   /* record per-tile s_memtime cost in the back end */
+    SRG_CFG_TILE_COSTS = 1u << 3,  /* record per-tile s_memtime cost in the back end */
+    SRG_CFG_GRAPH = 1u << 4        /* capture the frame's launches in a hipGraph and replay it while
+                                      the draw list, views and arenas are unchanged */
 };
 
 typedef struct srg_config {
@@ -100,8 +102,9 @@ typedef struct srg_frame_stats {
     uint64_t frags_raster;    /* covered pixel samples before the depth test (SRG_CFG_STATS) */
     uint32_t tiles_queued;    /* tiles processed by the back end */
     uint32_t retries;         /* frame re-runs after arena growth */
-    float ms_frontend;        /* SRG_CFG_TIMING: setup + scan + emit */
-    float ms_backend;         /* SRG_CFG_TIMING: raster + shade kernel */
+    uint32_t graph_captures;  /* SRG_CFG_GRAPH: graphs (re)captured by this frame */
+    float ms_frontend;        /* SRG_CFG_TIMING without SRG_CFG_GRAPH: setup + clip + scan + emit */
+    float ms_backend;         /* SRG_CFG_TIMING without SRG_CFG_GRAPH: raster + shade kernel */
     float ms_total;           /* SRG_CFG_TIMING: whole frame on the device */
 } srg_frame_stats;
 

@@ -32,7 +32,7 @@ SHADER_UNLIT_DIFFUSE = 0
 SHADER_VISUALIZE_NORMALS = 1
 SHADER_VISUALIZE_UVS = 2
 ORDER_NATURAL, ORDER_TRICOUNT, ORDER_LEARNED = 0, 1, 2
-CFG_KEEP_PRIMID, CFG_STATS, CFG_TIMING, CFG_TILE_COSTS = 1, 2, 4, 8
+CFG_KEEP_PRIMID, CFG_STATS, CFG_TIMING, CFG_TILE_COSTS, CFG_GRAPH = 1, 2, 4, 8, 16
 FB_COLOUR, FB_DEPTH = 1, 2
 
 SRG_SYMBOLS = (
@@ -70,7 +70,7 @@ class SrgDraw(ctypes.Structure):
 
 class SrgFrameStats(ctypes.Structure):
     _fields_ = [("tris_in", ctypes.c_uint64), ("tris_clipped", ctypes.c_uint64), ("prims", ctypes.c_uint64),
-                ("bin_entries", ctypes.c_uint64), ("work_entries", ctypes.c_uint64), ("frags_raster", ctypes.c_uint64), ("tiles_queued", ctypes.c_uint32), ("retries", ctypes.c_uint32),
+                ("bin_entries", ctypes.c_uint64), ("work_entries", ctypes.c_uint64), ("frags_raster", ctypes.c_uint64), ("tiles_queued", ctypes.c_uint32), ("retries", ctypes.c_uint32), ("graph_captures", ctypes.c_uint32),
                 ("ms_frontend", ctypes.c_float), ("ms_backend", ctypes.c_float), ("ms_total", ctypes.c_float)]
 
 

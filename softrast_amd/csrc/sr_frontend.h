@@ -503,6 +503,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(FrameDev F)
     }
 }
 
+// Longest-first ordering of the back end's queue (one workgroup): bins are
+// bucketed by a log-scaled cost key (16 buckets per octave) -- the entry
+// count, or the learned model's predicted cost -- and written out from the
+// largest bucket down. The order inside a bucket is arbitrary; the image does
+// not depend on the order bins are processed in.
+constexpr uint32_t kOrderBuckets = 512;
+
+__device__ __forceinline__ uint32_t cost_bucket(float c)
+{
+    if (!(c > 1.0f)) return 0u;
+    const float b = __log2f(c) * 16.0f;  // scheduling key only; never touches the image
+    return b >= (float)(kOrderBuckets - 1) ? kOrderBuckets - 1 : 1u + (uint32_t)b;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_order(FrameDev F, const uint32_t* in_queue, const float* pred)
+{
+    __shared__ uint32_t hist[kOrderBuckets];
+    const uint32_t n = F.counters[C_QLEN];
+    for (uint32_t i = threadIdx.x; i < kOrderBuckets; i += kScanThreads) hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
+        const uint32_t gb = in_queue[i];
+        atomicAdd(&hist[cost_bucket(pred ? pred[gb] : (float)F.bin_count[gb])], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // descending exclusive scan: largest bucket first
+        uint32_t run = 0;
+        for (int b = (int)kOrderBuckets - 1; b >= 0; --b) {
+            const uint32_t c = hist[b];
+            hist[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
+        const uint32_t gb = in_queue[i];
+        const uint32_t pos = atomicAdd(&hist[cost_bucket(pred ? pred[gb] : (float)F.bin_count[gb])], 1u);
+        F.queue[pos] = gb;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_emit(FrameDev F)
 {
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;

@@ -149,13 +149,15 @@ struct srg_ctx {
     DevArray<uint4> rec_raster;
     DevArray<float4> rec_shade;
     DevArray<uint32_t> rec_meta, clip_queue, clip_base, clip_order;
-    DevArray<uint32_t> bin_count, bin_start, bin_cursor, queue, entries;
+    DevArray<uint32_t> bin_count, bin_start, bin_cursor, queue, queue2, entries;
     DevArray<uint64_t> bin_cost;
     DevArray<uint32_t> counters;
     DevArray<uint32_t> blit_staging;
     uint32_t* h_counters = nullptr;
     uint32_t clip_cap = 0, entry_cap = 0;
     uint32_t last_bins = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uint8_t> graph_sig;
 
     hipEvent_t ev[4] = {};
     srg_frame_stats stats{};
@@ -381,7 +383,7 @@ int run_frame(srg_ctx* ctx)
     // ---- arenas ----
     const uint32_t TB = (uint32_t)std::max<uint64_t>(total_bins, 1);
     if ((st = ctx->bin_count.ensure(TB)) || (st = ctx->bin_start.ensure(TB + 1)) || (st = ctx->bin_cursor.ensure(TB)) ||
-        (st = ctx->queue.ensure(TB)) || (st = ctx->bin_cost.ensure(TB)) || (st = ctx->counters.ensure(C_NCOUNTERS)) ||
+        (st = ctx->queue.ensure(TB)) || (st = ctx->queue2.ensure(TB)) || (st = ctx->bin_cost.ensure(TB)) || (st = ctx->counters.ensure(C_NCOUNTERS)) ||
         (st = ctx->clip_queue.ensure(std::max<uint32_t>(NT, 1))) || (st = ctx->clip_base.ensure(std::max<uint32_t>(NT, 1))))
         return st;
     uint64_t want_clip = ctx->cfg.max_prims ? ctx->cfg.max_prims : (uint64_t)NT / 8 + 4096;
@@ -398,8 +400,6 @@ int run_frame(srg_ctx* ctx)
             return st;
         ctx->clip_cap = (uint32_t)want_clip;
         ctx->entry_cap = (uint32_t)want_entries;
-
-        HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
 
         FrameDev F;
         std::memset(&F, 0, sizeof(F));
@@ -433,27 +433,80 @@ int run_frame(srg_ctx* ctx)
                   ((ctx->cfg.flags & SRG_CFG_KEEP_PRIMID) ? FF_PRIMID : 0u) |
                   ((ctx->cfg.flags & SRG_CFG_TILE_COSTS) ? FF_TILE_COSTS : 0u);
 
+        const uint32_t emit_blocks = std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 16384));
+        const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + 255) / 256, 512));
+        const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((TB + 255) / 256, 1024));
+        const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
+        // The whole frame on the stream: tables, six kernels, counters back.
+        auto enqueue = [&](bool split_events) -> int {
+            HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(k_frame_init, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->counters.ptr,
+                               ctx->bin_count.ptr, ctx->bin_cursor.ptr, TB);
+            if (ctx->cfg.flags & SRG_CFG_KEEP_PRIMID)
+                for (auto& v : vdev)
+                    if (v.primid)
+                        hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, ctx->stream, v.primid, 0xFFFFFFFFu,
+                                           (size_t)v.tiles_x * v.tiles_y * kTilePixels);
+            if (nblocks) hipLaunchKernelGGL(k_setup, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
+            if (NT) hipLaunchKernelGGL(k_clip, dim3(clip_blocks), dim3(256), 0, ctx->stream, F);
+            if (ordered) {
+                // natural queue into the spare buffer, then longest-first into F.queue
+                FrameDev Fs = F;
+                Fs.queue = ctx->queue2.ptr;
+                hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs);
+                hipLaunchKernelGGL(k_order, dim3(1), dim3(kScanThreads), 0, ctx->stream, F, (const uint32_t*)ctx->queue2.ptr,
+                                   (const float*)nullptr);
+            } else {
+                hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, F);
+            }
+            hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
+            if (split_events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+            launch_backend(ctx, K, F);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters.ptr, sizeof(uint32_t) * C_NCOUNTERS,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            return SRG_OK;
+        };
+
         const bool timing = (ctx->cfg.flags & SRG_CFG_TIMING) != 0;
-        if (timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-        hipLaunchKernelGGL(k_frame_init, dim3(std::max<uint32_t>(1, std::min<uint32_t>((TB + 255) / 256, 1024))),
-                           dim3(256), 0, ctx->stream, ctx->counters.ptr, ctx->bin_count.ptr, ctx->bin_cursor.ptr, TB);
-        if (ctx->cfg.flags & SRG_CFG_KEEP_PRIMID)
-            for (auto& v : vdev)
-                if (v.primid)
-                    hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, ctx->stream, v.primid, 0xFFFFFFFFu,
-                                       (size_t)v.tiles_x * v.tiles_y * kTilePixels);
-        if (nblocks) hipLaunchKernelGGL(k_setup, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
-        if (NT) hipLaunchKernelGGL(k_clip, dim3(std::max<uint32_t>(1, std::min<uint32_t>((NT + 255) / 256, 512))),
-                                   dim3(256), 0, ctx->stream, F);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, F);
-        const uint32_t emit_blocks = (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 16384);
-        hipLaunchKernelGGL(k_emit, dim3(std::max<uint32_t>(emit_blocks, 1)), dim3(256), 0, ctx->stream, F);
-        if (timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-        launch_backend(ctx, K, F);
-        if (timing) HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters.ptr, sizeof(uint32_t) * C_NCOUNTERS,
-                               hipMemcpyDeviceToHost, ctx->stream));
+        bool split = false;
+        if (ctx->cfg.flags & SRG_CFG_GRAPH) {
+            // Replay the captured frame when nothing it depends on changed:
+            // the table bytes, every pointer and capacity in F, and the
+            // launch shapes.
+            std::vector<uint8_t> sig(ctx->h_tables, ctx->h_tables + off);
+            const uint64_t extra[] = { (uint64_t)(uintptr_t)ctx->h_tables, (uint64_t)(uintptr_t)ctx->tables.ptr,
+                                       (uint64_t)(uintptr_t)ctx->h_counters, nblocks, emit_blocks, clip_blocks,
+                                       init_blocks, TB, K, ctx->cfg.flags, ctx->cfg.wg_per_cu };
+            sig.insert(sig.end(), (const uint8_t*)&F, (const uint8_t*)&F + sizeof(F));
+            sig.insert(sig.end(), (const uint8_t*)extra, (const uint8_t*)extra + sizeof(extra));
+            if (!ctx->graph_exec || sig != ctx->graph_sig) {
+                if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+                ctx->graph_exec = nullptr;
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                const int est = enqueue(false);
+                const hipError_t cerr = hipStreamEndCapture(ctx->stream, &graph);
+                if (est) return est;
+                if (cerr != hipSuccess) return SRG_ERR_HIP;
+                const hipError_t ierr = hipGraphInstantiate(&ctx->graph_exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ierr != hipSuccess) {
+                    ctx->graph_exec = nullptr;
+                    return SRG_ERR_HIP;
+                }
+                ctx->graph_sig.swap(sig);
+                ctx->stats.graph_captures++;
+            }
+            if (timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+            HIP_TRY(hipGraphLaunch(ctx->graph_exec, ctx->stream));
+            if (timing) HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+        } else {
+            if (timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+            if ((st = enqueue(timing))) return st;
+            if (timing) HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+            split = timing;
+        }
         HIP_TRY(hipStreamSynchronize(ctx->stream));
 
         const uint32_t flags = ctx->h_counters[C_FLAGS];
@@ -475,8 +528,10 @@ int run_frame(srg_ctx* ctx)
         ctx->stats.frags_raster = (uint64_t)ctx->h_counters[C_FRAGS] | ((uint64_t)ctx->h_counters[C_FRAGS + 1] << 32);
         ctx->stats.tiles_queued = ctx->h_counters[C_QLEN];
         if (timing) {
-            (void)hipEventElapsedTime(&ctx->stats.ms_frontend, ctx->ev[0], ctx->ev[1]);
-            (void)hipEventElapsedTime(&ctx->stats.ms_backend, ctx->ev[1], ctx->ev[2]);
+            if (split) {
+                (void)hipEventElapsedTime(&ctx->stats.ms_frontend, ctx->ev[0], ctx->ev[1]);
+                (void)hipEventElapsedTime(&ctx->stats.ms_backend, ctx->ev[1], ctx->ev[2]);
+            }
             (void)hipEventElapsedTime(&ctx->stats.ms_total, ctx->ev[0], ctx->ev[2]);
         }
         break;
@@ -540,6 +595,7 @@ void srg_destroy(srg_ctx* ctx)
 {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     for (auto& kv : ctx->buffers)
         if (kv.second.owned) (void)hipFree(kv.second.dev);
     for (auto& kv : ctx->textures) (void)hipFree(kv.second.dev);
@@ -555,6 +611,7 @@ void srg_destroy(srg_ctx* ctx)
     ctx->bin_start.release();
     ctx->bin_cursor.release();
     ctx->queue.release();
+    ctx->queue2.release();
     ctx->entries.release();
     ctx->bin_cost.release();
     ctx->counters.release();

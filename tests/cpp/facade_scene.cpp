@@ -1,0 +1,75 @@
+// A host program written against the reference's frame API the way
+// Viewer/Scene.cpp:32-65 and Viewer/Main.cpp:41-86 use it, compiled against
+// include/sr/Renderer.h. Draws a small quad grid textured with a checker and
+// prints the number of non-clear pixels after the blit.
+// Exit codes: 0 = rendered (or no GPU: prints "no-device"), 1 = failure.
+#include <cstdio>
+#include <vector>
+
+#include "sr/Renderer.h"
+
+struct Vertex { float pos[3]; float norm[3]; float uv[2]; };
+
+int main()
+{
+    sr::RenderContext renderCtx;
+    if (renderCtx.Status() == SRG_ERR_NO_DEVICE) {
+        std::printf("no-device\n");
+        return 0;
+    }
+    if (renderCtx.Status()) return 1;
+    const uint32_t W = 320, H = 200;
+    sr::FrameBuffer framebuffer(renderCtx, W, H);
+    if (framebuffer.m_status) return 1;
+
+    std::vector<uint8_t> checker(64 * 64 * 4);
+    for (uint32_t y = 0; y < 64; ++y)
+        for (uint32_t x = 0; x < 64; ++x)
+            for (uint32_t c = 0; c < 4; ++c) checker[(y * 64 + x) * 4 + c] = ((x / 8 + y / 8) & 1) ? 230 : 20;
+    sr::Tex::TextureData tex;
+    if (tex.CreateFromRGBA8(checker.data(), 64, 64, true)) return 1;
+
+    // 4x4 quads facing the camera at z = 4 (clip = (x, y, z * 0.25, z))
+    std::vector<Vertex> verts;
+    std::vector<uint16_t> indices;
+    for (int qy = 0; qy < 4; ++qy)
+        for (int qx = 0; qx < 4; ++qx) {
+            uint16_t base = (uint16_t)verts.size();
+            for (int k = 0; k < 4; ++k) {
+                float x = -3.0f + 1.5f * qx + ((k & 1) ? 1.2f : 0.0f);
+                float y = -3.0f + 1.5f * qy + ((k & 2) ? 1.2f : 0.0f);
+                verts.push_back(Vertex{ { x, y, 4.0f }, { 0, 0, -1 }, { (k & 1) ? 1.0f : 0.0f, (k & 2) ? 1.0f : 0.0f } });
+            }
+            uint16_t q[6] = { (uint16_t)base, (uint16_t)(base + 1), (uint16_t)(base + 2),
+                              (uint16_t)(base + 1), (uint16_t)(base + 3), (uint16_t)(base + 2) };
+            indices.insert(indices.end(), q, q + 6);
+        }
+    sr::Mat4 mvp;  // x, y scaled into the frustum; w = z
+    float m[16] = { 0.5f, 0, 0, 0, 0, 0.8f, 0, 0, 0, 0, 0.25f, 1, 0, 0, 0, 0 };
+    for (int i = 0; i < 16; ++i) mvp.m[i] = m[i];
+
+    std::vector<uint32_t> pixels(W * H);
+    for (int frame = 0; frame < 2; ++frame) {
+        renderCtx.BeginFrame();
+        renderCtx.ClearFrameBuffer(framebuffer, 0);
+        sr::DrawCall call;
+        call.SetFrameBuffer(&framebuffer);
+        call.m_mvp = mvp;
+        call.SetAttributeBuffer(verts.data(), sizeof(Vertex), (uint32_t)verts.size(), 6);
+        call.m_positionBuffer.m_ptr = verts.data();
+        call.m_positionBuffer.m_stride = sizeof(Vertex);
+        call.m_positionBuffer.m_num = (uint32_t)verts.size();
+        call.m_indexBuffer.m_ptr = indices.data();
+        call.m_indexBuffer.m_num = (uint32_t)indices.size();
+        call.m_indexBuffer.m_stride = sizeof(uint16_t);
+        call.m_pixelUniforms = &tex;
+        call.m_pixelShader = sr::shader::UnlitDiffuseShader;
+        if (int st = renderCtx.DrawIndexed(call)) { std::printf("DrawIndexed: %s\n", srg_status_string(st)); return 1; }
+        if (int st = renderCtx.EndFrame()) { std::printf("EndFrame: %s\n", srg_status_string(st)); return 1; }
+        if (int st = renderCtx.Blit(framebuffer, (uint8_t*)pixels.data())) { std::printf("Blit: %s\n", srg_status_string(st)); return 1; }
+    }
+    uint32_t drawn = 0;
+    for (uint32_t p : pixels) drawn += p != 0;
+    std::printf("drawn=%u\n", drawn);
+    return drawn > 1000 ? 0 : 1;
+}

@@ -145,3 +145,62 @@ def test_bin_sizes_same_image(bin_size):
     """Work-bin size is a scheduling choice: every size reproduces the oracle."""
     sc = helpers.random_scene(40, 4000, 517, 333, clip_heavy=True, ndraws=2, shaders=(0, 1))
     _check(sc, f"bin_size_{bin_size}", bin_size=bin_size)
+
+
+def _multi_view_scene(seed):
+    """Three views of different sizes sharing vertex data, one draw list each."""
+    from softrast_amd import scenes
+    base = helpers.random_scene(seed, 3000, 64, 64, clip_heavy=True, ndraws=2, shaders=(0, 1))
+    sizes = [(333, 211), (640, 360), (128, 500)]
+    views = []
+    for i, (w, h) in enumerate(sizes):
+        mvp = scenes.camera_mvp((0.3 * i, 0.0, 0.0), (0.1 * i, 0.0, 1.0), 60.0 + 10 * i, w / h)
+        views.append((w, h, mvp))
+    return base, views
+
+
+@pytest.mark.parametrize("tile_order,graph", [(0, False), (1, False), (1, True)])
+def test_batched_views_schedules_same_image(tile_order, graph):
+    import softrast_amd as sr
+    from oracle.oracle import Oracle, Plane, make_texture
+    base, views = _multi_view_scene(41)
+    tex = helpers.build_textures(base)
+    _say(f"[parity] case=batched_views order={tile_order} graph={graph}")
+    # oracle: all views in one frame, one plane per view
+    orc = Oracle(nthreads=8)
+    planes = [Plane(w, h) for (w, h, _) in views]
+    for p in planes:
+        orc.clear(p, 0)
+    stex = [make_texture(t) for t in tex]
+    odraws = []
+    for vi, (w, h, mvp) in enumerate(views):
+        for d in base.draws:
+            vf = d.vertices.view(np.float32).reshape(-1, 8)
+            odraws.append(dict(shader=d.shader, tex=stex[0][0], idx=d.indices, idx_stride=4, pos=vf, pos_stride=32,
+                               pos_num=len(vf), attr=vf, attr_stride=32, attr_num=len(vf), uv_offset=6, plane=vi,
+                               mvp=mvp))
+    orc.render(odraws, planes)
+    orc.close()
+    # GPU: same frame, three consecutive times (graph capture then replays)
+    flags = sr.CFG_KEEP_PRIMID | (sr.CFG_GRAPH if graph else 0)
+    ctx = sr.RenderContext(flags=flags, tile_order=tile_order)
+    fbs = [sr.FrameBuffer(ctx, w, h) for (w, h, _) in views]
+    shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader}
+    for it in range(3):
+        ctx.BeginFrame()
+        for fb, (w, h, mvp) in zip(fbs, views):
+            ctx.ClearFrameBuffer(fb, 0)
+            for d in base.draws:
+                c = sr.DrawCall().SetFrameBuffer(fb).SetMVP(mvp)
+                c.SetIndexBuffer(d.indices, 4, len(d.indices)).SetPositionBuffer(d.vertices, 32, len(d.vertices))
+                c.SetAttributeBuffer(d.vertices, 32, len(d.vertices), 6).SetPixelShader(shader[d.shader], tex[0])
+                ctx.DrawIndexed(c)
+        ctx.EndFrame()
+        for vi, (fb, p) in enumerate(zip(fbs, planes)):
+            col, dep, pid = fb.read_tiles(primid=True)
+            d = helpers.compare(p, col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
+            _say(f"[parity] batched_views frame={it} view={vi} diffs={d}")
+            assert not any(d.values()), d
+    if graph:
+        assert ctx.frame_stats()["graph_captures"] == 0  # the last frame replayed the captured graph
+    ctx.Shutdown()
