@@ -72,7 +72,10 @@ typedef struct srg_config {
     uint32_t wg_per_cu;     /* persistent back-end workgroups per CU, 0 = default */
     uint32_t bin_size;      /* back-end work bin edge: 64, 32 or 16 pixels, 0 = 32. The
                                reference's 64x64 bin semantics are kept for every size. */
-    uint32_t reserved[8];
+    uint32_t sched_warmup_frames;    /* SRG_ORDER_LEARNED: frames recorded before the first
+                                        training (0 = 8); entry-count order meanwhile */
+    uint32_t sched_retrain_interval; /* SRG_ORDER_LEARNED: retrain every N frames (0 = never) */
+    uint32_t reserved[6];
 } srg_config;
 
 /* Mirrors DrawCall (Renderer.h:119-150). Buffers are registered handles plus
@@ -106,6 +109,9 @@ typedef struct srg_frame_stats {
     float ms_frontend;        /* SRG_CFG_TIMING without SRG_CFG_GRAPH: setup + clip + scan + emit */
     float ms_backend;         /* SRG_CFG_TIMING without SRG_CFG_GRAPH: raster + shade kernel */
     float ms_total;           /* SRG_CFG_TIMING: whole frame on the device */
+    float sched_err;          /* SRG_ORDER_LEARNED: mean |log2 predicted - log2 measured| bin cost */
+    uint32_t sched_trained;   /* SRG_ORDER_LEARNED: 1 once the model orders the queue */
+    uint32_t sched_trainings; /* SRG_ORDER_LEARNED: training runs done by this frame */
 } srg_frame_stats;
 
 /* ---- context: RenderContext ctor / Shutdown (Renderer.cpp:138-159) ---- */
@@ -156,6 +162,12 @@ int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out);
  * the reference: its per-tile scope timing lives in microprofile
  * (Rasterizer.cpp:527). */
 int srg_get_bin_stats(srg_ctx* ctx, uint32_t* counts, uint64_t* costs, uint32_t capacity, uint32_t* nbins);
+/* The learned tile-cost model's parameters (4-32-32-1 fp32 MLP, SRG_SCHED_PARAMS
+ * floats): saving and restoring them is the scheduler's checkpoint. Setting
+ * them marks the model trained. No counterpart in the reference. */
+#define SRG_SCHED_PARAMS 1249u
+int srg_sched_get_model(srg_ctx* ctx, float* params, uint32_t count);
+int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count);
 
 /* ---- host helper: TextureData::CreateFromRGBA8 (Texture.cpp:119-199) ----
  * Tiles a power-of-two RGBA8 image (>= 32 in both dimensions) into 32x32

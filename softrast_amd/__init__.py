@@ -40,7 +40,8 @@ SRG_SYMBOLS = (
     "srg_update_buffer", "srg_release_buffer", "srg_register_texture", "srg_release_texture",
     "srg_fb_create", "srg_fb_destroy", "srg_fb_write_plane", "srg_fb_swap_planes", "srg_fb_read_tiles",
     "srg_fb_write_tiles", "srg_begin_frame", "srg_clear", "srg_draw_indexed", "srg_end_frame", "srg_blit",
-    "srg_get_frame_stats", "srg_get_bin_stats", "srg_texture_build_rgba8",
+    "srg_get_frame_stats", "srg_get_bin_stats", "srg_sched_get_model", "srg_sched_set_model",
+    "srg_texture_build_rgba8",
 )
 
 
@@ -53,7 +54,9 @@ class SrgError(RuntimeError):
 class SrgConfig(ctypes.Structure):
     _fields_ = [("struct_size", ctypes.c_uint32), ("device", ctypes.c_int32), ("flags", ctypes.c_uint32),
                 ("tile_order", ctypes.c_uint32), ("max_prims", ctypes.c_uint32), ("max_entries", ctypes.c_uint32),
-                ("wg_per_cu", ctypes.c_uint32), ("bin_size", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 8)]
+                ("wg_per_cu", ctypes.c_uint32), ("bin_size", ctypes.c_uint32),
+                ("sched_warmup_frames", ctypes.c_uint32), ("sched_retrain_interval", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 6)]
 
 
 class SrgDraw(ctypes.Structure):
@@ -71,7 +74,8 @@ class SrgDraw(ctypes.Structure):
 class SrgFrameStats(ctypes.Structure):
     _fields_ = [("tris_in", ctypes.c_uint64), ("tris_clipped", ctypes.c_uint64), ("prims", ctypes.c_uint64),
                 ("bin_entries", ctypes.c_uint64), ("work_entries", ctypes.c_uint64), ("frags_raster", ctypes.c_uint64), ("tiles_queued", ctypes.c_uint32), ("retries", ctypes.c_uint32), ("graph_captures", ctypes.c_uint32),
-                ("ms_frontend", ctypes.c_float), ("ms_backend", ctypes.c_float), ("ms_total", ctypes.c_float)]
+                ("ms_frontend", ctypes.c_float), ("ms_backend", ctypes.c_float), ("ms_total", ctypes.c_float),
+                ("sched_err", ctypes.c_float), ("sched_trained", ctypes.c_uint32), ("sched_trainings", ctypes.c_uint32)]
 
 
 _lib = None
@@ -111,6 +115,8 @@ def load_library(path=None):
         "srg_blit": (i32, [vp, vp, vp, vp, vp]),
         "srg_get_frame_stats": (i32, [vp, ctypes.POINTER(SrgFrameStats)]),
         "srg_get_bin_stats": (i32, [vp, vp, vp, u32, ctypes.POINTER(u32)]),
+        "srg_sched_get_model": (i32, [vp, vp, u32]),
+        "srg_sched_set_model": (i32, [vp, vp, u32]),
         "srg_texture_build_rgba8": (i32, [vp, u32, u32, i32, vp, pu64, ctypes.POINTER(u32),
                                           ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     }
@@ -303,13 +309,14 @@ class RenderContext:
     """
 
     def __init__(self, device=0, flags=0, tile_order=ORDER_NATURAL, max_prims=0, max_entries=0, wg_per_cu=0,
-                 bin_size=0, lib_path=None):
+                 bin_size=0, sched_warmup_frames=0, sched_retrain_interval=0, lib_path=None):
         self._lib = load_library(lib_path)
         cfg = SrgConfig()
         cfg.struct_size = ctypes.sizeof(SrgConfig)
         cfg.device, cfg.flags, cfg.tile_order = device, flags, tile_order
         cfg.max_prims, cfg.max_entries, cfg.wg_per_cu = max_prims, max_entries, wg_per_cu
         cfg.bin_size = bin_size
+        cfg.sched_warmup_frames, cfg.sched_retrain_interval = sched_warmup_frames, sched_retrain_interval
         h = ctypes.c_void_p()
         _check(self._lib.srg_create(ctypes.byref(cfg), ctypes.byref(h)), "RenderContext")
         self._ctx = h
@@ -413,6 +420,18 @@ class RenderContext:
         _check(self._lib.srg_get_bin_stats(self._ctx, _ptr(counts), _ptr(costs), n.value, ctypes.byref(n)),
                "bin_stats")
         return counts, costs
+
+    SCHED_PARAMS = 1249
+
+    def sched_model(self):
+        """Learned tile-cost model parameters (the scheduler's checkpoint)."""
+        p = np.zeros(self.SCHED_PARAMS, np.float32)
+        _check(self._lib.srg_sched_get_model(self._ctx, _ptr(p), p.size), "sched_model")
+        return p
+
+    def set_sched_model(self, params):
+        p = np.ascontiguousarray(params, np.float32)
+        _check(self._lib.srg_sched_set_model(self._ctx, _ptr(p), p.size), "set_sched_model")
 
     def Shutdown(self):
         if self._ctx:

@@ -45,6 +45,8 @@ enum : uint32_t {
     C_TILE_ENTRIES = 6,// 64x64-tile entries as the reference counts them (stats)
     C_PRIMS = 7,       // primitives that survived the cull (stats)
     C_FRAGS = 8,       // 64-bit rasterised fragment count (words 8, 9)
+    C_SCHED_ERR = 10,  // float: summed |log2 predicted - log2 measured| bin cost
+    C_SCHED_N = 11,    // bins recorded by the scheduler this frame
     C_NCOUNTERS = 16,
 };
 

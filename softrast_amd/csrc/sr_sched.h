@@ -1,0 +1,241 @@
+// sr_sched.h -- learned work-bin cost model for the back end's queue order.
+//
+// The back end is persistent and takes bins from a queue; starting the most
+// expensive bins first shortens the frame's tail. Every kernel here changes
+// only the order bins are processed in, never the arithmetic of a pixel.
+//
+//   k_bin_features  per work bin, from its entry list: x0 = log2(1 + entries),
+//                   x1 = log2(1 + 8x8 blocks of lane-private primitives),
+//                   x2 = log2(1 + 8x8 blocks of wave-cooperative primitives),
+//                   x3 = log2(1 + summed bbox-in-bin area / bin area)
+//   k_mlp_infer     4 -> 32 -> 32 -> 1 ReLU MLP (fp32) -> predicted log2 cost
+//   k_mlp_record    after the back end: (features, log2 measured s_memtime
+//                   cost) samples into a ring buffer, and the model's error
+//   k_mlp_train     one workgroup, mini-batch Adam on the recorded samples
+#pragma once
+#include "sr_backend.h"
+
+namespace srg {
+
+constexpr uint32_t kFeat = 4, kHid = 32;
+constexpr uint32_t kMlpParams = kHid * kFeat + kHid + kHid * kHid + kHid + kHid + 1;  // 1249
+
+struct MlpParams {
+    float w1[kHid][kFeat];
+    float b1[kHid];
+    float w2[kHid][kHid];
+    float b2[kHid];
+    float w3[kHid];
+    float b3;
+};
+static_assert(sizeof(MlpParams) == kMlpParams * sizeof(float), "packed parameters");
+
+struct SchedSample {
+    float x[kFeat];
+    float y;  // log2 measured cost
+};
+
+__global__ __launch_bounds__(256) void k_bin_features(FrameDev F, float4* feat)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t gb = blockIdx.x * 4u + (threadIdx.x >> 6);
+    if (gb >= F.total_bins) return;  // whole wave exits together
+    const ViewDev V = F.views[view_of_bin(F, gb)];
+    const uint32_t S = F.bin_size, K = F.bin_k;
+    const uint32_t lb = gb - V.bin_base, sbx = lb % V.bins_x, sby = lb / V.bins_x;
+    const int32_t X0 = (int32_t)((sbx / K) * kBin), Y0 = (int32_t)((sby / K) * kBin);
+    const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S;
+    const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
+    uint32_t small_blocks = 0, large_blocks = 0, area = 0;
+    for (uint32_t e = lane; e < cnt; e += 64) {
+        const uint4 q2 = F.rec_raster[(size_t)F.entries[beg + e] * 4 + 2];
+        const int32_t xmin = q2.y & 0xFFFF, xmax = q2.y >> 16, ymin = q2.z & 0xFFFF, ymax = q2.z >> 16;
+        const uint32_t xs = max((uint32_t)clampi(xmin - X0, 0, 64) & ~7u, sx0);
+        const uint32_t xe = min((uint32_t)clampi(xmax - X0, 0, 64), sx0 + S);
+        const uint32_t ys = max((uint32_t)clampi(ymin - Y0, 0, 64) & ~7u, sy0);
+        const uint32_t ye = min((uint32_t)clampi(ymax - Y0, 0, 64), sy0 + S);
+        const uint32_t nb = (xe > xs ? (xe - xs + 7u) >> 3 : 0u) * (ye > ys ? (ye - ys + 7u) >> 3 : 0u);
+        if (nb <= kSmallBlocks) small_blocks += nb;
+        else large_blocks += nb;
+        const uint32_t ax = (uint32_t)max(0, min(xmax - X0 + 1, (int32_t)(sx0 + S)) - max(xmin - X0, (int32_t)sx0));
+        const uint32_t ay = (uint32_t)max(0, min(ymax - Y0 + 1, (int32_t)(sy0 + S)) - max(ymin - Y0, (int32_t)sy0));
+        area += ax * ay;
+    }
+    small_blocks = wave_sum(small_blocks);
+    large_blocks = wave_sum(large_blocks);
+    area = wave_sum(area);
+    if (lane == 0)
+        feat[gb] = make_float4(__log2f(1.0f + (float)cnt), __log2f(1.0f + (float)small_blocks),
+                               __log2f(1.0f + (float)large_blocks), __log2f(1.0f + (float)area / (float)(S * S)));
+}
+
+__device__ __forceinline__ float mlp_forward(const MlpParams& P, const float x[kFeat], float h1[kHid], float h2[kHid])
+{
+#pragma unroll
+    for (uint32_t i = 0; i < kHid; ++i) {
+        float a = P.b1[i];
+#pragma unroll
+        for (uint32_t j = 0; j < kFeat; ++j) a += P.w1[i][j] * x[j];
+        h1[i] = a > 0.0f ? a : 0.0f;
+    }
+#pragma unroll
+    for (uint32_t i = 0; i < kHid; ++i) {
+        float a = P.b2[i];
+#pragma unroll
+        for (uint32_t j = 0; j < kHid; ++j) a += P.w2[i][j] * h1[j];
+        h2[i] = a > 0.0f ? a : 0.0f;
+    }
+    float o = P.b3;
+#pragma unroll
+    for (uint32_t i = 0; i < kHid; ++i) o += P.w3[i] * h2[i];
+    return o;
+}
+
+// Predicted cost per bin (2^out); bins without entries predict 0.
+__global__ __launch_bounds__(256) void k_mlp_infer(FrameDev F, const float4* feat, const MlpParams* params, float* pred)
+{
+    __shared__ MlpParams P;
+    for (uint32_t i = threadIdx.x; i < kMlpParams; i += 256) ((float*)&P)[i] = ((const float*)params)[i];
+    __syncthreads();
+    const uint32_t gb = blockIdx.x * 256u + threadIdx.x;
+    if (gb >= F.total_bins) return;
+    if (F.bin_count[gb] == 0) {
+        pred[gb] = 0.0f;
+        return;
+    }
+    const float4 f = feat[gb];
+    const float x[kFeat] = { f.x, f.y, f.z, f.w };
+    float h1[kHid], h2[kHid];
+    const float o = mlp_forward(P, x, h1, h2);
+    pred[gb] = exp2f(fminf(o, 120.0f));
+}
+
+// After the back end: append (features, log2 cost) of every non-empty bin to
+// the ring buffer and accumulate |predicted - measured| in log2 units.
+__global__ __launch_bounds__(256) void k_mlp_record(FrameDev F, const float4* feat, const float* pred,
+                                                    SchedSample* ring, uint32_t ring_cap, uint32_t* ring_head)
+{
+    const uint32_t gb = blockIdx.x * 256u + threadIdx.x;
+    // a frame that overflowed an arena is rendered again; its costs are not recorded
+    const bool clean = (F.counters[C_FLAGS] & (FLAG_CLIP_OVERFLOW | FLAG_ENTRY_OVERFLOW)) == 0;
+    const bool ok = clean && gb < F.total_bins && F.bin_count[gb] > 0 && F.bin_cost[gb] > 0;
+    const uint32_t slot = wave_add_amount(ring_head, ok ? 1u : 0u);
+    const uint32_t nok = wave_sum(ok ? 1u : 0u);
+    if (lane_id() == 0 && nok) atomicAdd(&F.counters[C_SCHED_N], nok);
+    if (!ok) return;
+    const float4 f = feat[gb];
+    const float y = __log2f((float)F.bin_cost[gb]);
+    SchedSample s;
+    s.x[0] = f.x; s.x[1] = f.y; s.x[2] = f.z; s.x[3] = f.w;
+    s.y = y;
+    ring[slot % ring_cap] = s;
+    if (pred) atomicAdd((float*)&F.counters[C_SCHED_ERR], fabsf(__log2f(fmaxf(pred[gb], 1.0f)) - y));
+}
+
+struct AdamState {
+    float m[kMlpParams];
+    float v[kMlpParams];
+    uint32_t t;
+};
+
+constexpr uint32_t kTrainBatch = 256;
+
+// Mini-batch Adam on the squared log2-cost error, one workgroup of 256
+// threads, one sample per thread per step. Activations and output deltas of
+// the batch live in LDS; each thread then owns a slice of the parameters and
+// sums its gradient over the batch.
+__global__ __launch_bounds__(kTrainBatch) void k_mlp_train(MlpParams* params, AdamState* adam, const SchedSample* ring,
+                                                           const uint32_t* ring_head, uint32_t ring_cap, uint32_t steps,
+                                                           float lr, uint32_t seed)
+{
+    __shared__ MlpParams P;
+    __shared__ float sx[kTrainBatch][kFeat];
+    __shared__ float sh1[kTrainBatch][kHid];
+    __shared__ float sh2[kTrainBatch][kHid];
+    __shared__ float sd2[kTrainBatch][kHid];   // dL/d(pre-activation of layer 2)
+    __shared__ float sd1[kTrainBatch][kHid];   // dL/d(pre-activation of layer 1)
+    __shared__ float sdo[kTrainBatch];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < kMlpParams; i += kTrainBatch) ((float*)&P)[i] = ((const float*)params)[i];
+    const uint32_t n = min(*ring_head, ring_cap);
+    __syncthreads();
+    if (n == 0) return;
+    uint32_t rng = seed * 747796405u + t * 2891336453u + 1u;
+    const uint32_t t0 = adam->t;
+    const float b1c = 0.9f, b2c = 0.999f, eps = 1e-8f;
+    for (uint32_t step = 0; step < steps; ++step) {
+        rng = rng * 1664525u + 1013904223u;
+        const SchedSample s = ring[(rng >> 8) % n];
+        float x[kFeat] = { s.x[0], s.x[1], s.x[2], s.x[3] };
+        float h1[kHid], h2[kHid];
+        const float o = mlp_forward(P, x, h1, h2);
+        const float dout = 2.0f * (o - s.y) / (float)kTrainBatch;
+        for (uint32_t j = 0; j < kFeat; ++j) sx[t][j] = x[j];
+        for (uint32_t i = 0; i < kHid; ++i) {
+            sh1[t][i] = h1[i];
+            sh2[t][i] = h2[i];
+            sd2[t][i] = h2[i] > 0.0f ? dout * P.w3[i] : 0.0f;
+        }
+        sdo[t] = dout;
+        for (uint32_t j = 0; j < kHid; ++j) {
+            float a = 0.0f;
+            for (uint32_t i = 0; i < kHid; ++i) a += P.w2[i][j] * sd2[t][i];
+            sd1[t][j] = h1[j] > 0.0f ? a : 0.0f;
+        }
+        __syncthreads();
+        // gradients: thread t owns parameters t, t + 256, ...
+        const float tt = (float)(t0 + step + 1u);
+        for (uint32_t p = t; p < kMlpParams; p += kTrainBatch) {
+            float g = 0.0f;
+            if (p < kHid * kFeat) {
+                const uint32_t i = p / kFeat, j = p % kFeat;
+                for (uint32_t b = 0; b < kTrainBatch; ++b) g += sd1[b][i] * sx[b][j];
+            } else if (p < kHid * kFeat + kHid) {
+                const uint32_t i = p - kHid * kFeat;
+                for (uint32_t b = 0; b < kTrainBatch; ++b) g += sd1[b][i];
+            } else if (p < kHid * kFeat + kHid + kHid * kHid) {
+                const uint32_t q = p - (kHid * kFeat + kHid), i = q / kHid, j = q % kHid;
+                for (uint32_t b = 0; b < kTrainBatch; ++b) g += sd2[b][i] * sh1[b][j];
+            } else if (p < kHid * kFeat + 2 * kHid + kHid * kHid) {
+                const uint32_t i = p - (kHid * kFeat + kHid + kHid * kHid);
+                for (uint32_t b = 0; b < kTrainBatch; ++b) g += sd2[b][i];
+            } else if (p < kMlpParams - 1) {
+                const uint32_t i = p - (kHid * kFeat + 2 * kHid + kHid * kHid);
+                for (uint32_t b = 0; b < kTrainBatch; ++b) g += sdo[b] * sh2[b][i];
+            } else {
+                for (uint32_t b = 0; b < kTrainBatch; ++b) g += sdo[b];
+            }
+            const float m = b1c * adam->m[p] + (1.0f - b1c) * g;
+            const float v = b2c * adam->v[p] + (1.0f - b2c) * g * g;
+            adam->m[p] = m;
+            adam->v[p] = v;
+            const float mh = m / (1.0f - powf(b1c, tt)), vh = v / (1.0f - powf(b2c, tt));
+            ((float*)&P)[p] -= lr * mh / (sqrtf(vh) + eps);
+        }
+        __syncthreads();
+    }
+    for (uint32_t i = t; i < kMlpParams; i += kTrainBatch) ((float*)params)[i] = ((const float*)&P)[i];
+    if (t == 0) adam->t = t0 + steps;
+}
+
+// Deterministic small initialisation (He-style scale) and cleared Adam state.
+__global__ void k_mlp_init(MlpParams* params, AdamState* adam, uint32_t seed)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kMlpParams) return;
+    uint32_t h = (i + 1u) * 2654435761u ^ seed;
+    h ^= h >> 16;
+    h *= 2246822519u;
+    h ^= h >> 13;
+    const float u = (float)(h & 0xFFFFFFu) / 16777216.0f - 0.5f;
+    float scale = 0.0f;
+    if (i < kHid * kFeat) scale = 2.0f * sqrtf(2.0f / kFeat);
+    else if (i >= kHid * kFeat + kHid && i < kHid * kFeat + kHid + kHid * kHid) scale = 2.0f * sqrtf(2.0f / kHid);
+    else if (i >= kHid * kFeat + 2 * kHid && i < kMlpParams - 1) scale = 2.0f * sqrtf(1.0f / kHid);
+    ((float*)params)[i] = (i == kMlpParams - 1) ? 16.0f : u * scale;  // bias: about 2^16 ticks
+    adam->m[i] = 0.0f;
+    adam->v[i] = 0.0f;
+    if (i == 0) adam->t = 0;
+}
+
+}  // namespace srg

@@ -25,7 +25,7 @@
 #include <vector>
 
 #include "srgpu.h"
-#include "sr_backend.h"
+#include "sr_sched.h"
 
 namespace srg {
 
@@ -157,6 +157,15 @@ struct srg_ctx {
     uint32_t clip_cap = 0, entry_cap = 0;
     uint32_t last_bins = 0;
     hipGraphExec_t graph_exec = nullptr;
+    // learned tile-cost scheduler
+    void* sched_params = nullptr;
+    void* sched_adam = nullptr;
+    SchedSample* sched_ring = nullptr;
+    uint32_t* sched_head = nullptr;
+    DevArray<float4> sched_feat;
+    DevArray<float> sched_pred;
+    bool sched_trained = false;
+    uint32_t sched_frames = 0, sched_last_train = 0;
     std::vector<uint8_t> graph_sig;
 
     hipEvent_t ev[4] = {};
@@ -240,6 +249,57 @@ struct ViewKey {
     srg_fb* fb;
     uint32_t plane;
 };
+
+constexpr uint32_t kSchedRing = 1u << 18;  // recorded (features, cost) samples
+
+int ensure_sched(srg_ctx* ctx)
+{
+    if (ctx->sched_params) return SRG_OK;
+    if (hipMalloc(&ctx->sched_params, sizeof(MlpParams)) != hipSuccess ||
+        hipMalloc(&ctx->sched_adam, sizeof(AdamState)) != hipSuccess ||
+        hipMalloc((void**)&ctx->sched_ring, sizeof(SchedSample) * kSchedRing) != hipSuccess ||
+        hipMalloc((void**)&ctx->sched_head, sizeof(uint32_t)) != hipSuccess)
+        return SRG_ERR_NOMEM;
+    hipLaunchKernelGGL(k_mlp_init, dim3((kMlpParams + 255) / 256), dim3(256), 0, ctx->stream,
+                       (MlpParams*)ctx->sched_params, (AdamState*)ctx->sched_adam, 0x5EED5C4Du);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->sched_head, 0u, (size_t)1);
+    HIP_TRY(hipGetLastError());
+    return SRG_OK;
+}
+
+// After a frame of SRG_ORDER_LEARNED: report the model's error and train it
+// once enough frames are recorded (and again every retrain interval).
+int sched_after_frame(srg_ctx* ctx)
+{
+    const uint32_t n = ctx->h_counters[C_SCHED_N];
+    float err;
+    std::memcpy(&err, &ctx->h_counters[C_SCHED_ERR], sizeof(err));
+    ctx->stats.sched_err = (ctx->sched_trained && n) ? err / (float)n : 0.0f;
+    ctx->sched_frames++;
+    const uint32_t warm = ctx->cfg.sched_warmup_frames ? ctx->cfg.sched_warmup_frames : 8u;
+    uint32_t steps = 0;
+    float lr = 0.0f;
+    if (!ctx->sched_trained && ctx->sched_frames >= warm) {
+        steps = 600;
+        lr = 3e-3f;
+    } else if (ctx->sched_trained && ctx->cfg.sched_retrain_interval &&
+               ctx->sched_frames - ctx->sched_last_train >= ctx->cfg.sched_retrain_interval) {
+        steps = 150;
+        lr = 1e-3f;
+    }
+    if (steps) {
+        hipLaunchKernelGGL(k_mlp_train, dim3(1), dim3(kTrainBatch), 0, ctx->stream, (MlpParams*)ctx->sched_params,
+                           (AdamState*)ctx->sched_adam, (const SchedSample*)ctx->sched_ring,
+                           (const uint32_t*)ctx->sched_head, kSchedRing, steps, lr, ctx->sched_frames);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        ctx->sched_trained = true;
+        ctx->sched_last_train = ctx->sched_frames;
+        ctx->stats.sched_trainings++;
+    }
+    ctx->stats.sched_trained = ctx->sched_trained ? 1u : 0u;
+    return SRG_OK;
+}
 
 uint32_t bin_k_of(const srg_config& cfg)
 {
@@ -437,6 +497,12 @@ int run_frame(srg_ctx* ctx)
         const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + 255) / 256, 512));
         const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((TB + 255) / 256, 1024));
         const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
+        const bool learned = ctx->cfg.tile_order == SRG_ORDER_LEARNED;
+        if (learned) {
+            if ((st = ensure_sched(ctx)) || (st = ctx->sched_feat.ensure(TB)) || (st = ctx->sched_pred.ensure(TB)))
+                return st;
+            F.flags |= FF_TILE_COSTS;
+        }
         // The whole frame on the stream: tables, six kernels, counters back.
         auto enqueue = [&](bool split_events) -> int {
             HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
@@ -449,19 +515,28 @@ int run_frame(srg_ctx* ctx)
                                            (size_t)v.tiles_x * v.tiles_y * kTilePixels);
             if (nblocks) hipLaunchKernelGGL(k_setup, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
             if (NT) hipLaunchKernelGGL(k_clip, dim3(clip_blocks), dim3(256), 0, ctx->stream, F);
-            if (ordered) {
-                // natural queue into the spare buffer, then longest-first into F.queue
-                FrameDev Fs = F;
-                Fs.queue = ctx->queue2.ptr;
-                hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs);
-                hipLaunchKernelGGL(k_order, dim3(1), dim3(kScanThreads), 0, ctx->stream, F, (const uint32_t*)ctx->queue2.ptr,
-                                   (const float*)nullptr);
-            } else {
-                hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, F);
-            }
+            FrameDev Fs = F;  // with ordering, the natural queue goes to the spare buffer first
+            if (ordered) Fs.queue = ctx->queue2.ptr;
+            hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs);
             hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
+            if (learned) {
+                hipLaunchKernelGGL(k_bin_features, dim3((TB + 3) / 4), dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr);
+                if (ctx->sched_trained)
+                    hipLaunchKernelGGL(k_mlp_infer, dim3((TB + 255) / 256), dim3(256), 0, ctx->stream, F,
+                                       (const float4*)ctx->sched_feat.ptr, (const MlpParams*)ctx->sched_params,
+                                       ctx->sched_pred.ptr);
+            }
+            if (ordered)
+                hipLaunchKernelGGL(k_order, dim3(1), dim3(kScanThreads), 0, ctx->stream, F,
+                                   (const uint32_t*)ctx->queue2.ptr,
+                                   (const float*)(learned && ctx->sched_trained ? ctx->sched_pred.ptr : nullptr));
             if (split_events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
             launch_backend(ctx, K, F);
+            if (learned)
+                hipLaunchKernelGGL(k_mlp_record, dim3((TB + 255) / 256), dim3(256), 0, ctx->stream, F,
+                                   (const float4*)ctx->sched_feat.ptr,
+                                   (const float*)(ctx->sched_trained ? ctx->sched_pred.ptr : nullptr), ctx->sched_ring,
+                                   kSchedRing, ctx->sched_head);
             HIP_TRY(hipGetLastError());
             HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters.ptr, sizeof(uint32_t) * C_NCOUNTERS,
                                    hipMemcpyDeviceToHost, ctx->stream));
@@ -477,7 +552,9 @@ int run_frame(srg_ctx* ctx)
             std::vector<uint8_t> sig(ctx->h_tables, ctx->h_tables + off);
             const uint64_t extra[] = { (uint64_t)(uintptr_t)ctx->h_tables, (uint64_t)(uintptr_t)ctx->tables.ptr,
                                        (uint64_t)(uintptr_t)ctx->h_counters, nblocks, emit_blocks, clip_blocks,
-                                       init_blocks, TB, K, ctx->cfg.flags, ctx->cfg.wg_per_cu };
+                                       init_blocks, TB, K, ctx->cfg.flags, ctx->cfg.wg_per_cu,
+                                       (uint64_t)ctx->sched_trained, ctx->cfg.tile_order,
+                                       (uint64_t)(uintptr_t)ctx->sched_feat.ptr, (uint64_t)(uintptr_t)ctx->sched_pred.ptr };
             sig.insert(sig.end(), (const uint8_t*)&F, (const uint8_t*)&F + sizeof(F));
             sig.insert(sig.end(), (const uint8_t*)extra, (const uint8_t*)extra + sizeof(extra));
             if (!ctx->graph_exec || sig != ctx->graph_sig) {
@@ -527,6 +604,7 @@ int run_frame(srg_ctx* ctx)
         ctx->stats.tris_clipped = ctx->h_counters[C_CLIPQ];
         ctx->stats.frags_raster = (uint64_t)ctx->h_counters[C_FRAGS] | ((uint64_t)ctx->h_counters[C_FRAGS + 1] << 32);
         ctx->stats.tiles_queued = ctx->h_counters[C_QLEN];
+        if (learned && (st = sched_after_frame(ctx))) return st;
         if (timing) {
             if (split) {
                 (void)hipEventElapsedTime(&ctx->stats.ms_frontend, ctx->ev[0], ctx->ev[1]);
@@ -596,6 +674,12 @@ void srg_destroy(srg_ctx* ctx)
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->sched_params) (void)hipFree(ctx->sched_params);
+    if (ctx->sched_adam) (void)hipFree(ctx->sched_adam);
+    if (ctx->sched_ring) (void)hipFree(ctx->sched_ring);
+    if (ctx->sched_head) (void)hipFree(ctx->sched_head);
+    ctx->sched_feat.release();
+    ctx->sched_pred.release();
     for (auto& kv : ctx->buffers)
         if (kv.second.owned) (void)hipFree(kv.second.dev);
     for (auto& kv : ctx->textures) (void)hipFree(kv.second.dev);
@@ -909,6 +993,27 @@ int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out)
 {
     if (!ctx || !out) return SRG_ERR_INVALID;
     *out = ctx->stats;
+    return SRG_OK;
+}
+
+int srg_sched_get_model(srg_ctx* ctx, float* params, uint32_t count)
+{
+    if (!ctx || !params || count != kMlpParams) return SRG_ERR_INVALID;
+    int st = ensure_sched(ctx);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(params, ctx->sched_params, sizeof(MlpParams), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SRG_OK;
+}
+
+int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count)
+{
+    if (!ctx || !params || count != kMlpParams) return SRG_ERR_INVALID;
+    int st = ensure_sched(ctx);
+    if (st) return st;
+    HIP_TRY(hipMemcpyAsync(ctx->sched_params, params, sizeof(MlpParams), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->sched_trained = true;
     return SRG_OK;
 }
 

@@ -159,7 +159,7 @@ def _multi_view_scene(seed):
     return base, views
 
 
-@pytest.mark.parametrize("tile_order,graph", [(0, False), (1, False), (1, True)])
+@pytest.mark.parametrize("tile_order,graph", [(0, False), (1, False), (1, True), (2, False), (2, True)])
 def test_batched_views_schedules_same_image(tile_order, graph):
     import softrast_amd as sr
     from oracle.oracle import Oracle, Plane, make_texture
@@ -183,7 +183,8 @@ def test_batched_views_schedules_same_image(tile_order, graph):
     orc.close()
     # GPU: same frame, three consecutive times (graph capture then replays)
     flags = sr.CFG_KEEP_PRIMID | (sr.CFG_GRAPH if graph else 0)
-    ctx = sr.RenderContext(flags=flags, tile_order=tile_order)
+    # the learned order trains its model after the first frame
+    ctx = sr.RenderContext(flags=flags, tile_order=tile_order, sched_warmup_frames=1)
     fbs = [sr.FrameBuffer(ctx, w, h) for (w, h, _) in views]
     shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader}
     for it in range(3):
@@ -201,6 +202,32 @@ def test_batched_views_schedules_same_image(tile_order, graph):
             d = helpers.compare(p, col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
             _say(f"[parity] batched_views frame={it} view={vi} diffs={d}")
             assert not any(d.values()), d
+    st = ctx.frame_stats()
     if graph:
-        assert ctx.frame_stats()["graph_captures"] == 0  # the last frame replayed the captured graph
+        assert st["graph_captures"] == 0  # the last frame replayed the captured graph
+    if tile_order == 2:
+        assert st["sched_trained"] == 1 and st["sched_trainings"] == 0
+        assert np.isfinite(st["sched_err"]) and st["sched_err"] > 0.0
+    ctx.Shutdown()
+
+
+@pytest.mark.gpu
+def test_sched_model_roundtrip_and_order_only():
+    """A model set through the C ABI is read back unchanged, and an arbitrary
+    (even random) model changes only the bin order, never the image."""
+    import softrast_amd as sr
+    base = helpers.random_scene(43, 2500, 400, 300, clip_heavy=True, ndraws=1, shaders=(0,))
+    tex = helpers.build_textures(base)
+    _say("[parity] case=sched_model_roundtrip")
+    ref = helpers.gpu_render(base, tex, tile_order=0)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=2, sched_warmup_frames=1000)
+    rng = np.random.default_rng(5)
+    model = (0.3 * rng.standard_normal(sr.RenderContext.SCHED_PARAMS)).astype(np.float32)
+    ctx.set_sched_model(model)
+    assert np.array_equal(ctx.sched_model(), model)
+    for _ in range(2):
+        out = helpers.gpu_render(base, tex, ctx=ctx)
+        for a, b in zip(ref[:3], out[:3]):
+            assert np.array_equal(a, b)
+        assert out[3]["sched_trained"] == 1
     ctx.Shutdown()
