@@ -103,11 +103,16 @@ def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup
 
 
 def load_traffic(name):
+    """HBM bytes per launch from a committed rocprofv3 summary: the line
+    'traffic_bytes_per_launch = N' (FETCH_SIZE + WRITE_SIZE, calibrated)."""
     try:
         with open(os.path.join(ROOT, "profiles", name)) as f:
-            return json.load(f)
+            for line in f:
+                if line.strip().startswith("traffic_bytes_per_launch"):
+                    return int(line.split("=", 1)[1].strip())
     except (OSError, ValueError):
-        return None
+        pass
+    return None
 
 
 def main():
@@ -144,7 +149,7 @@ def main():
     fe_bytes = 4 * 3 * scene.num_tris + 32 * sum(len(d.vertices) for d in scene.draws)
     be_bytes = 8 * scene.width * scene.height + 16 * covered
     achieved = be_bytes / (be_ms * 1e-3) / 1e9
-    traffic = load_traffic("k_backend_spheres_1080p_1m_traffic.json")
+    traffic = load_traffic("k_backend_spheres_1080p_1m_traffic.md")
 
     # ---- batched path ----
     fps_batched = ms_batched = dev_batched = None
@@ -183,7 +188,7 @@ def main():
         "mtris_per_s": round(scene.num_tris * value / 1e6, 1),
         "roofline": {"bound": "hbm", "kernel": "k_backend", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": be_bytes,
-                     "traffic": traffic.get("bytes_per_launch") if traffic else None},
+                     "traffic": traffic},
         "frame_algorithmic_bytes": fe_bytes + be_bytes,
     }
     if fps_batched:

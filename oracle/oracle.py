@@ -11,6 +11,7 @@ import subprocess
 import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
+REFERENCE_ROOT = "/root/reference"  # reference sources, present only where oracle/_ref is built
 MAX_MIPS = 14
 
 
@@ -87,8 +88,43 @@ def load(fast=False):
     lib.sro_setup_triangle.argtypes = [ctypes.POINTER(f), ctypes.POINTER(f), u32, u32, u32,
                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(f)]
     lib.sro_setup_triangle.restype = i32
+    lib.sro_mip_resize_rgba8.argtypes = [vp, u32, u32, vp, u32, u32]
+    lib.sro_mip_resize_rgba8.restype = i32
     _libs[name] = lib
     return lib
+
+
+def mip_resize(img, ow, oh):
+    """One mip level of an (h, w, 4) uint8 image as the reference makes it
+    (stbir_resize_uint8, Mitchell); returns (oh, ow, 4) uint8."""
+    img = np.ascontiguousarray(img, np.uint8)
+    out = np.zeros((oh, ow, 4), np.uint8)
+    if load().sro_mip_resize_rgba8(img.ctypes.data, img.shape[1], img.shape[0], out.ctypes.data, ow, oh) != 0:
+        raise ValueError("sro_mip_resize_rgba8: unsupported size")
+    return out
+
+
+def load_reference_stbir():
+    """The reference's stb_image_resize compiled as is (oracle/_ref), or None
+    where it was not built (no reference sources on this machine)."""
+    path = HERE / "_ref" / "libstbir.so"
+    if not path.exists():
+        return None
+    lib = ctypes.CDLL(str(path))
+    vp, i = ctypes.c_void_p, ctypes.c_int
+    lib.stbir_resize_uint8.argtypes = [vp, i, i, i, vp, i, i, i, i]
+    lib.stbir_resize_uint8.restype = i
+    return lib
+
+
+def build_reference(quiet=True):
+    """Compiles oracle/_ref from the reference sources when they are present."""
+    if not pathlib.Path(REFERENCE_ROOT, "SoftRast", "stb_image_resize.h").exists():
+        return False
+    res = subprocess.run(["make", "-C", str(HERE), "ref", f"REF={REFERENCE_ROOT}"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("reference stbir build failed:\n" + res.stdout + res.stderr)
+    return True
 
 
 def fptr(a):

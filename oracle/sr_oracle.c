@@ -26,6 +26,9 @@
  *   pack                   SIMDUtil.h:87-121             table of packus saturation cases
  *   shaders                Viewer/Shaders.h:71-130       closed forms
  *   clear / blit           Renderer.cpp:168-194,319-347  closed forms
+ *   mip_resize_rgba8       Texture.cpp:188-198 ->        the vendored stb_image_resize.h compiled
+ *                          stb_image_resize.h:2464-2471  as is (oracle/_ref) and golden vectors made
+ *                                                        from it (tests/golden/stbir_mips.npz)
  * Semantics the reference leaves to its absent `kt` library (op order of
  * Mat4*Vec4, Dot, Lerp; radix-sort stability) are fixed here and marked
  * "defined here"; they have no reference test to pin them.
@@ -1092,4 +1095,144 @@ int sro_render(sro_ctx* ctx, const sro_draw* draws, uint32_t ndraws, sro_plane* 
     }
     if (stats) *stats = st;
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* mip generation: stb_image_resize v0.96 as Texture.cpp:196 calls it  */
+/* ------------------------------------------------------------------ */
+
+/* stbir__filter_mitchell, stb_image_resize.h:824-836: each operation in float. */
+static float stbir_mitchell(float x)
+{
+    x = fabsf(x);
+    if (x < 1.0f) {
+        const float t = 21.0f * x - 36.0f;
+        return (16.0f + (x * x) * t) / 18.0f;
+    }
+    if (x < 2.0f) {
+        float t = 36.0f - 7.0f * x;
+        t = -60.0f + x * t;
+        return (32.0f + x * t) / 18.0f;
+    }
+    return 0.0f;
+}
+
+/* One axis of a downsampling resize: for every input position n of the
+ * margin-extended range [-margin, in + margin) the output range [n0, n1] it
+ * scatters into and its weights (stb_image_resize.h:1023-1035, 1087-1115,
+ * 1117-1192, 1194-1232 for scale <= 1, Mitchell support 2). */
+typedef struct {
+    int margin, count;
+    int* n0;
+    int* n1;
+    float* w; /* 4 per contributor */
+} stbir_axis;
+
+static int stbir_axis_build(stbir_axis* a, uint32_t in, uint32_t out)
+{
+    const float scale = ((float)out / (float)in) / 1.0f;
+    const float radius = 2.0f / scale;
+    const int width = (int)ceil((double)(4.0f / scale));
+    a->margin = width / 2;
+    a->count = (int)in + 2 * a->margin;
+    a->n0 = (int*)malloc(sizeof(int) * a->count);
+    a->n1 = (int*)malloc(sizeof(int) * a->count);
+    a->w = (float*)calloc((size_t)a->count * 4 + 4, sizeof(float));
+    if (!a->n0 || !a->n1 || !a->w) return -1;
+    for (int n = 0; n < a->count; ++n) {
+        const float c = (float)(n - a->margin) + 0.5f;
+        const float lo = (c - radius) * scale - 0.0f;
+        const float hi = (c + radius) * scale - 0.0f;
+        const float oc = c * scale - 0.0f;
+        const int first = (int)floor((double)lo + 0.5);
+        const int last = (int)floor((double)hi - 0.5);
+        if (last - first > 4) return -1;
+        a->n0[n] = first;
+        a->n1[n] = last;
+        float* w = a->w + (size_t)n * 4;
+        /* a fifth weight (last - first == 4) lands in the next group and is
+         * overwritten by it, as in the reference's packed coefficient array */
+        for (int i = 0; i <= last - first; ++i) w[i] = stbir_mitchell(((float)(first + i) + 0.5f) - oc) * scale;
+        for (int i = last - first; i >= 0; --i) {
+            if (w[i] != 0.0f) break;
+            a->n1[n] = first + i - 1;
+        }
+    }
+    /* normalise, per output pixel, over every contributor that reaches it */
+    for (int o = 0; o < (int)out; ++o) {
+        float total = 0.0f;
+        for (int n = 0; n < a->count && o >= a->n0[n]; ++n)
+            if (o <= a->n1[n]) total += a->w[(size_t)n * 4 + (o - a->n0[n])];
+        const float k = 1.0f / total;
+        for (int n = 0; n < a->count && o >= a->n0[n]; ++n)
+            if (o <= a->n1[n]) a->w[(size_t)n * 4 + (o - a->n0[n])] *= k;
+    }
+    /* drop leading zero weights and outputs before 0; clamp to the last output */
+    for (int n = 0; n < a->count; ++n) {
+        float* w = a->w + (size_t)n * 4;
+        int skip = 0;
+        while (skip < 4 && w[skip] == 0.0f) ++skip;
+        a->n0[n] += skip;
+        while (a->n0[n] < 0) {
+            a->n0[n]++;
+            skip++;
+        }
+        const int range = a->n1[n] - a->n0[n] + 1;
+        const int m = range < 4 ? range : 4;
+        for (int i = 0; i < m && i + skip < 4; ++i) w[i] = w[i + skip];
+        if (a->n1[n] > (int)out - 1) a->n1[n] = (int)out - 1;
+    }
+    return 0;
+}
+
+static void stbir_axis_free(stbir_axis* a)
+{
+    free(a->n0);
+    free(a->n1);
+    free(a->w);
+}
+
+int sro_mip_resize_rgba8(const uint8_t* in, uint32_t w, uint32_t h, uint8_t* out, uint32_t ow, uint32_t oh)
+{
+    if (!in || !out || w == 0 || h == 0 || ow == 0 || oh == 0 || ow > w || oh > h) return -1;
+    stbir_axis ax = { 0 }, ay = { 0 };
+    float* hrow = (float*)malloc(sizeof(float) * ow * 4);
+    float* acc = (float*)calloc((size_t)ow * oh * 4, sizeof(float));
+    int rc = (hrow && acc) ? 0 : -1;
+    if (!rc) rc = stbir_axis_build(&ax, w, ow);
+    if (!rc) rc = stbir_axis_build(&ay, h, oh);
+    /* rows in order (stb_image_resize.h:2166-2205): decode with clamped
+     * edges, scatter horizontally into one row, then scatter that row into
+     * the output rows it reaches */
+    for (int yc = 0; !rc && yc < ay.count; ++yc) {
+        if (ay.n0[yc] > ay.n1[yc]) continue;
+        const uint8_t* src = in + (size_t)clampi(yc - ay.margin, 0, (int)h - 1) * w * 4;
+        memset(hrow, 0, sizeof(float) * ow * 4);
+        for (int xc = 0; xc < ax.count; ++xc) {
+            const uint8_t* p = src + (size_t)clampi(xc - ax.margin, 0, (int)w - 1) * 4;
+            float d[4];
+            for (int c = 0; c < 4; ++c) d[c] = (float)p[c] / 255.0f;
+            for (int k = ax.n0[xc]; k <= ax.n1[xc]; ++k) {
+                const float cw = ax.w[(size_t)xc * 4 + (k - ax.n0[xc])];
+                for (int c = 0; c < 4; ++c) hrow[k * 4 + c] += d[c] * cw;
+            }
+        }
+        for (int k = ay.n0[yc]; k <= ay.n1[yc]; ++k) {
+            const float cw = ay.w[(size_t)yc * 4 + (k - ay.n0[yc])];
+            float* dst = acc + (size_t)k * ow * 4;
+            for (uint32_t i = 0; i < ow * 4; ++i) dst[i] += hrow[i] * cw;
+        }
+    }
+    /* encode, stb_image_resize.h:1736-1757 (STBIR__SATURATE_INT undefined) */
+    for (size_t i = 0; !rc && i < (size_t)ow * oh * 4; ++i) {
+        float f = acc[i];
+        if (f < 0.0f) f = 0.0f;
+        if (f > 1.0f) f = 1.0f;
+        out[i] = (uint8_t)(int)((double)(f * 255.0f) + 0.5);
+    }
+    stbir_axis_free(&ax);
+    stbir_axis_free(&ay);
+    free(hrow);
+    free(acc);
+    return rc;
 }

@@ -123,6 +123,13 @@ uint32_t sro_clip_triangle(const float verts[12], const float attribs[24], uint3
 int sro_setup_triangle(const float v[12], const float attribs[24], uint32_t nattr, uint32_t width,
                        uint32_t height, int32_t ints[19], float floats[32]);
 
+/* One mip level as TextureData::CreateFromRGBA8 makes it (Texture.cpp:188-198):
+ * stbir_resize_uint8(base, w, h, 0, out, ow, oh, 0, 4) of the vendored
+ * stb_image_resize v0.96, i.e. Mitchell filter in both axes, clamped edges,
+ * linear colour space, no alpha handling. Downsampling only (ow <= w,
+ * oh <= h); returns 0, or -1 for other sizes. */
+int sro_mip_resize_rgba8(const uint8_t* in, uint32_t w, uint32_t h, uint8_t* out, uint32_t ow, uint32_t oh);
+
 #ifdef __cplusplus
 }
 #endif

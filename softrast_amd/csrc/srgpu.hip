@@ -16,11 +16,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -1042,6 +1044,143 @@ static void tile_texture(const uint8_t* src, uint8_t* dst, uint32_t w, uint32_t 
         }
 }
 
+// ---- mip filter: stbir_resize_uint8 of stb_image_resize v0.96 ----
+// Texture.cpp:196 builds each mip with stbir_resize_uint8(base, w, h, 0, out,
+// mw, mh, 0, 4): Mitchell filter in both axes (scale <= 1), clamped edges,
+// linear colour space, no alpha weighting (stb_image_resize.h:2464-2471).
+// The library scatters every input sample into the outputs it reaches; here
+// each output gathers the same weighted terms in the same order, so every
+// float operation and rounding matches.
+
+// stbir__filter_mitchell (stb_image_resize.h:824-836), float throughout.
+static float mitchell(float x)
+{
+    x = std::fabs(x);
+    if (x < 1.0f) return (16.0f + (x * x) * (21.0f * x - 36.0f)) / 18.0f;
+    if (x < 2.0f) return (32.0f + x * (-60.0f + x * (36.0f - 7.0f * x))) / 18.0f;
+    return 0.0f;
+}
+
+// Per output index o: taps [first[o], first[o+1]) of (clamped input index,
+// weight), in ascending input order.
+struct AxisTaps {
+    std::vector<uint32_t> first;
+    std::vector<uint32_t> src;
+    std::vector<float> weight;
+};
+
+// Contributor ranges and weights (stb_image_resize.h:1023-1035, 1087-1115),
+// per-output normalisation (:1117-1160), zero/edge trimming (:1162-1191).
+static bool mitchell_axis(uint32_t in, uint32_t out, AxisTaps& t)
+{
+    const float scale = ((float)out / (float)in) / 1.0f;
+    const float radius = 2.0f / scale;
+    const int margin = (int)std::ceil((double)(4.0f / scale)) / 2;
+    const int count = (int)in + 2 * margin;
+    std::vector<int> n0(count), n1(count);
+    std::vector<float> w((size_t)count * 4 + 4, 0.0f);
+    for (int n = 0; n < count; ++n) {
+        const float c = (float)(n - margin) + 0.5f;
+        const float oc = c * scale - 0.0f;
+        const int lo = (int)std::floor((double)((c - radius) * scale - 0.0f) + 0.5);
+        const int hi = (int)std::floor((double)((c + radius) * scale - 0.0f) - 0.5);
+        if (hi - lo > 3) return false;  // four taps per input (exact for power-of-two sizes)
+        n0[n] = lo;
+        n1[n] = hi;
+        for (int i = 0; i <= hi - lo; ++i) w[(size_t)n * 4 + i] = mitchell(((float)(lo + i) + 0.5f) - oc) * scale;
+        for (int i = hi - lo; i >= 0 && w[(size_t)n * 4 + i] == 0.0f; --i) n1[n] = lo + i - 1;
+    }
+    for (int o = 0; o < (int)out; ++o) {
+        float sum = 0.0f;
+        for (int n = 0; n < count && o >= n0[n]; ++n)
+            if (o <= n1[n]) sum += w[(size_t)n * 4 + (o - n0[n])];
+        const float inv = 1.0f / sum;
+        for (int n = 0; n < count && o >= n0[n]; ++n)
+            if (o <= n1[n]) w[(size_t)n * 4 + (o - n0[n])] *= inv;
+    }
+    std::vector<std::vector<std::pair<uint32_t, float>>> gather(out);
+    for (int n = 0; n < count; ++n) {
+        float* wn = &w[(size_t)n * 4];
+        int skip = 0;
+        while (skip < 4 && wn[skip] == 0.0f) ++skip;
+        int a = n0[n] + skip;
+        if (a < 0) {
+            skip -= a;
+            a = 0;
+        }
+        const int b = std::min(n1[n], (int)out - 1);
+        const uint32_t srcn = (uint32_t)std::min(std::max(n - margin, 0), (int)in - 1);
+        for (int k = a; k <= b; ++k) {
+            const int i = k - a;
+            // a weight the shift does not reach keeps its unshifted value
+            const float wk = (i + skip < 4) ? wn[i + skip] : wn[i];
+            gather[k].emplace_back(srcn, wk);
+        }
+    }
+    t.first.assign(out + 1, 0);
+    t.src.clear();
+    t.weight.clear();
+    for (uint32_t o = 0; o < out; ++o) {
+        t.first[o] = (uint32_t)t.src.size();
+        for (auto& p : gather[o]) {
+            t.src.push_back(p.first);
+            t.weight.push_back(p.second);
+        }
+    }
+    t.first[out] = (uint32_t)t.src.size();
+    return true;
+}
+
+// Runs f(i) for i in [0, n) on up to 16 host threads.
+extern "C++" template <class F>
+static void parallel_rows(uint32_t n, F f)
+{
+    const uint32_t nt = std::max(1u, std::min({ n, 16u, std::max(1u, std::thread::hardware_concurrency()) }));
+    std::atomic<uint32_t> next{ 0 };
+    auto run = [&]() {
+        for (uint32_t i; (i = next.fetch_add(1)) < n;) f(i);
+    };
+    std::vector<std::thread> pool;
+    for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(run);
+    run();
+    for (auto& th : pool) th.join();
+}
+
+static bool mitchell_resize_rgba8(const uint8_t* in, uint32_t w, uint32_t h, uint8_t* out, uint32_t ow, uint32_t oh)
+{
+    AxisTaps tx, ty;
+    if (!mitchell_axis(w, ow, tx) || !mitchell_axis(h, oh, ty)) return false;
+    float dec[256];
+    for (int i = 0; i < 256; ++i) dec[i] = (float)i / 255.0f;
+    // horizontal pass for every input row the vertical taps name
+    std::vector<float> hrows((size_t)h * ow * 4);
+    parallel_rows(h, [&](uint32_t y) {
+        const uint8_t* src = in + (size_t)y * w * 4;
+        float* dst = &hrows[(size_t)y * ow * 4];
+        for (uint32_t x = 0; x < ow; ++x) {
+            float a[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+            for (uint32_t t = tx.first[x]; t < tx.first[x + 1]; ++t) {
+                const uint8_t* p = src + (size_t)tx.src[t] * 4;
+                const float k = tx.weight[t];
+                for (int c = 0; c < 4; ++c) a[c] += dec[p[c]] * k;
+            }
+            for (int c = 0; c < 4; ++c) dst[x * 4 + c] = a[c];
+        }
+    });
+    // vertical pass and encode (stb_image_resize.h:1736-1757)
+    parallel_rows(oh, [&](uint32_t y) {
+        for (uint32_t x = 0; x < ow; ++x)
+            for (int c = 0; c < 4; ++c) {
+                float a = 0.0f;
+                for (uint32_t t = ty.first[y]; t < ty.first[y + 1]; ++t)
+                    a += hrows[((size_t)ty.src[t] * ow + x) * 4 + c] * ty.weight[t];
+                a = a < 0.0f ? 0.0f : (a > 1.0f ? 1.0f : a);
+                out[((size_t)y * ow + x) * 4 + c] = (uint8_t)(int)((double)(a * 255.0f) + 0.5);
+            }
+    });
+    return true;
+}
+
 int srg_texture_build_rgba8(const uint8_t* linear, uint32_t width, uint32_t height, int calc_mips, uint8_t* out,
                             uint64_t* out_bytes, uint32_t mip_offsets[14], uint32_t* wlog2, uint32_t* hlog2,
                             uint32_t* nmips)
@@ -1072,23 +1211,12 @@ int srg_texture_build_rgba8(const uint8_t* linear, uint32_t width, uint32_t heig
     if (!linear) return SRG_ERR_INVALID;
     std::memset(out, 0, total);
     tile_texture(linear, out, width, height);
-    // Mip m is a box average of the (2^m x 2^m) base-level footprint, rounded
-    // half up, computed from the base level as the reference's chain is.
+    // Every mip is resized from the base level (Texture.cpp:188-198).
     std::vector<uint8_t> tmp;
     for (uint32_t m = 1; m < n; ++m) {
         const uint32_t mw = std::max(1u, width >> m), mh = std::max(1u, height >> m);
-        const uint32_t fx = width / mw, fy = height / mh;
         tmp.assign((size_t)mw * mh * 4, 0);
-        for (uint32_t y = 0; y < mh; ++y)
-            for (uint32_t x = 0; x < mw; ++x)
-                for (uint32_t c = 0; c < 4; ++c) {
-                    uint64_t s = 0;
-                    for (uint32_t yy = 0; yy < fy; ++yy)
-                        for (uint32_t xx = 0; xx < fx; ++xx)
-                            s += linear[(((size_t)(y * fy + yy)) * width + (x * fx + xx)) * 4 + c];
-                    const uint64_t cnt = (uint64_t)fx * fy;
-                    tmp[((size_t)y * mw + x) * 4 + c] = (uint8_t)((s + cnt / 2) / cnt);
-                }
+        if (!mitchell_resize_rgba8(linear, width, height, tmp.data(), mw, mh)) return SRG_ERR_INVALID;
         tile_texture(tmp.data(), out + offs[m], mw, mh);
     }
     return SRG_OK;
