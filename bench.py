@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="profiling aid: time only the batched path (--steps launches) and print its rate")
     return ap.parse_args()
 
 
@@ -115,6 +117,17 @@ def load_traffic(name):
     return None
 
 
+def profile_batched(sr, scene, texs, args):
+    """--no-plain: the batched path alone (for a profiler); prints its rate only."""
+    fps, ms, _, ctx, fbs = time_path(sr, scene, texs, args.views, sr.CFG_TIMING | sr.CFG_GRAPH, args.tile_order,
+                                     args.bin_size, args.steps, args.warmup)
+    for fb in fbs:
+        fb.destroy()
+    ctx.Shutdown()
+    print(json.dumps({"metric": METRIC, "value": round(fps, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "ms_per_step": round(ms, 4), "path": f"batched x{args.views} only"}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -131,6 +144,8 @@ def main():
     print(f"[bench] scene ready: {scene.num_tris} tris, {len(scene.draws)} draws, {time.time() - t0:.1f}s",
           file=sys.stderr, flush=True)
 
+    if args.no_plain:
+        return profile_batched(sr, scene, texs, args)
     # ---- plain path: one view per EndFrame, no graph, natural tile order ----
     fps_plain, ms_plain, st_plain, ctx, fbs = time_path(sr, scene, texs, 1, sr.CFG_TIMING, sr.ORDER_NATURAL,
                                                         args.bin_size, args.steps, args.warmup)

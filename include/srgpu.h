@@ -177,6 +177,20 @@ int srg_texture_build_rgba8(const uint8_t* linear_rgba8, uint32_t width, uint32_
                             uint8_t* out_tiled, uint64_t* out_bytes, uint32_t mip_offsets[14],
                             uint32_t* width_log2, uint32_t* height_log2, uint32_t* num_mips);
 
+/* ---- TextureData::CreateFromRGBA8 on the device (Texture.cpp:119-199) ----
+ * Uploads a linear power-of-two RGBA8 image (>= 32 in both dimensions),
+ * builds the tiled level 0 and, if calc_mips, every mip on the GPU, and
+ * registers the result as a texture. The bytes equal what
+ * srg_texture_build_rgba8 makes (each mip is stbir_resize_uint8 of the base
+ * level, Texture.cpp:196). */
+int srg_texture_create_rgba8(srg_ctx* ctx, const uint8_t* linear_rgba8, uint32_t width, uint32_t height,
+                             int calc_mips, srg_handle* out);
+/* A registered texture's tiled bytes and layout (TextureData's m_texels,
+ * m_mipOffsets, m_widthLog2, m_heightLog2, m_numMips; Texture.h:21-41).
+ * With out == NULL only the layout is returned. */
+int srg_texture_read(srg_ctx* ctx, srg_handle h, uint8_t* out, uint64_t capacity, uint64_t* bytes,
+                     uint32_t mip_offsets[14], uint32_t* width_log2, uint32_t* height_log2, uint32_t* num_mips);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,7 +41,7 @@ SRG_SYMBOLS = (
     "srg_fb_create", "srg_fb_destroy", "srg_fb_write_plane", "srg_fb_swap_planes", "srg_fb_read_tiles",
     "srg_fb_write_tiles", "srg_begin_frame", "srg_clear", "srg_draw_indexed", "srg_end_frame", "srg_blit",
     "srg_get_frame_stats", "srg_get_bin_stats", "srg_sched_get_model", "srg_sched_set_model",
-    "srg_texture_build_rgba8",
+    "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read",
 )
 
 
@@ -101,6 +101,9 @@ def load_library(path=None):
         "srg_update_buffer": (i32, [vp, u64, sz, vp, sz]),
         "srg_release_buffer": (i32, [vp, u64]),
         "srg_register_texture": (i32, [vp, vp, u64, ctypes.POINTER(u32), u32, u32, u32, pu64]),
+        "srg_texture_create_rgba8": (i32, [vp, vp, u32, u32, i32, pu64]),
+        "srg_texture_read": (i32, [vp, u64, vp, u64, pu64, ctypes.POINTER(u32), ctypes.POINTER(u32),
+                                   ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "srg_release_texture": (i32, [vp, u64]),
         "srg_fb_create": (i32, [vp, u32, u32, u32, ctypes.POINTER(vp)]),
         "srg_fb_destroy": (None, [vp, vp]),
@@ -182,6 +185,37 @@ class TextureData:
 
     def Clear(self):
         self.m_texels = np.zeros(0, np.uint8)
+
+
+class DeviceTexture:
+    """A texture built and kept on the device: TextureData::CreateFromRGBA8
+    run on the GPU (srg_texture_create_rgba8). Usable wherever a TextureData
+    is, as the UnlitDiffuse uniforms of a draw of the same context."""
+
+    def __init__(self, ctx, texels, width, height, calcMips=True):
+        src = np.ascontiguousarray(texels, dtype=np.uint8).reshape(-1)
+        if src.size != width * height * 4:
+            raise ValueError("texel array does not match width*height*4")
+        out = ctypes.c_uint64()
+        _check(ctx._lib.srg_texture_create_rgba8(ctx._ctx, _ptr(src), width, height, int(calcMips), ctypes.byref(out)),
+               "texture_create_rgba8")
+        self._ctx, self.handle = ctx, out.value
+
+    def read(self):
+        """Host copy as a TextureData (tiled bytes and layout)."""
+        lib, c = self._ctx._lib, self._ctx._ctx
+        nbytes = ctypes.c_uint64()
+        offs = (ctypes.c_uint32 * 14)()
+        wl, hl, nm = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib.srg_texture_read(c, self.handle, None, 0, ctypes.byref(nbytes), offs, ctypes.byref(wl),
+                                    ctypes.byref(hl), ctypes.byref(nm)), "texture_read")
+        data = np.zeros(nbytes.value, np.uint8)
+        _check(lib.srg_texture_read(c, self.handle, _ptr(data), data.nbytes, ctypes.byref(nbytes), offs,
+                                    ctypes.byref(wl), ctypes.byref(hl), ctypes.byref(nm)), "texture_read")
+        t = TextureData()
+        t.m_texels, t.m_mipOffsets = data, np.array(list(offs), np.uint32)
+        t.m_widthLog2, t.m_heightLog2, t.m_numMips, t.m_bytesPerPixel = wl.value, hl.value, nm.value, 4
+        return t
 
 
 # ---------------------------------------------------------------------------
@@ -347,6 +381,10 @@ class RenderContext:
     def texture_handle(self, tex):
         if tex is None:
             return 0
+        if isinstance(tex, DeviceTexture):
+            if tex._ctx is not self:
+                raise SrgError(-1, "DeviceTexture belongs to another RenderContext")
+            return tex.handle
         key = id(tex)
         h = self._textures.get(key)
         if h is None:
@@ -377,7 +415,7 @@ class RenderContext:
         shader = call.m_pixelShader if call.m_pixelShader is not None else VisualizeNormalsShader
         d.shader = shader.id
         d.texture = self.texture_handle(call.m_pixelUniforms) if shader.id == SHADER_UNLIT_DIFFUSE else 0
-        if call.m_pixelUniforms is not None and shader.id == SHADER_UNLIT_DIFFUSE and \
+        if isinstance(call.m_pixelUniforms, TextureData) and shader.id == SHADER_UNLIT_DIFFUSE and \
                 call.m_pixelUniforms.m_texels.size == 0:
             d.texture = 0
         for name, buf in (("index", call.m_indexBuffer), ("position", call.m_positionBuffer),
@@ -422,6 +460,10 @@ class RenderContext:
         return counts, costs
 
     SCHED_PARAMS = 1249
+
+    def CreateTexture(self, texels, width, height, calcMips=True):
+        """TextureData::CreateFromRGBA8 on the device; see DeviceTexture."""
+        return DeviceTexture(self, texels, width, height, calcMips)
 
     def sched_model(self):
         """Learned tile-cost model parameters (the scheduler's checkpoint)."""

@@ -4,13 +4,13 @@
 // k_backend<S> is persistent: each workgroup (4 waves) takes S x S bins from
 // the frame's queue. A 64-bit visibility key per pixel lives in LDS. Waves
 // stride over the bin's entries 64 at a time, each lane loading one
-// primitive record so the loads overlap, then walk the entries one by one
-// with the record broadcast from its lane; an 8x8 block is evaluated with
-// one pixel per lane (RasterizeTrisInBin_OutputFragments and
-// ComputeBlockMask8x8, Rasterizer.cpp:97-304). Then each lane resolves its
-// pixels (the key names the fragment the reference would have shaded last)
-// and shades them (ComputeInterpolantsDrawCallImpl :306-419, Shaders.h,
-// Texture.cpp), writing RGBA8 and depth once.
+// primitive record so the loads overlap; the entries' 8x8 blocks then form a
+// queue that groups of 8 lanes take one block at a time, a lane per pixel
+// column (RasterizeTrisInBin_OutputFragments and ComputeBlockMask8x8,
+// Rasterizer.cpp:97-304). Then each lane resolves its pixels (the key names
+// the fragment the reference would have shaded last) and shades them
+// (ComputeInterpolantsDrawCallImpl :306-419, Shaders.h, Texture.cpp),
+// writing RGBA8 and depth once.
 #pragma once
 #include "sr_frontend.h"
 
@@ -18,7 +18,6 @@ namespace srg {
 
 constexpr uint32_t kBackendThreads = 256;
 constexpr uint32_t kBackendWaves = kBackendThreads / 64;
-constexpr uint32_t kSmallBlocks = 4;  // primitives touching at most this many 8x8 blocks of a bin go lane-private
 
 // The reference keeps a fragment iff z > 0 and z > D (ordered compare) and
 // shades every kept fragment in submission order, so the final pixel is the
@@ -131,11 +130,28 @@ __device__ __forceinline__ uint32_t block_class(const uint32_t ct[3], const uint
     return reject ? 0u : (full ? 2u : 1u);
 }
 
+// LDS slot of the key of bin-local pixel (x, y): rows of S keys, and inside
+// each group of 16 keys of a row the slot is rotated by 2*(y>>3) + (x>>4) +
+// 8*(y&1). Keys are 8 bytes, so 16 consecutive keys cover the LDS banks once.
+// Without the rotation the 64 lanes of a wave hit the same few banks: one
+// block per wave puts its 8 rows on the same banks, and one block per lane
+// puts the same pixel of every lane's block on the same bank. With it, a
+// block's 64 pixels fall on 16 bank pairs 4 apiece and the same pixel of the
+// 16 blocks of a 32x32 bin on 16 different bank pairs.
+template <uint32_t S>
+__device__ __forceinline__ uint32_t key_slot(uint32_t x, uint32_t y)
+{
+    const uint32_t rot = 2u * (y >> 3) + (x >> 4) + 8u * (y & 1u);
+    return y * S + (x & ~15u) + ((x + rot) & 15u);
+}
+
 // One primitive's tile-relative setup (Binning.cpp:423-451 re-basing) and its
 // block range inside the work bin at (sx0, sy0).
 struct EntryRaster {
     uint32_t ct[3], dxu[3], dyu[3];
     uint32_t xs, xe, ys, ye;
+    uint32_t cx0, cx1, cy0, cy1;  // tight primitives: pixels that can be covered, tile-relative, ends exclusive
+    bool tight;
     float zdx, zdy, zc0;
     unsigned long long low;
 };
@@ -149,13 +165,22 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
     const uint32_t c[3] = { q0.x, q0.y, q0.z };
 #pragma unroll
     for (int k = 0; k < 3; ++k) e.ct[k] = c[k] + (e.dxu[k] * (uint32_t)Y0 + e.dyu[k] * (uint32_t)X0);
-    const int32_t xmin = q2.y & 0xFFFF, xmax = q2.y >> 16, ymin = q2.z & 0xFFFF, ymax = q2.z >> 16;
+    const int32_t xmin = q2.y & 0x3FFF, xmax = (q2.y >> 16) & 0x3FFF, ymin = q2.z & 0x3FFF, ymax = (q2.z >> 16) & 0x3FFF;
     const uint32_t bminx = (uint32_t)clampi(xmin - X0, 0, 64), bmaxx = (uint32_t)clampi(xmax - X0, 0, 64);
     const uint32_t bminy = (uint32_t)clampi(ymin - Y0, 0, 64), bmaxy = (uint32_t)clampi(ymax - Y0, 0, 64);
     e.xs = max(bminx & ~7u, sx0);
     e.xe = min(bmaxx, sx0 + S);
     e.ys = max(bminy & ~7u, sy0);
     e.ye = min(bmaxy, sy0 + S);
+    // walked pixels (whole blocks from xs while xb < xe) within the tight bound
+    e.tight = ((q2.y >> 14) & 1u) != 0;
+    const int32_t wx1 = e.xe > e.xs ? (int32_t)(e.xs + ((e.xe - e.xs + 7u) & ~7u)) : (int32_t)e.xs;
+    const int32_t wy1 = e.ye > e.ys ? (int32_t)(e.ys + ((e.ye - e.ys + 7u) & ~7u)) : (int32_t)e.ys;
+    e.cx0 = (uint32_t)max((int32_t)e.xs, xmin - (int32_t)((q2.y >> 15) & 1u) - X0);
+    e.cy0 = (uint32_t)max((int32_t)e.ys, ymin - (int32_t)((q2.z >> 14) & 1u) - Y0);
+    const int32_t bx1 = ((q2.y >> 30) & 1u) ? wx1 : xmax - X0 + 1, by1 = ((q2.z >> 30) & 1u) ? wy1 : ymax - Y0 + 1;
+    e.cx1 = (uint32_t)max((int32_t)e.cx0, min(wx1, bx1));
+    e.cy1 = (uint32_t)max((int32_t)e.cy0, min(wy1, by1));
     const float r0x = ffrom(q2.w), r0y = ffrom(q3.x);
     e.zdx = ffrom(q3.y);
     e.zdy = ffrom(q3.z);
@@ -165,46 +190,92 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
     return e;
 }
 
-// Lane-private rasterisation of one small primitive: its blocks one after
-// another, 8 rows x 8 columns per block with the reference's arithmetic
-// (ComputeBlockMask8x8, Rasterizer.cpp:134-192): edge values stepped by dx per
-// row, depth per column as fma(yb, dy, fma(l, dx, c0)) + xb*dx then += dy per row.
+// Rasterisation of a wave's 64 entries (one per lane, `e`, `nb` walked 8x8
+// blocks each). The (entry, block) pairs form a queue that the wave takes 8
+// at a time: lanes 8g..8g+7 take one pair, one lane per pixel column, and
+// walk the block's rows, carrying the reference's per-column depth down the
+// rows (ComputeBlockMask8x8, Rasterizer.cpp:134-192): z = fma(yb, dy,
+// fma(l, dx, c0)) + xb*dx at the block's top row, then + dy per row, and the
+// edge values + dx_k per row.
+//
+// A block the reference rejects by its 64-wide corner test is skipped and a
+// fully accepted one is covered without per-pixel tests (Rasterizer.cpp:
+// 230-272). For a tight primitive (tight_bound in sr_frontend.h) only the
+// rows and columns of its bound are visited, and the corner test is needed
+// only where some edge value is exactly 0: with exact edge values a rejected
+// block has no pixel with all three values positive, and a fully accepted
+// block no pixel with any negative.
 template <uint32_t S>
-__device__ __forceinline__ void raster_small(unsigned long long* keys, const EntryRaster& e, uint32_t sx0, uint32_t sy0,
-                                             unsigned long long& covered_count)
+__device__ __forceinline__ void raster_wave(unsigned long long* keys, const EntryRaster& e, uint32_t nb, uint32_t sx0,
+                                            uint32_t sy0, bool stats, unsigned long long& frags)
 {
-    float ztop[8];
+    const uint32_t lane = lane_id(), g = lane >> 3, l = lane & 7u;
+    uint32_t incl = nb;
 #pragma unroll
-    for (uint32_t l = 0; l < 8; ++l) ztop[l] = __builtin_fmaf((float)l, e.zdx, e.zc0);
-    for (uint32_t yb = e.ys; yb < e.ye; yb += 8)
-        for (uint32_t xb = e.xs; xb < e.xe; xb += 8) {
-            const uint32_t cls = block_class(e.ct, e.dxu, e.dyu, xb, yb);
-            if (cls == 0u) continue;
-            const bool full = cls == 2u;
-            uint32_t er[3];
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    const uint32_t total = bcast(incl, 63);
+    // per-entry values the block workers need, packed
+    const uint32_t nbx = e.xe > e.xs ? (e.xe - e.xs + 7u) >> 3 : 1u;
+    const uint32_t magic = (1024u + nbx - 1u) / nbx;  // blk / nbx == (blk * magic) >> 10 for blk < 64
+    const uint32_t pk_blk = e.xs | (e.ys << 7) | (nbx << 14) | ((e.tight ? 1u : 0u) << 18) | (magic << 19);
+    const uint32_t pk_cand = e.cx0 | (e.cx1 << 8) | (e.cy0 << 16) | (e.cy1 << 24);
+    const uint32_t first = incl - nb;
+    const uint32_t low = (uint32_t)e.low;
+    for (uint32_t base = 0; base < total; base += 8) {
+        uint32_t owner = 0;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) er[k] = e.ct[k] + e.dyu[k] * xb + e.dxu[k] * yb;
-            float z[8];
-            const float xoff = (float)xb * e.zdx;
-#pragma unroll
-            for (uint32_t l = 0; l < 8; ++l) z[l] = __builtin_fmaf((float)yb, e.zdy, ztop[l]) + xoff;
-            unsigned long long* row = keys + (yb - sy0) * S + (xb - sx0);
-#pragma unroll
-            for (uint32_t r = 0; r < 8; ++r) {
-#pragma unroll
-                for (uint32_t l = 0; l < 8; ++l) {
-                    const bool cov = full || (int32_t)((er[0] + e.dyu[0] * l) | (er[1] + e.dyu[1] * l) |
-                                                       (er[2] + e.dyu[2] * l)) >= 0;
-                    covered_count += cov ? 1u : 0u;
-                    if (cov && z[l] > 0.0f) atomicMax(&row[l], ((unsigned long long)fbits(z[l]) << 32) | e.low);
-                }
-#pragma unroll
-                for (int k = 0; k < 3; ++k) er[k] += e.dxu[k];
-#pragma unroll
-                for (uint32_t l = 0; l < 8; ++l) z[l] = z[l] + e.zdy;
-                row += S;
-            }
+        for (uint32_t gg = 0; gg < 8; ++gg) {
+            const uint32_t o = (uint32_t)__popcll(__ballot(incl <= base + gg));
+            owner = g == gg ? o : owner;
         }
+        const uint32_t it = base + g;
+        const bool valid = it < total;
+        const int src = (int)(valid ? owner : 0u);
+        uint32_t ct[3], dxu[3], dyu[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            ct[k] = __shfl(e.ct[k], src, 64);
+            dxu[k] = __shfl(e.dxu[k], src, 64);
+            dyu[k] = __shfl(e.dyu[k], src, 64);
+        }
+        const uint32_t pb = __shfl(pk_blk, src, 64), pc = __shfl(pk_cand, src, 64);
+        const float zdx = __shfl(e.zdx, src, 64), zdy = __shfl(e.zdy, src, 64), zc0 = __shfl(e.zc0, src, 64);
+        const unsigned long long lo = (unsigned long long)__shfl(low, src, 64);
+        const uint32_t blk = it - __shfl(first, src, 64);
+        const uint32_t xs = pb & 127u, ys = (pb >> 7) & 127u, bnx = (pb >> 14) & 15u;
+        const bool tight = ((pb >> 18) & 1u) != 0;
+        const uint32_t by = (blk * (pb >> 19)) >> 10, bx = blk - by * bnx;
+        const uint32_t xb = xs + 8u * bx, yb = ys + 8u * by, x = xb + l;
+        const uint32_t cx0 = pc & 255u, cx1 = (pc >> 8) & 255u, cy0 = (pc >> 16) & 255u, cy1 = pc >> 24;
+        const bool col = valid && x >= cx0 && x < cx1;
+        const uint32_t r_lo = cy0 > yb ? cy0 - yb : 0u;
+        const uint32_t r_hi = cy1 > yb ? min(cy1 - yb, 8u) : 0u;
+        uint32_t cls = 1u;
+        if (valid && !tight) cls = block_class(ct, dxu, dyu, xb, yb);
+        float z = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
+        uint32_t E[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) E[k] = ct[k] + dyu[k] * x + dxu[k] * yb;
+        for (uint32_t r = 0; r < r_lo; ++r) {
+            z = z + zdy;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) E[k] += dxu[k];
+        }
+        const uint32_t sxl = x - sx0;
+        for (uint32_t r = r_lo; r < r_hi; ++r) {
+            bool cov = col && cls != 0u && (cls == 2u || (int32_t)(E[0] | E[1] | E[2]) >= 0);
+            if (cov && tight && min(min(E[0], E[1]), E[2]) == 0u) cov = block_class(ct, dxu, dyu, xb, yb) != 0u;
+            if (stats) frags += cov ? 1u : 0u;
+            if (cov && z > 0.0f)
+                atomicMax(&keys[key_slot<S>(sxl, yb + r - sy0)], ((unsigned long long)fbits(z) << 32) | lo);
+            z = z + zdy;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) E[k] += dxu[k];
+        }
+    }
 }
 
 template <uint32_t S>
@@ -219,7 +290,6 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
     __shared__ uint32_t s_q;
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
-    const uint32_t l = lane & 7u, rr = lane >> 3;
     unsigned long long frags = 0;
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
     const uint32_t qlen = F.counters[C_QLEN];
@@ -244,7 +314,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
 
         for (uint32_t p = tid; p < NPIX; p += kBackendThreads) {
             const uint32_t x = sx0 + p % S, y = sy0 + p / S;
-            keys[p] = clear_d ? 0ull : init_key(V.depth[tofs + y * kBin + x]);
+            keys[key_slot<S>(p % S, p / S)] = clear_d ? 0ull : init_key(V.depth[tofs + y * kBin + x]);
         }
         __syncthreads();
 
@@ -265,54 +335,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
             const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S);
             const uint32_t nbx = e.xe > e.xs ? (e.xe - e.xs + 7u) >> 3 : 0u;
             const uint32_t nby = e.ye > e.ys ? (e.ye - e.ys + 7u) >> 3 : 0u;
-            const bool small = has && nbx * nby <= kSmallBlocks;
-            // small primitives: one per lane
-            if (small) {
-                if (stats) raster_small<S>(keys, e, sx0, sy0, frags);
-                else {
-                    unsigned long long unused = 0;
-                    raster_small<S>(keys, e, sx0, sy0, unused);
-                }
-            }
-            // large primitives: the whole wave on one 8x8 block at a time, one pixel per lane
-            unsigned long long lm = __ballot(has && !small);
-            while (lm) {
-                const uint32_t j = (uint32_t)(__ffsll((long long)lm) - 1);
-                lm &= lm - 1;
-                uint32_t ct[3], dxu[3], dyu[3];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    ct[k] = bcast(e.ct[k], j);
-                    dxu[k] = bcast(e.dxu[k], j);
-                    dyu[k] = bcast(e.dyu[k], j);
-                }
-                const uint32_t xs = bcast(e.xs, j), xe = bcast(e.xe, j), ys = bcast(e.ys, j), ye = bcast(e.ye, j);
-                const float zdx = ffrom(bcast(fbits(e.zdx), j)), zdy = ffrom(bcast(fbits(e.zdy), j));
-                const float zc0 = ffrom(bcast(fbits(e.zc0), j));
-                const unsigned long long low = 0xFFFFFFFEull - bcast(0xFFFFFFFEu - (uint32_t)e.low, j);
-                const float ztop = __builtin_fmaf((float)l, zdx, zc0);
-                for (uint32_t yb = ys; yb < ye; yb += 8)
-                    for (uint32_t xb = xs; xb < xe; xb += 8) {
-                        const uint32_t cls = block_class(ct, dxu, dyu, xb, yb);
-                        if (cls == 0u) continue;
-                        const uint32_t x = xb + l, y = yb + rr;
-                        bool covered = cls == 2u;
-                        if (!covered) {
-                            const uint32_t e0 = ct[0] + dyu[0] * x + dxu[0] * y;
-                            const uint32_t e1 = ct[1] + dyu[1] * x + dxu[1] * y;
-                            const uint32_t e2 = ct[2] + dyu[2] * x + dxu[2] * y;
-                            covered = (int32_t)(e0 | e1 | e2) >= 0;
-                        }
-                        // depth: row 0 of the block, then += dy per row (ComputeBlockMask8x8)
-                        float z = __builtin_fmaf((float)yb, zdy, ztop) + (float)xb * zdx;
-#pragma unroll
-                        for (uint32_t i = 0; i < 7; ++i)
-                            if (i < rr) z = z + zdy;
-                        if (stats) frags += covered ? 1u : 0u;
-                        if (covered && z > 0.0f)
-                            atomicMax(&keys[(y - sy0) * S + (x - sx0)], ((unsigned long long)fbits(z) << 32) | low);
-                    }
-            }
+            raster_wave<S>(keys, e, has ? nbx * nby : 0u, sx0, sy0, stats, frags);
         }
         __syncthreads();
 
@@ -323,7 +346,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
 #pragma unroll
             for (uint32_t i = 0; i < BATCH; ++i) {
                 const uint32_t p = tid + (p0 + i) * kBackendThreads;
-                kk[i] = keys[p];
+                kk[i] = keys[key_slot<S>(p % S, p / S)];
                 const uint32_t lo = (uint32_t)kk[i];
                 win[i] = lo != 0xFFFFFFFFu && kk[i] != 0ull;
                 rec[i] = win[i] ? record_of_order(F, 0xFFFFFFFEu - lo) : 0u;

@@ -15,6 +15,7 @@ namespace srg {
 struct Prim {
     int32_t c[3], dx[3], dy[3];
     int32_t xmin, xmax, ymin, ymax;
+    uint32_t tight;  // bit 0: tight bound holds; bits 1, 2: the bound starts one pixel left of xmin / above ymin
     float r0x, r0y, zdx, zdy, zf0, wdx, wdy, wf0;
     float adx[kMaxSlots], ady[kMaxSlots], af0[kMaxSlots];
 };
@@ -70,6 +71,32 @@ __device__ __forceinline__ void setup_plane(float cterm, float d10x, float d10y,
 
 __device__ __forceinline__ int32_t clampi(int32_t v, int32_t lo, int32_t hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// Which pixels can the reference's coverage test accept? It accepts pixel
+// (x, y) iff E_k = ((c64_k + b_k) >> 8) + dy_k*x + dx_k*y >= 0 for the three
+// edges (int32, wrapping), b_k in {0, 256} the fill bias. Without wrap this
+// is E'_k(P) + b_k >= 0 for the exact edge function E' at P = 256*(x, y),
+// and E'_0 + E'_1 + E'_2 = A, the doubled area (A >= 256 after the cull). So
+// the barycentric coordinates of an accepted P are >= -256/A, and P lies
+// within 2*256*W/A (W = x extent of the vertices) of the vertex x range. With
+// A > 2*max(W, H) that is less than one subpixel short of 256: every
+// accepted pixel has mnx >> 8 <= x <= ceil(mxx / 256) (same for y). No wrap:
+// W, H < 2^18 keeps |E'| < 2^38 over the 64-pixel block extents used, below
+// the 2^39 at which E would wrap. The blocks the reference walks may reach
+// past the last column or row into the tile's padding, so where the bbox
+// maximum was clamped to the framebuffer the bound stays open on that side.
+// Returns the record's tight bits: bit 0 valid, bit 1 the x bound starts at
+// xmin - 1, bit 2 the y bound at ymin - 1, bit 3 / 4 no x / y upper bound.
+__device__ __forceinline__ uint32_t tight_bound(int64_t area2, int32_t mnx, int32_t mxx, int32_t mny, int32_t mxy,
+                                                int32_t xmin, int32_t ymin, int32_t xmax, int32_t ymax)
+{
+    const int64_t w = (int64_t)mxx - mnx, h = (int64_t)mxy - mny;
+    if (w >= (1 << 18) || h >= (1 << 18) || area2 <= 2 * (w > h ? w : h)) return 0u;
+    const int32_t dxl = xmin - max(mnx >> 8, 0), dyl = ymin - max(mny >> 8, 0);
+    if ((uint32_t)dxl > 1u || (uint32_t)dyl > 1u) return 0u;
+    const bool xopen = (((int64_t)mxx + 255) >> 8) > xmax, yopen = (((int64_t)mxy + 255) >> 8) > ymax;
+    return 1u | ((uint32_t)dxl << 1) | ((uint32_t)dyl << 2) | (xopen ? 8u : 0u) | (yopen ? 16u : 0u);
+}
+
 // Viewport, snap, cull, bbox, edges and planes (Binning.cpp:291-350) for the
 // kept attribute slots. Returns false if the triangle is culled.
 __device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const float a[3][kMaxSlots], uint32_t nslots,
@@ -84,10 +111,9 @@ __device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const flo
     const int32_t f1x = cvtt_i32(r1x * 256.0f + 0.5f), f1y = cvtt_i32(r1y * 256.0f + 0.5f);
     const int32_t f2x = cvtt_i32(r2x * 256.0f + 0.5f), f2y = cvtt_i32(r2y * 256.0f + 0.5f);
 
-    int64_t area = (int64_t)(int32_t)((uint32_t)f2x - (uint32_t)f0x) * (int64_t)(int32_t)((uint32_t)f1y - (uint32_t)f0y) -
-                   (int64_t)(int32_t)((uint32_t)f2y - (uint32_t)f0y) * (int64_t)(int32_t)((uint32_t)f1x - (uint32_t)f0x);
-    area >>= 8;
-    if (area <= 0) return false;
+    const int64_t area2 = (int64_t)(int32_t)((uint32_t)f2x - (uint32_t)f0x) * (int64_t)(int32_t)((uint32_t)f1y - (uint32_t)f0y) -
+                          (int64_t)(int32_t)((uint32_t)f2y - (uint32_t)f0y) * (int64_t)(int32_t)((uint32_t)f1x - (uint32_t)f0x);
+    if ((area2 >> 8) <= 0) return false;
 
     const int32_t mnx = min(min(f0x, f1x), f2x), mny = min(min(f0y, f1y), f2y);
     const int32_t mxx = max(max(f0x, f1x), f2x), mxy = max(max(f0y, f1y), f2y);
@@ -95,6 +121,7 @@ __device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const flo
     s.ymin = (uint16_t)clampi((int32_t)((uint32_t)mny + 255u) >> 8, 0, (int32_t)height - 1);
     s.xmax = (uint16_t)clampi((int32_t)((uint32_t)mxx + 255u) >> 8, 0, (int32_t)width - 1);
     s.ymax = (uint16_t)clampi((int32_t)((uint32_t)mxy + 255u) >> 8, 0, (int32_t)height - 1);
+    s.tight = tight_bound(area2, mnx, mxx, mny, mxy, s.xmin, s.ymin, s.xmax, s.ymax);
 
     setup_edge(s, 0, f0x, f0y, f1x, f1y);
     setup_edge(s, 1, f1x, f1y, f2x, f2y);
@@ -219,8 +246,13 @@ __device__ __forceinline__ void write_record(const FrameDev& F, uint32_t r, cons
     uint4* R = F.rec_raster + (size_t)r * 4;
     R[0] = make_uint4((uint32_t)s.c[0], (uint32_t)s.c[1], (uint32_t)s.c[2], (uint32_t)s.dx[0]);
     R[1] = make_uint4((uint32_t)s.dx[1], (uint32_t)s.dx[2], (uint32_t)s.dy[0], (uint32_t)s.dy[1]);
-    R[2] = make_uint4((uint32_t)s.dy[2], (uint32_t)s.xmin | ((uint32_t)s.xmax << 16),
-                      (uint32_t)s.ymin | ((uint32_t)s.ymax << 16), fbits(s.r0x));
+    // bbox fields: bits 0-13 min, 16-29 max; tight bits: x field 14 valid, 15 min - 1,
+    // 30 open above; y field 14 min - 1, 30 open above
+    R[2] = make_uint4((uint32_t)s.dy[2],
+                      (uint32_t)s.xmin | ((s.tight & 3u) << 14) | ((uint32_t)s.xmax << 16) | (((s.tight >> 3) & 1u) << 30),
+                      (uint32_t)s.ymin | (((s.tight >> 2) & 1u) << 14) | ((uint32_t)s.ymax << 16) |
+                          (((s.tight >> 4) & 1u) << 30),
+                      fbits(s.r0x));
     R[3] = make_uint4(fbits(s.r0y), fbits(s.zdx), fbits(s.zdy), fbits(s.zf0));
     float4* S = F.rec_shade + (size_t)r * 4;
     S[0] = make_float4(s.wdx, s.wdy, s.wf0, s.adx[0]);
@@ -334,30 +366,44 @@ __device__ __forceinline__ float clip_dot(int plane, float4 v)
 }
 __device__ __forceinline__ float lerp_kt(float a, float b, float t) { return a + (b - a) * t; }
 
+// The clipper's two polygon buffers (up to 9 vertices of 4 position floats
+// and kMaxSlots attribute floats) of every lane of a one-wave workgroup, in
+// LDS: component-major with the lane innermost, so a wave's accesses to the
+// same component are conflict-free.
+constexpr uint32_t kClipThreads = 64;
+constexpr uint32_t kClipComps = 4 + kMaxSlots;
 struct ClipPoly {
-    float4 v[2][9];
-    float a[2][9][kMaxSlots];
+    float* lds;
+    uint32_t lane;
+    __device__ __forceinline__ float& at(uint32_t buf, uint32_t vtx, uint32_t comp) const
+    {
+        return lds[((buf * 9u + vtx) * kClipComps + comp) * kClipThreads + lane];
+    }
+    __device__ __forceinline__ float4 pos(uint32_t buf, uint32_t vtx) const
+    {
+        return make_float4(at(buf, vtx, 0), at(buf, vtx, 1), at(buf, vtx, 2), at(buf, vtx, 3));
+    }
 };
 
 // ClipPlane (Binning.cpp:85-165) over the set planes of `mask`, lowest first,
 // stopping when the polygon empties (:517-523). Returns the vertex count; the
 // result is in buffer `pin`.
-__device__ __noinline__ uint32_t clip_polygon(ClipPoly& poly, uint32_t mask, uint32_t& pin)
+__device__ __forceinline__ uint32_t clip_polygon(const ClipPoly& poly, uint32_t mask, uint32_t& pin)
 {
     uint32_t n = 3;
     pin = 0;
     do {
         const int plane = __builtin_ctz(mask);
         mask ^= 1u << plane;
+        const uint32_t po = pin ^ 1u;
         uint32_t nout = 0;
         uint32_t i0 = n - 1;
-        float d0 = clip_dot(plane, poly.v[pin][i0]);
+        float d0 = clip_dot(plane, poly.pos(pin, i0));
         for (uint32_t i1 = 0; i1 < n; ++i1) {
-            const float d1 = clip_dot(plane, poly.v[pin][i1]);
+            const float d1 = clip_dot(plane, poly.pos(pin, i1));
             const bool in0 = d0 >= 0.0f, in1 = d1 >= 0.0f;
             if (in0) {
-                poly.v[pin ^ 1][nout] = poly.v[pin][i0];
-                for (uint32_t t = 0; t < kMaxSlots; ++t) poly.a[pin ^ 1][nout][t] = poly.a[pin][i0][t];
+                for (uint32_t c = 0; c < kClipComps; ++c) poly.at(po, nout, c) = poly.at(pin, i0, c);
                 ++nout;
             }
             if (in0 != in1) {
@@ -365,18 +411,15 @@ __device__ __noinline__ uint32_t clip_polygon(ClipPoly& poly, uint32_t mask, uin
                 float tt;
                 if (in1) { tt = d1 / (d1 - d0); ia = i1; ib = i0; }
                 else     { tt = d0 / (d0 - d1); ia = i0; ib = i1; }
-                const float4 pa = poly.v[pin][ia], pb = poly.v[pin][ib];
-                poly.v[pin ^ 1][nout] = make_float4(lerp_kt(pa.x, pb.x, tt), lerp_kt(pa.y, pb.y, tt),
-                                                    lerp_kt(pa.z, pb.z, tt), lerp_kt(pa.w, pb.w, tt));
-                for (uint32_t t = 0; t < kMaxSlots; ++t)
-                    poly.a[pin ^ 1][nout][t] = lerp_kt(poly.a[pin][ia][t], poly.a[pin][ib][t], tt);
+                for (uint32_t c = 0; c < kClipComps; ++c)
+                    poly.at(po, nout, c) = lerp_kt(poly.at(pin, ia, c), poly.at(pin, ib, c), tt);
                 ++nout;
             }
             d0 = d1;
             i0 = i1;
         }
         n = nout;
-        pin ^= 1;
+        pin = po;
     } while (mask && n);
     return n;
 }
@@ -392,18 +435,19 @@ __device__ __forceinline__ uint32_t draw_of_triangle(const FrameDev& F, uint32_t
     return lo;
 }
 
-// Slow path: triangles that cross a clip plane (BinTrisEntry's clip branch,
-// Binning.cpp:502-533). Each fan triangle gets a record after the
-// per-triangle slots; its order key is (g << 3) | fan index.
-__global__ __launch_bounds__(256) void k_clip(FrameDev F)
+// Slow path: the queued triangles that cross a clip plane, one per lane of
+// one-wave workgroups; the fan triangles (0, i-1, i) (Binning.cpp:525-533) get
+// records after the submitted triangles' and bin counts like any other.
+__global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
 {
+    __shared__ float s_poly[2 * 9 * kClipComps * kClipThreads];
     const uint32_t n = F.counters[C_CLIPQ];
     const uint32_t stride = gridDim.x * blockDim.x;
+    const ClipPoly poly{ s_poly, threadIdx.x };
     for (uint32_t base_i = blockIdx.x * blockDim.x; base_i < n; base_i += stride) {
         const uint32_t qi = base_i + threadIdx.x;
         const bool active = qi < n;
         uint32_t g = 0, d = 0, nfan = 0, pin = 0;
-        ClipPoly poly;
         if (active) {
             g = F.clip_queue[qi];
             d = draw_of_triangle(F, g);
@@ -412,9 +456,12 @@ __global__ __launch_bounds__(256) void k_clip(FrameDev F)
             float a[3][kMaxSlots];
             fetch_triangle(F, D, g - D.tri_base, v, a);
             const uint32_t mor = clip_codes(v[0]) | clip_codes(v[1]) | clip_codes(v[2]);
-            for (int k = 0; k < 3; ++k) {
-                poly.v[0][k] = v[k];
-                for (uint32_t t = 0; t < kMaxSlots; ++t) poly.a[0][k][t] = a[k][t];
+            for (uint32_t k = 0; k < 3; ++k) {
+                poly.at(0, k, 0) = v[k].x;
+                poly.at(0, k, 1) = v[k].y;
+                poly.at(0, k, 2) = v[k].z;
+                poly.at(0, k, 3) = v[k].w;
+                for (uint32_t t = 0; t < kMaxSlots; ++t) poly.at(0, k, 4 + t) = a[k][t];
             }
             const uint32_t np = clip_polygon(poly, mor, pin);
             nfan = np >= 3 ? np - 2 : 0;
@@ -429,12 +476,12 @@ __global__ __launch_bounds__(256) void k_clip(FrameDev F)
             Prim s;
             bool ok = false;
             if (j < nfan && fits) {
-                const float4 fv[3] = { poly.v[pin][0], poly.v[pin][j + 1], poly.v[pin][j + 2] };
+                const float4 fv[3] = { poly.pos(pin, 0), poly.pos(pin, j + 1), poly.pos(pin, j + 2) };
                 float fa[3][kMaxSlots];
                 for (uint32_t t = 0; t < kMaxSlots; ++t) {
-                    fa[0][t] = poly.a[pin][0][t];
-                    fa[1][t] = poly.a[pin][j + 1][t];
-                    fa[2][t] = poly.a[pin][j + 2][t];
+                    fa[0][t] = poly.at(pin, 0, 4 + t);
+                    fa[1][t] = poly.at(pin, j + 1, 4 + t);
+                    fa[2][t] = poly.at(pin, j + 2, 4 + t);
                 }
                 const uint32_t r = F.ntris + base + j;
                 ok = setup_prim(s, fv, fa, D.nslots, V.width, V.height);
@@ -563,7 +610,7 @@ __global__ __launch_bounds__(256) void k_emit(FrameDev F)
             c[0] = (int32_t)q0.x; c[1] = (int32_t)q0.y; c[2] = (int32_t)q0.z;
             dx[0] = (int32_t)q0.w; dx[1] = (int32_t)q1.x; dx[2] = (int32_t)q1.y;
             dy[0] = (int32_t)q1.z; dy[1] = (int32_t)q1.w; dy[2] = (int32_t)q2.x;
-            xmin = q2.y & 0xFFFF; xmax = q2.y >> 16; ymin = q2.z & 0xFFFF; ymax = q2.z >> 16;
+            xmin = q2.y & 0x3FFF; xmax = (q2.y >> 16) & 0x3FFF; ymin = q2.z & 0x3FFF; ymax = (q2.z >> 16) & 0x3FFF;
             view = F.draws[d].view;
         }
         walk_bins(F, F.views[view], active, c, dx, dy, xmin, xmax, ymin, ymax, true, r);

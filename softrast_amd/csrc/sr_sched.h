@@ -5,8 +5,8 @@
 // only the order bins are processed in, never the arithmetic of a pixel.
 //
 //   k_bin_features  per work bin, from its entry list: x0 = log2(1 + entries),
-//                   x1 = log2(1 + 8x8 blocks of lane-private primitives),
-//                   x2 = log2(1 + 8x8 blocks of wave-cooperative primitives),
+//                   x1 = log2(1 + walked 8x8 blocks),
+//                   x2 = log2(1 + walked blocks of primitives without a tight bound),
 //                   x3 = log2(1 + summed bbox-in-bin area / bin area)
 //   k_mlp_infer     4 -> 32 -> 32 -> 1 ReLU MLP (fp32) -> predicted log2 cost
 //   k_mlp_record    after the back end: (features, log2 measured s_memtime
@@ -46,27 +46,27 @@ __global__ __launch_bounds__(256) void k_bin_features(FrameDev F, float4* feat)
     const int32_t X0 = (int32_t)((sbx / K) * kBin), Y0 = (int32_t)((sby / K) * kBin);
     const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S;
     const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
-    uint32_t small_blocks = 0, large_blocks = 0, area = 0;
+    uint32_t blocks = 0, open_blocks = 0, area = 0;
     for (uint32_t e = lane; e < cnt; e += 64) {
         const uint4 q2 = F.rec_raster[(size_t)F.entries[beg + e] * 4 + 2];
-        const int32_t xmin = q2.y & 0xFFFF, xmax = q2.y >> 16, ymin = q2.z & 0xFFFF, ymax = q2.z >> 16;
+        const int32_t xmin = q2.y & 0x3FFF, xmax = (q2.y >> 16) & 0x3FFF, ymin = q2.z & 0x3FFF, ymax = (q2.z >> 16) & 0x3FFF;
         const uint32_t xs = max((uint32_t)clampi(xmin - X0, 0, 64) & ~7u, sx0);
         const uint32_t xe = min((uint32_t)clampi(xmax - X0, 0, 64), sx0 + S);
         const uint32_t ys = max((uint32_t)clampi(ymin - Y0, 0, 64) & ~7u, sy0);
         const uint32_t ye = min((uint32_t)clampi(ymax - Y0, 0, 64), sy0 + S);
         const uint32_t nb = (xe > xs ? (xe - xs + 7u) >> 3 : 0u) * (ye > ys ? (ye - ys + 7u) >> 3 : 0u);
-        if (nb <= kSmallBlocks) small_blocks += nb;
-        else large_blocks += nb;
+        blocks += nb;
+        if (!((q2.y >> 14) & 1u)) open_blocks += nb;  // no tight bound: every row of every block is visited
         const uint32_t ax = (uint32_t)max(0, min(xmax - X0 + 1, (int32_t)(sx0 + S)) - max(xmin - X0, (int32_t)sx0));
         const uint32_t ay = (uint32_t)max(0, min(ymax - Y0 + 1, (int32_t)(sy0 + S)) - max(ymin - Y0, (int32_t)sy0));
         area += ax * ay;
     }
-    small_blocks = wave_sum(small_blocks);
-    large_blocks = wave_sum(large_blocks);
+    blocks = wave_sum(blocks);
+    open_blocks = wave_sum(open_blocks);
     area = wave_sum(area);
     if (lane == 0)
-        feat[gb] = make_float4(__log2f(1.0f + (float)cnt), __log2f(1.0f + (float)small_blocks),
-                               __log2f(1.0f + (float)large_blocks), __log2f(1.0f + (float)area / (float)(S * S)));
+        feat[gb] = make_float4(__log2f(1.0f + (float)cnt), __log2f(1.0f + (float)blocks),
+                               __log2f(1.0f + (float)open_blocks), __log2f(1.0f + (float)area / (float)(S * S)));
 }
 
 __device__ __forceinline__ float mlp_forward(const MlpParams& P, const float x[kFeat], float h1[kHid], float h2[kHid])

@@ -28,6 +28,7 @@
 
 #include "srgpu.h"
 #include "sr_sched.h"
+#include "sr_texture.h"
 
 namespace srg {
 
@@ -496,7 +497,7 @@ int run_frame(srg_ctx* ctx)
                   ((ctx->cfg.flags & SRG_CFG_TILE_COSTS) ? FF_TILE_COSTS : 0u);
 
         const uint32_t emit_blocks = std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 16384));
-        const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + 255) / 256, 512));
+        const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + kClipThreads - 1) / kClipThreads, 2048));
         const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((TB + 255) / 256, 1024));
         const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
         const bool learned = ctx->cfg.tile_order == SRG_ORDER_LEARNED;
@@ -516,7 +517,7 @@ int run_frame(srg_ctx* ctx)
                         hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, ctx->stream, v.primid, 0xFFFFFFFFu,
                                            (size_t)v.tiles_x * v.tiles_y * kTilePixels);
             if (nblocks) hipLaunchKernelGGL(k_setup, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
-            if (NT) hipLaunchKernelGGL(k_clip, dim3(clip_blocks), dim3(256), 0, ctx->stream, F);
+            if (NT) hipLaunchKernelGGL(k_clip, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
             FrameDev Fs = F;  // with ordering, the natural queue goes to the spare buffer first
             if (ordered) Fs.queue = ctx->queue2.ptr;
             hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs);
@@ -1181,31 +1182,47 @@ static bool mitchell_resize_rgba8(const uint8_t* in, uint32_t w, uint32_t h, uin
     return true;
 }
 
+// Layout of CreateFromRGBA8 (Texture.cpp:119-177): power-of-two sizes, each
+// a multiple of the 32-texel tile and below 2^14; a full chain when mips are
+// made; every level padded to whole 32x32 tiles.
+struct TexLayout {
+    uint32_t wl = 0, hl = 0, n = 0;
+    uint32_t offs[14] = {};
+    uint64_t total = 0;
+};
+
+static int texture_layout(uint32_t width, uint32_t height, int calc_mips, TexLayout& L)
+{
+    auto is_pow2 = [](uint32_t v) { return v && !(v & (v - 1)); };
+    if (!is_pow2(width) || !is_pow2(height) || width < 32 || height < 32 || width >= 16384 || height >= 16384)
+        return SRG_ERR_INVALID;
+    while ((1u << L.wl) < width) ++L.wl;
+    while ((1u << L.hl) < height) ++L.hl;
+    L.n = calc_mips ? std::max(L.wl, L.hl) + 1 : 1;
+    L.total = 0;
+    for (uint32_t m = 0; m < L.n; ++m) {
+        const uint32_t mw = std::max(1u, width >> m), mh = std::max(1u, height >> m);
+        L.offs[m] = (uint32_t)L.total;
+        L.total += (uint64_t)((mw + 31u) & ~31u) * ((mh + 31u) & ~31u) * 4u;
+    }
+    return SRG_OK;
+}
+
 int srg_texture_build_rgba8(const uint8_t* linear, uint32_t width, uint32_t height, int calc_mips, uint8_t* out,
                             uint64_t* out_bytes, uint32_t mip_offsets[14], uint32_t* wlog2, uint32_t* hlog2,
                             uint32_t* nmips)
 {
-    // CreateFromRGBA8 (Texture.cpp:119-199) requires power-of-two sizes that
-    // are multiples of the 32-texel tile and below 2^14.
-    auto is_pow2 = [](uint32_t v) { return v && !(v & (v - 1)); };
-    if (!out_bytes || !is_pow2(width) || !is_pow2(height) || width < 32 || height < 32 || width >= 16384 ||
-        height >= 16384)
-        return SRG_ERR_INVALID;
-    uint32_t wl = 0, hl = 0;
-    while ((1u << wl) < width) ++wl;
-    while ((1u << hl) < height) ++hl;
-    const uint32_t n = calc_mips ? std::max(wl, hl) + 1 : 1;
-    uint32_t offs[14] = {};
-    uint64_t total = 0;
-    for (uint32_t m = 0; m < n; ++m) {
-        const uint32_t mw = std::max(1u, width >> m), mh = std::max(1u, height >> m);
-        offs[m] = (uint32_t)total;
-        total += (uint64_t)((mw + 31u) & ~31u) * ((mh + 31u) & ~31u) * 4u;
-    }
+    TexLayout L;
+    if (!out_bytes) return SRG_ERR_INVALID;
+    int st = texture_layout(width, height, calc_mips, L);
+    if (st) return st;
+    const uint32_t n = L.n;
+    const uint32_t* offs = L.offs;
+    const uint64_t total = L.total;
     *out_bytes = total;
-    if (mip_offsets) std::memcpy(mip_offsets, offs, sizeof(offs));
-    if (wlog2) *wlog2 = wl;
-    if (hlog2) *hlog2 = hl;
+    if (mip_offsets) std::memcpy(mip_offsets, offs, sizeof(L.offs));
+    if (wlog2) *wlog2 = L.wl;
+    if (hlog2) *hlog2 = L.hl;
     if (nmips) *nmips = n;
     if (!out) return SRG_OK;
     if (!linear) return SRG_ERR_INVALID;
@@ -1219,6 +1236,110 @@ int srg_texture_build_rgba8(const uint8_t* linear, uint32_t width, uint32_t heig
         if (!mitchell_resize_rgba8(linear, width, height, tmp.data(), mw, mh)) return SRG_ERR_INVALID;
         tile_texture(tmp.data(), out + offs[m], mw, mh);
     }
+    return SRG_OK;
+}
+
+int srg_texture_create_rgba8(srg_ctx* ctx, const uint8_t* linear, uint32_t width, uint32_t height, int calc_mips,
+                             srg_handle* out)
+{
+    TexLayout L;
+    if (!ctx || !out || !linear) return SRG_ERR_INVALID;
+    int st = texture_layout(width, height, calc_mips, L);
+    if (st) return st;
+    // per-level tap lists, built on the host before anything is allocated
+    std::vector<AxisTaps> tx(L.n), ty(L.n);
+    size_t ntaps = 0;
+    for (uint32_t m = 1; m < L.n; ++m) {
+        const uint32_t mw = std::max(1u, width >> m), mh = std::max(1u, height >> m);
+        if (!mitchell_axis(width, mw, tx[m]) || !mitchell_axis(height, mh, ty[m])) return SRG_ERR_INVALID;
+        ntaps += tx[m].src.size() + ty[m].src.size() + mw + mh + 2;
+    }
+    Texture t;
+    t.bytes = L.total;
+    t.wlog2 = L.wl;
+    t.hlog2 = L.hl;
+    t.nmips = L.n;
+    std::memcpy(t.mip_offsets, L.offs, sizeof(L.offs));
+    uint32_t* base = nullptr;
+    float4* rows = nullptr;
+    uint8_t* taps = nullptr;
+    std::vector<uint8_t> htaps;
+    const size_t row_bytes = L.n > 1 ? sizeof(float4) * (size_t)height * std::max(1u, width >> 1) : 0;
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(ctx->stream);
+        if (base) (void)hipFree(base);
+        if (rows) (void)hipFree(rows);
+        if (taps) (void)hipFree(taps);
+        if (code && t.dev) (void)hipFree(t.dev);
+        return code;
+    };
+    if (hipMalloc((void**)&t.dev, L.total) != hipSuccess || hipMalloc((void**)&base, (size_t)width * height * 4) != hipSuccess ||
+        (row_bytes && hipMalloc((void**)&rows, row_bytes) != hipSuccess) ||
+        (ntaps && hipMalloc((void**)&taps, ntaps * 4) != hipSuccess))
+        return cleanup(SRG_ERR_NOMEM);
+    if (hipMemcpyAsync(base, linear, (size_t)width * height * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return cleanup(SRG_ERR_HIP);
+    // all tap lists in one upload: per level first_x, src_x, w_x, first_y, src_y, w_y
+    htaps.resize(ntaps * 4);
+    std::vector<MipTaps> dx(L.n), dy(L.n);
+    size_t at = 0;
+    auto put = [&](const void* p, size_t words) {
+        const size_t o = at;
+        std::memcpy(htaps.data() + o * 4, p, words * 4);
+        at += words;
+        return o;
+    };
+    for (uint32_t m = 1; m < L.n; ++m) {
+        const size_t fx = put(tx[m].first.data(), tx[m].first.size()), sx = put(tx[m].src.data(), tx[m].src.size());
+        const size_t wx = put(tx[m].weight.data(), tx[m].weight.size());
+        const size_t fy = put(ty[m].first.data(), ty[m].first.size()), sy = put(ty[m].src.data(), ty[m].src.size());
+        const size_t wy = put(ty[m].weight.data(), ty[m].weight.size());
+        const uint32_t* T = (const uint32_t*)taps;
+        dx[m] = MipTaps{ T + fx, T + sx, (const float*)(T + wx) };
+        dy[m] = MipTaps{ T + fy, T + sy, (const float*)(T + wy) };
+    }
+    if (ntaps && hipMemcpyAsync(taps, htaps.data(), at * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        return cleanup(SRG_ERR_HIP);
+    for (uint32_t m = 0; m < L.n; ++m) {
+        const uint32_t mw = std::max(1u, width >> m), mh = std::max(1u, height >> m);
+        const uint32_t tiles_w = (mw + 31u) >> 5, tiles_h = (mh + 31u) >> 5;
+        const uint32_t ntex = tiles_w * tiles_h * 1024u;
+        uint32_t* dst = (uint32_t*)(t.dev + L.offs[m]);
+        if (m == 0) {
+            hipLaunchKernelGGL(k_tex_tile, dim3((ntex + 255) / 256), dim3(256), 0, ctx->stream, base, width, height, dst,
+                               tiles_w, ntex);
+        } else {
+            hipLaunchKernelGGL(k_mip_rows, dim3((height * mw + 255) / 256), dim3(256), 0, ctx->stream, base, width, height,
+                               mw, dx[m], rows);
+            hipLaunchKernelGGL(k_mip_cols, dim3((ntex + 255) / 256), dim3(256), 0, ctx->stream, rows, mw, mh, dy[m], dst,
+                               tiles_w, ntex);
+        }
+        if (hipGetLastError() != hipSuccess) return cleanup(SRG_ERR_HIP);
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return cleanup(SRG_ERR_HIP);
+    cleanup(SRG_OK);
+    const uint64_t h = new_handle(ctx);
+    ctx->textures[h] = t;
+    *out = h;
+    return SRG_OK;
+}
+
+int srg_texture_read(srg_ctx* ctx, srg_handle h, uint8_t* out, uint64_t capacity, uint64_t* bytes,
+                     uint32_t mip_offsets[14], uint32_t* wlog2, uint32_t* hlog2, uint32_t* nmips)
+{
+    if (!ctx) return SRG_ERR_INVALID;
+    auto it = ctx->textures.find(h);
+    if (it == ctx->textures.end()) return SRG_ERR_HANDLE;
+    const Texture& t = it->second;
+    if (bytes) *bytes = t.bytes;
+    if (mip_offsets) std::memcpy(mip_offsets, t.mip_offsets, sizeof(t.mip_offsets));
+    if (wlog2) *wlog2 = t.wlog2;
+    if (hlog2) *hlog2 = t.hlog2;
+    if (nmips) *nmips = t.nmips;
+    if (!out) return SRG_OK;
+    if (capacity < t.bytes) return SRG_ERR_INVALID;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (t.bytes) HIP_TRY(hipMemcpy(out, t.dev, t.bytes, hipMemcpyDeviceToHost));
     return SRG_OK;
 }
 
