@@ -154,6 +154,19 @@ int srg_end_frame(srg_ctx* ctx);
 /* Copies the write plane's colour to linear RGBA8 rows (width*height*4 bytes),
  * swaps the planes, then calls on_finish(user) (Renderer.cpp:350-372). */
 int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear_rgba8, void (*on_finish)(void*), void* user);
+/* Blit (Renderer.cpp:350-372) with the reference's asynchrony: returns at
+ * once; on a second stream the write plane is de-tiled into pinned memory
+ * (overlapping the next frame), copied to linear_rgba8 (width*height*4
+ * bytes) and on_finish(user) is called from a runtime thread, which must not
+ * call into HIP. The planes are swapped at once. Like the reference, a blit
+ * first waits for the previous blit, of the other plane, to finish; later
+ * frames wait on the device until the blitted plane has been read.
+ * SRG_BLIT_BGRA exchanges red and blue on the way (the present swizzle of
+ * Window_Win32::Flip, Viewer/Platform/Window_Win32.cpp:85-112). */
+#define SRG_BLIT_BGRA 1u
+int srg_blit_async(srg_ctx* ctx, srg_fb* fb, void* linear_rgba8, uint32_t flags, void (*on_finish)(void*), void* user);
+/* Waits until every asynchronous blit of fb, callback included, is done. */
+int srg_blit_wait(srg_ctx* ctx, srg_fb* fb);
 int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out);
 /* Per work-bin data of the last frame, bins numbered view by view in the
  * order views were first drawn to, row-major inside a view: entry counts and,

@@ -41,7 +41,7 @@ SRG_SYMBOLS = (
     "srg_fb_create", "srg_fb_destroy", "srg_fb_write_plane", "srg_fb_swap_planes", "srg_fb_read_tiles",
     "srg_fb_write_tiles", "srg_begin_frame", "srg_clear", "srg_draw_indexed", "srg_end_frame", "srg_blit",
     "srg_get_frame_stats", "srg_get_bin_stats", "srg_sched_get_model", "srg_sched_set_model",
-    "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read",
+    "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
 )
 
 
@@ -102,6 +102,8 @@ def load_library(path=None):
         "srg_release_buffer": (i32, [vp, u64]),
         "srg_register_texture": (i32, [vp, vp, u64, ctypes.POINTER(u32), u32, u32, u32, pu64]),
         "srg_texture_create_rgba8": (i32, [vp, vp, u32, u32, i32, pu64]),
+        "srg_blit_async": (i32, [vp, vp, vp, u32, vp, vp]),
+        "srg_blit_wait": (i32, [vp, vp]),
         "srg_texture_read": (i32, [vp, u64, vp, u64, pu64, ctypes.POINTER(u32), ctypes.POINTER(u32),
                                    ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "srg_release_texture": (i32, [vp, u64]),
@@ -357,6 +359,7 @@ class RenderContext:
         self._buffers = {}
         self._textures = {}
         self._keep = []
+        self._blit_keep = {}
         self._draws = []
 
     # -- resources ---------------------------------------------------------
@@ -443,6 +446,24 @@ class RenderContext:
         if onFinish is not None:
             onFinish(user)
         return out
+
+    _BLIT_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+
+    def BlitAsync(self, fb, linear=None, onFinish=None, user=None, bgra=False):
+        """Blit with the reference's asynchrony (srg_blit_async): returns the
+        target array at once; onFinish(user) runs from a runtime thread when
+        the rows are in it. The planes are swapped at once."""
+        out = linear if linear is not None else np.empty((fb.m_height, fb.m_width), np.uint32)
+        if out.nbytes < fb.m_width * fb.m_height * 4 or not out.flags.c_contiguous:
+            raise ValueError("Blit target too small or not contiguous")
+        cb = self._BLIT_CB((lambda _p: onFinish(user)) if onFinish is not None else (lambda _p: None))
+        self._blit_keep.setdefault(id(fb), []).append((out, cb))  # alive until BlitWait
+        _check(self._lib.srg_blit_async(self._ctx, fb._h, _ptr(out), 1 if bgra else 0, cb, None), "BlitAsync")
+        return out
+
+    def BlitWait(self, fb):
+        _check(self._lib.srg_blit_wait(self._ctx, fb._h), "BlitWait")
+        self._blit_keep.pop(id(fb), None)
 
     def frame_stats(self):
         s = SrgFrameStats()

@@ -268,4 +268,58 @@ uint32_t srs_spheres(uint64_t seed, uint32_t nspheres, uint32_t slices, uint32_t
     return (uint32_t)vtx.size();
 }
 
+// Independent triangles for a width x height target, given directly in clip
+// space (draw them with the identity MVP): a fraction `clustered` of the
+// triangles around 3 Gaussian cluster centres (sigma = sigma_frac * width,
+// centres uniform in the middle half of the screen), the rest uniform over
+// the screen. Screen area log-uniform in [area_min, area_max] px^2, random
+// orientation and aspect in [1, 3]; depth z/w uniform in (0.05, 0.95) so the
+// reverse-Z test orders them. Normals random unit, UVs uniform in [0, 4).
+// Triangle t uses vertices 3t..3t+2; 6 * n * 4 bytes of u32 indices are not
+// needed, so only vertices are written (3 * n).
+void srs_clustered_tris(uint64_t seed, uint32_t n, uint32_t width, uint32_t height, double clustered,
+                        double sigma_frac, double area_min, double area_max, Vertex* vout)
+{
+    Pcg32 rng(seed, 3);
+    double cx[3], cy[3];
+    for (int k = 0; k < 3; ++k) {
+        cx[k] = (0.25 + 0.5 * rng.uniform()) * width;
+        cy[k] = (0.25 + 0.5 * rng.uniform()) * height;
+    }
+    const double sig = sigma_frac * width, la = std::log(area_min), lb = std::log(area_max);
+    for (uint32_t t = 0; t < n; ++t) {
+        double px, py;
+        if (rng.uniform() < clustered) {
+            const int k = (int)(rng.next() % 3u);
+            px = cx[k] + sig * rng.normal();
+            py = cy[k] + sig * rng.normal();
+        } else {
+            px = rng.uniform() * width;
+            py = rng.uniform() * height;
+        }
+        const double area = std::exp(la + (lb - la) * rng.uniform());
+        const double aspect = 1.0 + 2.0 * rng.uniform(), rot = 6.283185307179586 * rng.uniform();
+        // triangle with base b and height h = aspect * b / 2: area = b * h / 2
+        const double b = std::sqrt(4.0 * area / aspect), h = aspect * b * 0.5;
+        const double lx[3] = { -0.5 * b, 0.5 * b, (rng.uniform() - 0.5) * b };
+        const double ly[3] = { 0.0, 0.0, h };
+        const double z = 0.05 + 0.9 * rng.uniform();
+        const double c = std::cos(rot), sn = std::sin(rot);
+        for (int k = 0; k < 3; ++k) {
+            // counter-clockwise in pixel space (y down) is front-facing here
+            const int kk = k == 0 ? 0 : 3 - k;
+            const double sx = px + c * lx[kk] - sn * ly[kk], sy = py + sn * lx[kk] + c * ly[kk];
+            Vertex v;
+            v.pos[0] = (float)(2.0 * sx / width - 1.0);
+            v.pos[1] = (float)(1.0 - 2.0 * sy / height);
+            v.pos[2] = (float)z;
+            V3 nn = norm({ rng.normal(), rng.normal(), rng.normal() });
+            v.nrm[0] = (float)nn.x; v.nrm[1] = (float)nn.y; v.nrm[2] = (float)nn.z;
+            v.uv[0] = (float)(4.0 * rng.uniform());
+            v.uv[1] = (float)(4.0 * rng.uniform());
+            vout[(size_t)t * 3 + k] = v;
+        }
+    }
+}
+
 }  // extern "C"

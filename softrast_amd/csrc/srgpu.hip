@@ -54,13 +54,18 @@ __global__ void k_fill_f32(float* p, float value, size_t n)
 }
 
 // BlitJobFn (Renderer.cpp:319-347): 64x64 tiles -> linear rows.
-__global__ void k_detile(const uint32_t* tiles, uint32_t* linear, uint32_t width, uint32_t height, uint32_t tiles_x)
+// With bgra, red and blue are exchanged on the way (the present swizzle of
+// Window_Win32::Flip, Viewer/Platform/Window_Win32.cpp:85-112).
+__global__ void k_detile(const uint32_t* tiles, uint32_t* linear, uint32_t width, uint32_t height, uint32_t tiles_x,
+                         uint32_t bgra)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t y = blockIdx.y;
     if (x >= width || y >= height) return;
     const uint32_t t = (y >> 6) * tiles_x + (x >> 6);
-    linear[(size_t)y * width + x] = tiles[(size_t)t * kTilePixels + (y & 63u) * 64u + (x & 63u)];
+    uint32_t v = tiles[(size_t)t * kTilePixels + (y & 63u) * 64u + (x & 63u)];
+    if (bgra) v = (v & 0xFF00FF00u) | ((v >> 16) & 0xFFu) | ((v & 0xFFu) << 16);
+    linear[(size_t)y * width + x] = v;
 }
 
 }  // namespace srg
@@ -114,12 +119,28 @@ struct Texture {
     uint32_t wlog2 = 0, hlog2 = 0, nmips = 0;
 };
 
+// An asynchronous blit: the host function copies the pinned rows to the
+// caller's buffer and calls its callback (BlitJobFn's tail, Renderer.cpp:341-346).
+struct BlitJob {
+    void* dst = nullptr;
+    const void* src = nullptr;
+    size_t bytes = 0;
+    void (*cb)(void*) = nullptr;
+    void* user = nullptr;
+};
+
 struct PlaneState {
     uint32_t* colour = nullptr;
     float* depth = nullptr;
     uint32_t* primid = nullptr;
     bool pending = false;
     int clear_colour = 0, clear_depth = 0;
+    // asynchronous blit of this plane
+    uint32_t* pinned = nullptr;      // host rows written by the de-tile kernel
+    hipEvent_t read_done = nullptr;  // the plane's tiles have been read
+    hipEvent_t blit_done = nullptr;  // the callback has returned
+    bool blit_pending = false;
+    BlitJob job;
     uint32_t clear_value = 0;
 };
 
@@ -137,6 +158,7 @@ struct srg_ctx {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // asynchronous blits
     std::unordered_map<uint64_t, Buffer> buffers;
     std::unordered_map<uint64_t, Texture> textures;
     uint64_t next_handle = 1;
@@ -193,9 +215,32 @@ int ensure_host_tables(srg_ctx* ctx, size_t n)
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Work on the library stream that touches a plane waits until an
+// asynchronous blit of it has read its tiles.
+void stream_wait_blit_read(srg_ctx* ctx, PlaneState& p)
+{
+    if (p.blit_pending) (void)hipStreamWaitEvent(ctx->stream, p.read_done, 0);
+}
+
+// Host side: the blit of this plane, callback included, has finished.
+int finish_blit(PlaneState& p)
+{
+    if (!p.blit_pending) return SRG_OK;
+    p.blit_pending = false;
+    return hipEventSynchronize(p.blit_done) == hipSuccess ? SRG_OK : SRG_ERR_HIP;
+}
+
+void blit_host_fn(void* arg)
+{
+    const BlitJob* j = (const BlitJob*)arg;
+    std::memcpy(j->dst, j->src, j->bytes);
+    if (j->cb) j->cb(j->user);
+}
+
 int flush_pending_clear(srg_ctx* ctx, srg_fb* fb, uint32_t plane)
 {
     PlaneState& ps = fb->planes[plane];
+    stream_wait_blit_read(ctx, ps);
     if (!ps.pending) return SRG_OK;
     const size_t n = (size_t)fb->tiles_x * fb->tiles_y * kTilePixels;
     const uint32_t b = ps.clear_value & 0xFFu;
@@ -454,6 +499,9 @@ int run_frame(srg_ctx* ctx)
     uint64_t want_entries = ctx->cfg.max_entries ? ctx->cfg.max_entries : 2ull * NT + 4096;
     want_entries = std::max<uint64_t>(want_entries, ctx->entry_cap);
 
+    // planes an asynchronous blit is still reading are written only after it
+    for (const ViewKey& v : views) stream_wait_blit_read(ctx, v.fb->planes[v.plane]);
+
     for (uint32_t attempt = 0;; ++attempt) {
         if (want_clip > 0xFFFFFFF0ull - NT || want_entries > 0xFFFFFFF0ull) return SRG_ERR_OVERFLOW;
         const uint64_t nrec = (uint64_t)NT + want_clip;
@@ -662,6 +710,10 @@ int srg_create(const srg_config* cfg, srg_ctx** out)
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) { delete ctx; return SRG_ERR_HIP; }
     ctx->num_cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return SRG_ERR_HIP; }
+    if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+        srg_destroy(ctx);
+        return SRG_ERR_HIP;
+    }
     for (auto& e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) { srg_destroy(ctx); return SRG_ERR_HIP; }
     if (hipHostMalloc((void**)&ctx->h_counters, sizeof(uint32_t) * C_NCOUNTERS, 0) != hipSuccess) {
@@ -708,6 +760,7 @@ void srg_destroy(srg_ctx* ctx)
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx;
 }
 
@@ -845,11 +898,16 @@ int srg_fb_create(srg_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags,
 void srg_fb_destroy(srg_ctx* ctx, srg_fb* fb)
 {
     if (!ctx || !fb) return;
+    for (auto& p : fb->planes) (void)finish_blit(p);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
     for (auto& p : fb->planes) {
         if (p.colour) (void)hipFree(p.colour);
         if (p.depth) (void)hipFree(p.depth);
         if (p.primid) (void)hipFree(p.primid);
+        if (p.pinned) (void)hipHostFree(p.pinned);
+        if (p.read_done) (void)hipEventDestroy(p.read_done);
+        if (p.blit_done) (void)hipEventDestroy(p.blit_done);
     }
     ctx->fbs.erase(std::remove(ctx->fbs.begin(), ctx->fbs.end(), fb), ctx->fbs.end());
     delete fb;
@@ -886,6 +944,7 @@ int srg_fb_write_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, const uint32_t*
     if (!ctx || !fb || plane > 1) return SRG_ERR_INVALID;
     const size_t n = (size_t)fb->tiles_x * fb->tiles_y * kTilePixels;
     PlaneState& p = fb->planes[plane];
+    stream_wait_blit_read(ctx, p);
     if (colour) {
         if (!p.colour) return SRG_ERR_INVALID;
         if (p.pending) p.clear_colour = 0;
@@ -983,13 +1042,54 @@ int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear, void (*on_finish)(void*), v
     const size_t npx = (size_t)fb->width * fb->height;
     if ((st = ctx->blit_staging.ensure(npx))) return st;
     hipLaunchKernelGGL(k_detile, dim3((fb->width + 255) / 256, fb->height), dim3(256), 0, ctx->stream, p.colour,
-                       ctx->blit_staging.ptr, fb->width, fb->height, fb->tiles_x);
+                       ctx->blit_staging.ptr, fb->width, fb->height, fb->tiles_x, 0u);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(linear, ctx->blit_staging.ptr, npx * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     fb->write_plane ^= 1u;
     if (on_finish) on_finish(user);
     return SRG_OK;
+}
+
+int srg_blit_async(srg_ctx* ctx, srg_fb* fb, void* linear, uint32_t flags, void (*on_finish)(void*), void* user)
+{
+    if (!ctx || !fb || !linear || (flags & ~SRG_BLIT_BGRA)) return SRG_ERR_INVALID;
+    if (ctx->in_frame) return SRG_ERR_STATE;
+    const uint32_t wp = fb->write_plane;
+    PlaneState& p = fb->planes[wp];
+    if (!p.colour) return SRG_ERR_INVALID;
+    // the reference waits for the previous blit, of the other plane (Renderer.cpp:353)
+    int st = finish_blit(fb->planes[wp ^ 1u]);
+    if (!st) st = finish_blit(p);
+    if (!st) st = flush_pending_clear(ctx, fb, wp);
+    if (st) return st;
+    const size_t npx = (size_t)fb->width * fb->height;
+    if (!p.pinned && hipHostMalloc((void**)&p.pinned, npx * 4, 0) != hipSuccess) return SRG_ERR_NOMEM;
+    if (!p.read_done) HIP_TRY(hipEventCreateWithFlags(&p.read_done, hipEventDisableTiming));
+    if (!p.blit_done) HIP_TRY(hipEventCreateWithFlags(&p.blit_done, hipEventDisableTiming));
+    uint32_t* dst = nullptr;
+    HIP_TRY(hipHostGetDevicePointer((void**)&dst, p.pinned, 0));
+    // the copy stream starts once the frame that wrote the plane is done
+    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+    HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->ev[3], 0));
+    hipLaunchKernelGGL(k_detile, dim3((fb->width + 255) / 256, fb->height), dim3(256), 0, ctx->copy_stream, p.colour,
+                       dst, fb->width, fb->height, fb->tiles_x, (flags & SRG_BLIT_BGRA) ? 1u : 0u);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(p.read_done, ctx->copy_stream));
+    p.job = BlitJob{ linear, p.pinned, npx * 4, on_finish, user };
+    HIP_TRY(hipLaunchHostFunc(ctx->copy_stream, blit_host_fn, &p.job));
+    HIP_TRY(hipEventRecord(p.blit_done, ctx->copy_stream));
+    p.blit_pending = true;
+    fb->write_plane ^= 1u;
+    return SRG_OK;
+}
+
+int srg_blit_wait(srg_ctx* ctx, srg_fb* fb)
+{
+    if (!ctx || !fb) return SRG_ERR_INVALID;
+    int st = finish_blit(fb->planes[0]);
+    const int st1 = finish_blit(fb->planes[1]);
+    return st ? st : st1;
 }
 
 int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out)
@@ -1252,7 +1352,8 @@ int srg_texture_create_rgba8(srg_ctx* ctx, const uint8_t* linear, uint32_t width
     for (uint32_t m = 1; m < L.n; ++m) {
         const uint32_t mw = std::max(1u, width >> m), mh = std::max(1u, height >> m);
         if (!mitchell_axis(width, mw, tx[m]) || !mitchell_axis(height, mh, ty[m])) return SRG_ERR_INVALID;
-        ntaps += tx[m].src.size() + ty[m].src.size() + mw + mh + 2;
+        // words: first (n + 1), src and weight per axis
+        ntaps += tx[m].first.size() + 2 * tx[m].src.size() + ty[m].first.size() + 2 * ty[m].src.size();
     }
     Texture t;
     t.bytes = L.total;
@@ -1285,7 +1386,7 @@ int srg_texture_create_rgba8(srg_ctx* ctx, const uint8_t* linear, uint32_t width
     size_t at = 0;
     auto put = [&](const void* p, size_t words) {
         const size_t o = at;
-        std::memcpy(htaps.data() + o * 4, p, words * 4);
+        if (at + words <= ntaps) std::memcpy(htaps.data() + o * 4, p, words * 4);
         at += words;
         return o;
     };
@@ -1298,6 +1399,7 @@ int srg_texture_create_rgba8(srg_ctx* ctx, const uint8_t* linear, uint32_t width
         dx[m] = MipTaps{ T + fx, T + sx, (const float*)(T + wx) };
         dy[m] = MipTaps{ T + fy, T + sy, (const float*)(T + wy) };
     }
+    if (at != ntaps) return cleanup(SRG_ERR_INVALID);  // tap lists must fill the upload exactly
     if (ntaps && hipMemcpyAsync(taps, htaps.data(), at * 4, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
         return cleanup(SRG_ERR_HIP);
     for (uint32_t m = 0; m < L.n; ++m) {

@@ -39,6 +39,8 @@ def _load():
         lib.srs_cube_grid.restype = u32
         lib.srs_spheres.argtypes = [u64, u32, u32, u32, d, d, d, d, d, u32, u32, d, vp, vp, ctypes.POINTER(d)]
         lib.srs_spheres.restype = u32
+        lib.srs_clustered_tris.argtypes = [u64, u32, u32, u32, d, d, d, d, vp]
+        lib.srs_clustered_tris.restype = None
         _lib = lib
     return _lib
 
@@ -122,3 +124,25 @@ def spheres_scene(seed=SEED_C2, nspheres=4000, ndraws=8, width=1920, height=1080
         for d in draws:
             d.texture = -1
     return Scene(width, height, mvp, draws, texs, {"config": "spheres", "radius_scale": scale.value})
+
+
+def clustered_scene(seed=SEED_C3, ntris=4_000_000, width=3840, height=2160, ndraws=16, tex_size=4096,
+                    clustered=0.9, sigma_frac=0.05, area_min=0.5, area_max=2000.0):
+    """3840x2160, 4M independent triangles, 90% in 3 Gaussian clusters (sigma =
+    5% of the width), areas log-uniform 0.5-2000 px^2, 16 draws x 4096^2
+    textures. Given in clip space: the MVP is the identity. Textures are
+    returned as linear images; build them on the device for this size."""
+    lib = _load()
+    v = np.empty(ntris * 3, VERTEX_DTYPE)
+    lib.srs_clustered_tris(seed, ntris, width, height, clustered, sigma_frac, area_min, area_max, v.ctypes.data)
+    mvp = np.eye(4, dtype=np.float32).reshape(-1)
+    per = (ntris + ndraws - 1) // ndraws
+    draws = []
+    for k in range(ndraws):
+        t0, t1 = k * per, min(ntris, (k + 1) * per)
+        if t0 >= t1:
+            break
+        vv = v[t0 * 3:t1 * 3].copy()
+        draws.append(Draw(vv, np.arange((t1 - t0) * 3, dtype=np.uint32), k if tex_size else -1))
+    texs = [value_noise_rgba8(seed ^ (0x7E470000 + k), tex_size, tex_size) for k in range(ndraws)] if tex_size else []
+    return Scene(width, height, mvp, draws, texs, {"config": "clustered"})

@@ -92,15 +92,18 @@ def oracle_render(scene, texdata, nthreads=1, clear=0, plane_init=None, fast=Fal
     return p, stats
 
 
-def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, **ctx_kw):
-    """Render one frame on the GPU through the C ABI; returns (colour, depth, primid, stats)."""
+def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, fb=None, **ctx_kw):
+    """Render one frame on the GPU through the C ABI; returns (colour, depth, primid, stats).
+    With fb given, renders into it and returns the stats only."""
     import softrast_amd as sr
     if flags is None:
         flags = sr.CFG_KEEP_PRIMID | sr.CFG_STATS
     own = ctx is None
     if own:
         ctx = sr.RenderContext(flags=flags, **ctx_kw)
-    fb = sr.FrameBuffer(ctx, scene.width, scene.height)
+    own_fb = fb is None
+    if own_fb:
+        fb = sr.FrameBuffer(ctx, scene.width, scene.height)
     if plane_init is not None:
         fb.write_tiles(plane_init[0], plane_init[1])
     ctx.BeginFrame()
@@ -117,6 +120,8 @@ def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, *
         ctx.DrawIndexed(call)
     ctx.EndFrame()
     stats = ctx.frame_stats()
+    if not own_fb:
+        return stats
     res = fb.read_tiles(primid=True) if flags & sr.CFG_KEEP_PRIMID else fb.read_tiles() + (None,)
     fb.destroy()
     if own:

@@ -231,3 +231,58 @@ def test_sched_model_roundtrip_and_order_only():
             assert np.array_equal(a, b)
         assert out[3]["sched_trained"] == 1
     ctx.Shutdown()
+
+
+@pytest.mark.gpu
+def test_async_blit_overlaps_next_frame_and_matches_sync_blit():
+    """Blit with the reference's asynchrony (Renderer.cpp:350-372): the
+    callback fires with the rows in place while the next frame renders into
+    the other plane; the rows equal the synchronous blit of the same frame;
+    BGRA exchanges red and blue (Window_Win32.cpp:85-112)."""
+    import threading
+    import softrast_amd as sr
+    base = helpers.random_scene(51, 2000, 300, 170, ndraws=1, shaders=(1,), with_texture=False)
+    other = helpers.random_scene(52, 2000, 300, 170, ndraws=1, shaders=(1,), with_texture=False)
+    _say("[parity] case=async_blit")
+    ctx = sr.RenderContext()
+    fb = sr.FrameBuffer(ctx, base.width, base.height)
+    fired = threading.Event()
+    rows = []
+    for bgra in (False, True):
+        for sc in (base, other):
+            helpers.gpu_render(sc, [], ctx=ctx, fb=fb, flags=0)
+            rows.append(ctx.BlitAsync(fb, onFinish=lambda u: fired.set(), bgra=bgra))
+        ctx.BlitWait(fb)
+    assert fired.is_set()
+    sync = []
+    for sc in (base, other):
+        helpers.gpu_render(sc, [], ctx=ctx, fb=fb, flags=0)
+        sync.append(ctx.Blit(fb).copy())
+    for i in range(2):
+        assert np.array_equal(rows[i], sync[i])
+        a, b = rows[2 + i].view(np.uint8).reshape(-1, 4), sync[i].view(np.uint8).reshape(-1, 4)
+        assert np.array_equal(a[:, [2, 1, 0, 3]], b)
+    assert not np.array_equal(sync[0], sync[1])
+    ctx.Shutdown()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile_order", [0, 1, 2])
+def test_clustered_scene_every_order(tile_order):
+    """The clustered layout of the learned-scheduler workload (90% of the
+    triangles in 3 clusters, areas 0.5-2000 px^2) at reduced size: every queue
+    order gives the oracle's image."""
+    import softrast_amd as sr
+    from softrast_amd import scenes
+    sc = scenes.clustered_scene(ntris=120000, width=1280, height=720, ndraws=4, tex_size=256)
+    tex = helpers.build_textures(sc)
+    _say(f"[parity] case=clustered order={tile_order} tris={sc.num_tris}")
+    ref, rstats = helpers.oracle_render(sc, tex, nthreads=8)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=tile_order, sched_warmup_frames=1)
+    for frame in range(3):
+        col, dep, pid, st = helpers.gpu_render(sc, tex, ctx=ctx)
+        d = helpers.compare(ref, col, dep, pid)
+        _say(f"[parity] clustered order={tile_order} frame={frame} diffs={d}")
+        assert not any(d.values()), d
+    assert st["frags_raster"] == rstats["frags_raster"]
+    ctx.Shutdown()
