@@ -149,8 +149,10 @@ __device__ __forceinline__ uint32_t key_slot(uint32_t x, uint32_t y)
 // block range inside the work bin at (sx0, sy0).
 struct EntryRaster {
     uint32_t ct[3], dxu[3], dyu[3];
-    uint32_t xs, xe, ys, ye;
-    uint32_t cx0, cx1, cy0, cy1;  // tight primitives: pixels that can be covered, tile-relative, ends exclusive
+    uint32_t xs, xe, ys, ye;      // the reference's walk: blocks from (xs, ys) while xb < xe, yb < ye
+    uint32_t cx0, cx1, cy0, cy1;  // pixels that can be covered (tile-relative, ends exclusive): the walked
+                                  // pixels, cut to the tight bound where the primitive has one
+    uint32_t bx0, by0, nbx, nby;  // the 8x8 blocks holding those pixels
     bool tight;
     float zdx, zdy, zc0;
     unsigned long long low;
@@ -172,15 +174,26 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
     e.xe = min(bmaxx, sx0 + S);
     e.ys = max(bminy & ~7u, sy0);
     e.ye = min(bmaxy, sy0 + S);
-    // walked pixels (whole blocks from xs while xb < xe) within the tight bound
+    // walked pixels: whole blocks from xs while xb < xe (xs, ys are multiples of 8)
     e.tight = ((q2.y >> 14) & 1u) != 0;
     const int32_t wx1 = e.xe > e.xs ? (int32_t)(e.xs + ((e.xe - e.xs + 7u) & ~7u)) : (int32_t)e.xs;
     const int32_t wy1 = e.ye > e.ys ? (int32_t)(e.ys + ((e.ye - e.ys + 7u) & ~7u)) : (int32_t)e.ys;
-    e.cx0 = (uint32_t)max((int32_t)e.xs, xmin - (int32_t)((q2.y >> 15) & 1u) - X0);
-    e.cy0 = (uint32_t)max((int32_t)e.ys, ymin - (int32_t)((q2.z >> 14) & 1u) - Y0);
-    const int32_t bx1 = ((q2.y >> 30) & 1u) ? wx1 : xmax - X0 + 1, by1 = ((q2.z >> 30) & 1u) ? wy1 : ymax - Y0 + 1;
-    e.cx1 = (uint32_t)max((int32_t)e.cx0, min(wx1, bx1));
-    e.cy1 = (uint32_t)max((int32_t)e.cy0, min(wy1, by1));
+    if (e.tight) {
+        e.cx0 = (uint32_t)max((int32_t)e.xs, xmin - (int32_t)((q2.y >> 15) & 1u) - X0);
+        e.cy0 = (uint32_t)max((int32_t)e.ys, ymin - (int32_t)((q2.z >> 14) & 1u) - Y0);
+        const int32_t bx1 = ((q2.y >> 30) & 1u) ? wx1 : xmax - X0 + 1, by1 = ((q2.z >> 30) & 1u) ? wy1 : ymax - Y0 + 1;
+        e.cx1 = (uint32_t)max((int32_t)e.cx0, min(wx1, bx1));
+        e.cy1 = (uint32_t)max((int32_t)e.cy0, min(wy1, by1));
+    } else {
+        e.cx0 = e.xs;
+        e.cy0 = e.ys;
+        e.cx1 = (uint32_t)wx1;
+        e.cy1 = (uint32_t)wy1;
+    }
+    e.bx0 = e.cx0 & ~7u;
+    e.by0 = e.cy0 & ~7u;
+    e.nbx = e.cx1 > e.cx0 ? ((e.cx1 - 1u) >> 3) - (e.cx0 >> 3) + 1u : 0u;
+    e.nby = e.cy1 > e.cy0 ? ((e.cy1 - 1u) >> 3) - (e.cy0 >> 3) + 1u : 0u;
     const float r0x = ffrom(q2.w), r0y = ffrom(q3.x);
     e.zdx = ffrom(q3.y);
     e.zdy = ffrom(q3.z);
@@ -205,9 +218,9 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
 // only where some edge value is exactly 0: with exact edge values a rejected
 // block has no pixel with all three values positive, and a fully accepted
 // block no pixel with any negative.
-template <uint32_t S>
+template <uint32_t S, bool STATS>
 __device__ __forceinline__ void raster_wave(unsigned long long* keys, const EntryRaster& e, uint32_t nb, uint32_t sx0,
-                                            uint32_t sy0, bool stats, unsigned long long& frags)
+                                            uint32_t sy0, unsigned long long& frags)
 {
     const uint32_t lane = lane_id(), g = lane >> 3, l = lane & 7u;
     uint32_t incl = nb;
@@ -218,9 +231,9 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     }
     const uint32_t total = bcast(incl, 63);
     // per-entry values the block workers need, packed
-    const uint32_t nbx = e.xe > e.xs ? (e.xe - e.xs + 7u) >> 3 : 1u;
+    const uint32_t nbx = max(e.nbx, 1u);
     const uint32_t magic = (1024u + nbx - 1u) / nbx;  // blk / nbx == (blk * magic) >> 10 for blk < 64
-    const uint32_t pk_blk = e.xs | (e.ys << 7) | (nbx << 14) | ((e.tight ? 1u : 0u) << 18) | (magic << 19);
+    const uint32_t pk_blk = e.bx0 | (e.by0 << 7) | (nbx << 14) | ((e.tight ? 1u : 0u) << 18) | (magic << 19);
     const uint32_t pk_cand = e.cx0 | (e.cx1 << 8) | (e.cy0 << 16) | (e.cy1 << 24);
     const uint32_t first = incl - nb;
     const uint32_t low = (uint32_t)e.low;
@@ -268,7 +281,7 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         for (uint32_t r = r_lo; r < r_hi; ++r) {
             bool cov = col && cls != 0u && (cls == 2u || (int32_t)(E[0] | E[1] | E[2]) >= 0);
             if (cov && tight && min(min(E[0], E[1]), E[2]) == 0u) cov = block_class(ct, dxu, dyu, xb, yb) != 0u;
-            if (stats) frags += cov ? 1u : 0u;
+            if (STATS) frags += cov ? 1u : 0u;
             if (cov && z > 0.0f)
                 atomicMax(&keys[key_slot<S>(sxl, yb + r - sy0)], ((unsigned long long)fbits(z) << 32) | lo);
             z = z + zdy;
@@ -278,7 +291,7 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     }
 }
 
-template <uint32_t S>
+template <uint32_t S, bool STATS>
 __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
 {
     constexpr uint32_t K = kBin / S;
@@ -319,7 +332,6 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
         __syncthreads();
 
         const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
-        const bool stats = (F.flags & FF_STATS) != 0;
         for (uint32_t b0 = 0; b0 + wave < cnt; b0 += kBackendThreads) {
             // lane j of wave w holds entry b0 + w + 4j
             const uint32_t my = b0 + wave + kBackendWaves * lane;
@@ -333,9 +345,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
                 order = r < F.ntris ? ((r << 3) | kFanUnclipped) : F.clip_order[r - F.ntris];
             }
             const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S);
-            const uint32_t nbx = e.xe > e.xs ? (e.xe - e.xs + 7u) >> 3 : 0u;
-            const uint32_t nby = e.ye > e.ys ? (e.ye - e.ys + 7u) >> 3 : 0u;
-            raster_wave<S>(keys, e, has ? nbx * nby : 0u, sx0, sy0, stats, frags);
+            raster_wave<S, STATS>(keys, e, has ? e.nbx * e.nby : 0u, sx0, sy0, frags);
         }
         __syncthreads();
 
@@ -374,7 +384,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
         if ((F.flags & FF_TILE_COSTS) && tid == 0) F.bin_cost[gb] = clock64() - t_start;
         __syncthreads();
     }
-    if (F.flags & FF_STATS) {
+    if (STATS) {
         // per-lane counts -> wave total -> one atomic per wave
         uint32_t lo = (uint32_t)frags, hi = (uint32_t)(frags >> 32);
         lo = wave_sum(lo);

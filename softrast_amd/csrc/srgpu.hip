@@ -362,10 +362,20 @@ int launch_backend(srg_ctx* ctx, uint32_t k, const FrameDev& F)
 {
     const uint32_t wg = ctx->cfg.wg_per_cu ? ctx->cfg.wg_per_cu : (k == 1 ? 4u : 8u);
     const dim3 grid((uint32_t)ctx->num_cus * wg), block(kBackendThreads);
+    const bool stats = (F.flags & FF_STATS) != 0;
     switch (k) {
-    case 1: hipLaunchKernelGGL(k_backend<64>, grid, block, 0, ctx->stream, F); break;
-    case 4: hipLaunchKernelGGL(k_backend<16>, grid, block, 0, ctx->stream, F); break;
-    default: hipLaunchKernelGGL(k_backend<32>, grid, block, 0, ctx->stream, F); break;
+    case 1:
+        if (stats) hipLaunchKernelGGL((k_backend<64, true>), grid, block, 0, ctx->stream, F);
+        else hipLaunchKernelGGL((k_backend<64, false>), grid, block, 0, ctx->stream, F);
+        break;
+    case 4:
+        if (stats) hipLaunchKernelGGL((k_backend<16, true>), grid, block, 0, ctx->stream, F);
+        else hipLaunchKernelGGL((k_backend<16, false>), grid, block, 0, ctx->stream, F);
+        break;
+    default:
+        if (stats) hipLaunchKernelGGL((k_backend<32, true>), grid, block, 0, ctx->stream, F);
+        else hipLaunchKernelGGL((k_backend<32, false>), grid, block, 0, ctx->stream, F);
+        break;
     }
     return SRG_OK;
 }

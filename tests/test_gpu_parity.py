@@ -286,3 +286,37 @@ def test_clustered_scene_every_order(tile_order):
         assert not any(d.values()), d
     assert st["frags_raster"] == rstats["frags_raster"]
     ctx.Shutdown()
+
+
+def _sliver_scene(seed, n, width, height):
+    """Needle-thin triangles (0.0005-0.008 px wide, 2-40 px long) in clip
+    space, both windings: mostly primitives without a tight pixel bound,
+    whose accepted pixels the fill bias can push outside their bbox."""
+    from softrast_amd import scenes
+    rng = np.random.default_rng(seed)
+    p0 = rng.uniform([0, 0], [width, height], (n, 2))
+    ang = rng.uniform(0, 2 * np.pi, n)
+    d = np.stack([np.cos(ang), np.sin(ang)], 1) * rng.uniform(2, 40, n)[:, None]
+    nrm = np.stack([-d[:, 1], d[:, 0]], 1) / np.linalg.norm(d, axis=1)[:, None]
+    w = rng.uniform(0.0005, 0.008, n)[:, None]
+    t = rng.uniform(0.2, 0.8, n)[:, None]
+    p1 = p0 + d
+    p2 = p0 + d * t + nrm * w
+    flip = rng.random(n) < 0.5
+    p1[flip], p2[flip] = p2[flip].copy(), p1[flip].copy()
+    pts = np.stack([p0, p1, p2], 1).reshape(-1, 2)
+    v = np.zeros(n * 3, scenes.VERTEX_DTYPE)
+    v["pos"][:, 0] = 2 * pts[:, 0] / width - 1
+    v["pos"][:, 1] = 1 - 2 * pts[:, 1] / height
+    v["pos"][:, 2] = np.repeat(rng.uniform(0.05, 0.95, n), 3)
+    v["nrm"] = rng.normal(0, 1, (n * 3, 3))
+    v["uv"] = rng.uniform(0, 2, (n * 3, 2))
+    draw = scenes.Draw(v, np.arange(n * 3, dtype=np.uint32), -1, 1)
+    return scenes.Scene(width, height, np.eye(4, dtype=np.float32).reshape(-1), [draw], [], {"config": "slivers"})
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bin_size", [16, 32, 64])
+def test_needle_slivers(bin_size):
+    sc = _sliver_scene(61, 20000, 400, 300)
+    _check(sc, f"slivers_bin{bin_size}", bin_size=bin_size)
