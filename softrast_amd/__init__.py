@@ -360,6 +360,7 @@ class RenderContext:
         self._textures = {}
         self._keep = []
         self._blit_keep = {}
+        self._gen = 0
         self._draws = []
 
     # -- resources ---------------------------------------------------------
@@ -380,6 +381,7 @@ class RenderContext:
         h = self._buffers.pop(key, None)
         if h is not None:
             self._lib.srg_release_buffer(self._ctx, h)
+        self._gen += 1  # draws prepared with the old handle are rebuilt
 
     def texture_handle(self, tex):
         if tex is None:
@@ -412,8 +414,26 @@ class RenderContext:
                "ClearFrameBuffer")
 
     def DrawIndexed(self, call):
-        if call.m_indexBuffer.m_ptr is None:
+        ib, pb, ab = call.m_indexBuffer, call.m_positionBuffer, call.m_attributeBuffer
+        if ib.m_ptr is None:
             raise SrgError(-1, "DrawIndexed: no index buffer bound")
+        mvp = np.asarray(call.m_mvp, np.float32).reshape(-1)
+        # the srg_draw of a DrawCall is rebuilt only when its state changed
+        key = (id(ib.m_ptr), ib.m_stride, ib.m_num, id(pb.m_ptr), pb.m_stride, pb.m_num, id(ab.m_ptr), ab.m_stride,
+               ab.m_num, call.m_uvOffset, id(call.m_pixelShader), id(call.m_pixelUniforms), mvp.tobytes(), self._gen)
+        cached = call.__dict__.get("_srg_draw")
+        if cached is not None and cached[0] is self and cached[1] == key:
+            d = cached[2]
+        else:
+            d = self._prepare_draw(call, mvp)
+            call._srg_draw = (self, key, d)
+        fb, plane = call.m_frameBuffer
+        d.fb, d.plane = fb._h, plane
+        _check(self._lib.srg_draw_indexed(self._ctx, ctypes.byref(d)), "DrawIndexed")
+        call.m_drawCallIdx = len(self._draws)
+        self._draws.append(call)
+
+    def _prepare_draw(self, call, mvp):
         d = SrgDraw()
         shader = call.m_pixelShader if call.m_pixelShader is not None else VisualizeNormalsShader
         d.shader = shader.id
@@ -428,12 +448,8 @@ class RenderContext:
             setattr(d, name + "_stride", buf.m_stride)
             setattr(d, name + "_num", buf.m_num)
         d.uv_offset = call.m_uvOffset
-        fb, plane = call.m_frameBuffer
-        d.fb, d.plane = fb._h, plane
-        d.mvp[:] = [float(x) for x in np.asarray(call.m_mvp, np.float32).reshape(-1)]
-        _check(self._lib.srg_draw_indexed(self._ctx, ctypes.byref(d)), "DrawIndexed")
-        call.m_drawCallIdx = len(self._draws)
-        self._draws.append(call)
+        d.mvp[:] = [float(x) for x in mvp]
+        return d
 
     def EndFrame(self):
         _check(self._lib.srg_end_frame(self._ctx), "EndFrame")

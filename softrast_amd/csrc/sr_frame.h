@@ -15,6 +15,8 @@ struct DrawDev {
     uint32_t pos_num, attr_num, tri_base, view;
     uint32_t shader, tex, nslots, deriv_slot;  // deriv_slot: first slot of (u, v) at uv_offset
     uint32_t slot_attr[kMaxSlots];
+    uint32_t block_base;  // first k_setup workgroup of the draw (kSetupThreads triangles each)
+    uint32_t pad[3];
     float mvp[16];
 };
 
@@ -68,8 +70,6 @@ struct FrameDev {
     const DrawDev* draws;
     const ViewDev* views;
     const TexDev* texs;
-    const uint32_t* block_draw;
-    const uint32_t* block_tri0;
     uint32_t ndraws, nviews, total_bins, ntris;
     uint32_t bin_k, bin_size, clip_cap, entry_cap;
     uint4* rec_raster;     // 4 x uint4 per record: edges, bbox, r0, z plane

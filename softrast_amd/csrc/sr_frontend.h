@@ -323,10 +323,16 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
     const bool stats = (F.flags & FF_STATS) != 0;
     if (stats && threadIdx.x == 0) bs = BlockStats{ 0, 0 };
     if (stats) __syncthreads();
-    const uint32_t d = F.block_draw[blockIdx.x];
+    // the draw of this workgroup: the last with block_base <= blockIdx.x
+    uint32_t d = 0, hi = F.ndraws;
+    while (hi - d > 1) {
+        const uint32_t mid = (d + hi) >> 1;
+        if (F.draws[mid].block_base <= blockIdx.x) d = mid;
+        else hi = mid;
+    }
     const DrawDev& D = F.draws[d];
     const ViewDev& V = F.views[D.view];
-    const uint32_t tri = F.block_tri0[blockIdx.x] + threadIdx.x;
+    const uint32_t tri = (blockIdx.x - D.block_base) * kSetupThreads + threadIdx.x;
     const bool valid = tri < D.ntris;
     const uint32_t g = D.tri_base + tri;
 
@@ -503,50 +509,120 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
     }
 }
 
-// Exclusive scan of bin counts and construction of the back end's queue: every
-// non-empty bin, plus every bin of a view with a clear pending (one workgroup).
+// Exclusive scan of bin counts and the back end's queue (every non-empty bin,
+// plus every bin of a view with a clear pending), in three coalesced passes
+// over kScanThreads-bin blocks: block totals, a scan of the totals, and the
+// block-local scan plus the block's offset.
 constexpr uint32_t kScanThreads = 1024;
 
-__global__ __launch_bounds__(kScanThreads) void k_scan(FrameDev F)
+// Exclusive scan of (a, b) over the workgroup; returns the totals in ta, tb.
+// All threads of a kScanThreads workgroup must call it.
+__device__ __forceinline__ void block_scan2(uint32_t& a, uint32_t& b, uint32_t& ta, uint32_t& tb)
 {
-    __shared__ uint32_t s_cnt[kScanThreads];
-    __shared__ uint32_t s_q[kScanThreads];
-    const uint32_t T = F.total_bins;
-    const uint32_t per = (T + kScanThreads - 1) / kScanThreads;
-    const uint32_t b = threadIdx.x * per, e = min(T, b + per);
-    uint32_t v = 0;
-    while (v + 1 < F.nviews && F.views[v + 1].bin_base <= b) ++v;
-    uint32_t sum = 0, nq = 0;
-    for (uint32_t t = b, vv = v; t < e; ++t) {
-        while (vv + 1 < F.nviews && F.views[vv + 1].bin_base <= t) ++vv;
-        const uint32_t c = F.bin_count[t];
-        sum += c;
-        nq += (c || (F.views[vv].flags & (VF_CLEAR_COLOUR | VF_CLEAR_DEPTH))) ? 1u : 0u;
+    __shared__ uint32_t s_a[kScanThreads / 64], s_b[kScanThreads / 64];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t ia = a, ib = b;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t ya = __shfl_up(ia, o, 64), yb = __shfl_up(ib, o, 64);
+        if (lane >= o) {
+            ia += ya;
+            ib += yb;
+        }
     }
-    s_cnt[threadIdx.x] = sum;
-    s_q[threadIdx.x] = nq;
+    if (lane == 63) {
+        s_a[w] = ia;
+        s_b[w] = ib;
+    }
     __syncthreads();
-    for (uint32_t o = 1; o < kScanThreads; o <<= 1) {
-        const uint32_t a1 = threadIdx.x >= o ? s_cnt[threadIdx.x - o] : 0u;
-        const uint32_t a2 = threadIdx.x >= o ? s_q[threadIdx.x - o] : 0u;
-        __syncthreads();
-        s_cnt[threadIdx.x] += a1;
-        s_q[threadIdx.x] += a2;
-        __syncthreads();
+    uint32_t oa = 0, ob = 0;
+    ta = 0;
+    tb = 0;
+    for (uint32_t i = 0; i < kScanThreads / 64; ++i) {
+        const uint32_t va = s_a[i], vb = s_b[i];
+        if (i < w) {
+            oa += va;
+            ob += vb;
+        }
+        ta += va;
+        tb += vb;
     }
-    uint32_t run = s_cnt[threadIdx.x] - sum, qrun = s_q[threadIdx.x] - nq;
-    for (uint32_t t = b, vv = v; t < e; ++t) {
-        while (vv + 1 < F.nviews && F.views[vv + 1].bin_base <= t) ++vv;
-        const uint32_t c = F.bin_count[t];
-        F.bin_start[t] = run;
-        run += c;
-        if (c || (F.views[vv].flags & (VF_CLEAR_COLOUR | VF_CLEAR_DEPTH))) F.queue[qrun++] = t;
+    __syncthreads();
+    a = oa + ia - a;
+    b = ob + ib - b;
+}
+
+__device__ __forceinline__ bool view_cleared(const FrameDev& F, uint32_t bin)
+{
+    uint32_t lo = 0, hi = F.nviews;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (F.views[mid].bin_base <= bin) lo = mid;
+        else hi = mid;
     }
-    if (threadIdx.x == kScanThreads - 1) {
-        F.bin_start[T] = s_cnt[kScanThreads - 1];
-        F.counters[C_ENTRIES] = s_cnt[kScanThreads - 1];
-        F.counters[C_QLEN] = s_q[kScanThreads - 1];
-        if (s_cnt[kScanThreads - 1] > F.entry_cap) atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
+    return (F.views[lo].flags & (VF_CLEAR_COLOUR | VF_CLEAR_DEPTH)) != 0;
+}
+
+__device__ __forceinline__ void scan_inputs(const FrameDev& F, uint32_t t, uint32_t& c, uint32_t& q)
+{
+    c = 0;
+    q = 0;
+    if (t < F.total_bins) {
+        c = F.bin_count[t];
+        q = (c || view_cleared(F, t)) ? 1u : 0u;
+    }
+}
+
+// Pass 1: per block of bins, the entry and queue totals.
+__global__ __launch_bounds__(kScanThreads) void k_scan_blocks(FrameDev F, uint32_t* blk)
+{
+    uint32_t c, q, tc, tq;
+    scan_inputs(F, blockIdx.x * kScanThreads + threadIdx.x, c, q);
+    block_scan2(c, q, tc, tq);
+    if (threadIdx.x == 0) {
+        blk[2 * blockIdx.x] = tc;
+        blk[2 * blockIdx.x + 1] = tq;
+    }
+}
+
+// Pass 2 (one workgroup): block offsets and the frame totals.
+__global__ __launch_bounds__(kScanThreads) void k_scan_top(FrameDev F, uint32_t* blk, uint32_t nblk)
+{
+    uint32_t tc = 0, tq = 0;
+    for (uint32_t base = 0; base < nblk; base += kScanThreads) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t c = 0, q = 0, bc, bq;
+        if (i < nblk) {
+            c = blk[2 * i];
+            q = blk[2 * i + 1];
+        }
+        block_scan2(c, q, bc, bq);
+        if (i < nblk) {
+            blk[2 * i] = tc + c;
+            blk[2 * i + 1] = tq + q;
+        }
+        tc += bc;
+        tq += bq;
+    }
+    if (threadIdx.x == 0) {
+        F.bin_start[F.total_bins] = tc;
+        F.counters[C_ENTRIES] = tc;
+        F.counters[C_QLEN] = tq;
+        if (tc > F.entry_cap) atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
+    }
+}
+
+// Pass 3: bin starts and queue slots.
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(FrameDev F, const uint32_t* blk)
+{
+    const uint32_t t = blockIdx.x * kScanThreads + threadIdx.x;
+    uint32_t c, q, tc, tq;
+    scan_inputs(F, t, c, q);
+    const bool enq = q != 0;
+    block_scan2(c, q, tc, tq);
+    if (t < F.total_bins) {
+        F.bin_start[t] = blk[2 * blockIdx.x] + c;
+        if (enq) F.queue[blk[2 * blockIdx.x + 1] + q] = t;
     }
 }
 
@@ -575,13 +651,11 @@ __global__ __launch_bounds__(kScanThreads) void k_order(FrameDev F, const uint32
         atomicAdd(&hist[cost_bucket(pred ? pred[gb] : (float)F.bin_count[gb])], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {  // descending exclusive scan: largest bucket first
-        uint32_t run = 0;
-        for (int b = (int)kOrderBuckets - 1; b >= 0; --b) {
-            const uint32_t c = hist[b];
-            hist[b] = run;
-            run += c;
-        }
+    {  // descending exclusive scan: largest bucket first
+        const uint32_t bkt = kOrderBuckets - 1u - threadIdx.x;
+        uint32_t c = threadIdx.x < kOrderBuckets ? hist[bkt] : 0u, unused = 0u, tc, tu;
+        block_scan2(c, unused, tc, tu);
+        if (threadIdx.x < kOrderBuckets) hist[bkt] = c;
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {

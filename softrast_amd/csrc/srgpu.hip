@@ -5,7 +5,7 @@
 //   k_frame_init  zero counters and per-bin arrays (a fill kernel, no memset)
 //   k_setup       one lane per submitted triangle (sr_frontend.h)
 //   k_clip        slow path for triangles crossing a clip plane
-//   k_scan        bin offsets and the back end's work queue
+//   k_scan_*      bin offsets and the back end's work queue (three passes)
 //   k_emit        bin lists
 //   k_backend<S>  persistent raster + resolve + shade per S x S bin (sr_backend.h)
 //
@@ -174,7 +174,7 @@ struct srg_ctx {
     DevArray<uint4> rec_raster;
     DevArray<float4> rec_shade;
     DevArray<uint32_t> rec_meta, clip_queue, clip_base, clip_order;
-    DevArray<uint32_t> bin_count, bin_start, bin_cursor, queue, queue2, entries;
+    DevArray<uint32_t> bin_count, bin_start, bin_cursor, queue, queue2, entries, scan_blk;
     DevArray<uint64_t> bin_cost;
     DevArray<uint32_t> counters;
     DevArray<uint32_t> blit_staging;
@@ -427,8 +427,8 @@ int run_frame(srg_ctx* ctx)
     std::vector<DrawDev> ddev(ctx->draws.size());
     std::vector<TexDev> tdev;
     std::unordered_map<uint64_t, uint32_t> tex_index;
-    std::vector<uint32_t> block_draw, block_tri0;
     uint64_t total_tris = 0;
+    uint32_t nblocks = 0;
     for (size_t i = 0; i < ctx->draws.size(); ++i) {
         const srg_draw& s = ctx->draws[i];
         DrawDev& d = ddev[i];
@@ -471,37 +471,29 @@ int run_frame(srg_ctx* ctx)
             }
         }
         std::memcpy(d.mvp, s.mvp, sizeof(d.mvp));
-        for (uint32_t t0 = 0; t0 < d.ntris; t0 += kSetupThreads) {
-            block_draw.push_back((uint32_t)i);
-            block_tri0.push_back(t0);
-        }
+        d.block_base = nblocks;
+        nblocks += (d.ntris + kSetupThreads - 1) / kSetupThreads;
         total_tris += d.ntris;
     }
     if (total_tris >= (1ull << 29)) return SRG_ERR_INVALID;  // order key is (triangle << 3) | fan
     const uint32_t NT = (uint32_t)total_tris;
 
-    const uint32_t nblocks = (uint32_t)block_draw.size();
     size_t off = 0;
     const size_t o_draw = off; off = align_up(off + sizeof(DrawDev) * std::max<size_t>(ddev.size(), 1), 256);
     const size_t o_view = off; off = align_up(off + sizeof(ViewDev) * vdev.size(), 256);
     const size_t o_tex = off;  off = align_up(off + sizeof(TexDev) * std::max<size_t>(tdev.size(), 1), 256);
-    const size_t o_bd = off;   off = align_up(off + sizeof(uint32_t) * std::max<uint32_t>(nblocks, 1), 256);
-    const size_t o_bt = off;   off = align_up(off + sizeof(uint32_t) * std::max<uint32_t>(nblocks, 1), 256);
     int st = ensure_host_tables(ctx, off);
     if (st) return st;
     if ((st = ctx->tables.ensure(off))) return st;
     if (!ddev.empty()) std::memcpy(ctx->h_tables + o_draw, ddev.data(), sizeof(DrawDev) * ddev.size());
     std::memcpy(ctx->h_tables + o_view, vdev.data(), sizeof(ViewDev) * vdev.size());
     if (!tdev.empty()) std::memcpy(ctx->h_tables + o_tex, tdev.data(), sizeof(TexDev) * tdev.size());
-    if (nblocks) {
-        std::memcpy(ctx->h_tables + o_bd, block_draw.data(), sizeof(uint32_t) * nblocks);
-        std::memcpy(ctx->h_tables + o_bt, block_tri0.data(), sizeof(uint32_t) * nblocks);
-    }
 
     // ---- arenas ----
     const uint32_t TB = (uint32_t)std::max<uint64_t>(total_bins, 1);
     if ((st = ctx->bin_count.ensure(TB)) || (st = ctx->bin_start.ensure(TB + 1)) || (st = ctx->bin_cursor.ensure(TB)) ||
-        (st = ctx->queue.ensure(TB)) || (st = ctx->queue2.ensure(TB)) || (st = ctx->bin_cost.ensure(TB)) || (st = ctx->counters.ensure(C_NCOUNTERS)) ||
+        (st = ctx->queue.ensure(TB)) || (st = ctx->queue2.ensure(TB)) ||
+        (st = ctx->scan_blk.ensure(2 * ((TB + kScanThreads - 1) / kScanThreads))) || (st = ctx->bin_cost.ensure(TB)) || (st = ctx->counters.ensure(C_NCOUNTERS)) ||
         (st = ctx->clip_queue.ensure(std::max<uint32_t>(NT, 1))) || (st = ctx->clip_base.ensure(std::max<uint32_t>(NT, 1))))
         return st;
     uint64_t want_clip = ctx->cfg.max_prims ? ctx->cfg.max_prims : (uint64_t)NT / 8 + 4096;
@@ -527,8 +519,6 @@ int run_frame(srg_ctx* ctx)
         F.draws = (const DrawDev*)(ctx->tables.ptr + o_draw);
         F.views = (const ViewDev*)(ctx->tables.ptr + o_view);
         F.texs = (const TexDev*)(ctx->tables.ptr + o_tex);
-        F.block_draw = (const uint32_t*)(ctx->tables.ptr + o_bd);
-        F.block_tri0 = (const uint32_t*)(ctx->tables.ptr + o_bt);
         F.ndraws = (uint32_t)ddev.size();
         F.nviews = (uint32_t)vdev.size();
         F.total_bins = (uint32_t)total_bins;
@@ -557,6 +547,7 @@ int run_frame(srg_ctx* ctx)
         const uint32_t emit_blocks = std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 16384));
         const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + kClipThreads - 1) / kClipThreads, 2048));
         const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((TB + 255) / 256, 1024));
+        const uint32_t scan_blocks = (TB + kScanThreads - 1) / kScanThreads;
         const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
         const bool learned = ctx->cfg.tile_order == SRG_ORDER_LEARNED;
         if (learned) {
@@ -578,7 +569,10 @@ int run_frame(srg_ctx* ctx)
             if (NT) hipLaunchKernelGGL(k_clip, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
             FrameDev Fs = F;  // with ordering, the natural queue goes to the spare buffer first
             if (ordered) Fs.queue = ctx->queue2.ptr;
-            hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs);
+            hipLaunchKernelGGL(k_scan_blocks, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs, ctx->scan_blk.ptr);
+            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs, ctx->scan_blk.ptr, scan_blocks);
+            hipLaunchKernelGGL(k_scan_apply, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs,
+                               (const uint32_t*)ctx->scan_blk.ptr);
             hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
             if (learned) {
                 hipLaunchKernelGGL(k_bin_features, dim3((TB + 3) / 4), dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr);
@@ -761,6 +755,7 @@ void srg_destroy(srg_ctx* ctx)
     ctx->bin_cursor.release();
     ctx->queue.release();
     ctx->queue2.release();
+    ctx->scan_blk.release();
     ctx->entries.release();
     ctx->bin_cost.release();
     ctx->counters.release();
