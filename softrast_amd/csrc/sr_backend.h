@@ -203,11 +203,12 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
     return e;
 }
 
-// Rasterisation of a wave's 64 entries (one per lane, `e`, `nb` walked 8x8
-// blocks each). The (entry, block) pairs form a queue that the wave takes 8
-// at a time: lanes 8g..8g+7 take one pair, one lane per pixel column, and
-// walk the block's rows, carrying the reference's per-column depth down the
-// rows (ComputeBlockMask8x8, Rasterizer.cpp:134-192): z = fma(yb, dy,
+// Rasterisation of a wave's 64 entries (one per lane, `e`). Every entry's
+// candidate pixels (cx0..cx1 x cy0..cy1) split into columns of 8x8 block rows:
+// cw columns in each of its block rows. These (entry, block row, column)
+// items form a queue the wave takes 64 at a time, one lane per item; the lane
+// walks its column down the block row, carrying the reference's per-column
+// depth (ComputeBlockMask8x8, Rasterizer.cpp:134-192): z = fma(yb, dy,
 // fma(l, dx, c0)) + xb*dx at the block's top row, then + dy per row, and the
 // edge values + dx_k per row.
 //
@@ -219,34 +220,39 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
 // block has no pixel with all three values positive, and a fully accepted
 // block no pixel with any negative.
 template <uint32_t S, bool STATS>
-__device__ __forceinline__ void raster_wave(unsigned long long* keys, const EntryRaster& e, uint32_t nb, uint32_t sx0,
+__device__ __forceinline__ void raster_wave(unsigned long long* keys, const EntryRaster& e, bool has, uint32_t sx0,
                                             uint32_t sy0, unsigned long long& frags)
 {
-    const uint32_t lane = lane_id(), g = lane >> 3, l = lane & 7u;
-    uint32_t incl = nb;
+    const uint32_t lane = lane_id();
+    const uint32_t cw = e.cx1 > e.cx0 ? e.cx1 - e.cx0 : 0u;
+    const uint32_t n = has ? e.nby * cw : 0u;
+    uint32_t incl = n;
 #pragma unroll
     for (uint32_t o = 1; o < 64; o <<= 1) {
         const uint32_t y = __shfl_up(incl, o, 64);
         if (lane >= o) incl += y;
     }
     const uint32_t total = bcast(incl, 63);
-    // per-entry values the block workers need, packed
-    const uint32_t nbx = max(e.nbx, 1u);
-    const uint32_t magic = (1024u + nbx - 1u) / nbx;  // blk / nbx == (blk * magic) >> 10 for blk < 64
-    const uint32_t pk_blk = e.bx0 | (e.by0 << 7) | (nbx << 14) | ((e.tight ? 1u : 0u) << 18) | (magic << 19);
-    const uint32_t pk_cand = e.cx0 | (e.cx1 << 8) | (e.cy0 << 16) | (e.cy1 << 24);
-    const uint32_t first = incl - nb;
+    // per-entry values the column workers need, packed
+    const uint32_t magic = (65536u + max(cw, 1u) - 1u) / max(cw, 1u);  // k / cw == (k * magic) >> 16 for k < 512
+    const uint32_t pk_cand = e.cx0 | (cw << 7) | (e.cy0 << 14) | (e.cy1 << 21) | ((e.tight ? 1u : 0u) << 28);
+    const uint32_t pk_div = e.by0 | (magic << 7);
+    const uint32_t first = incl - n;
     const uint32_t low = (uint32_t)e.low;
-    for (uint32_t base = 0; base < total; base += 8) {
-        uint32_t owner = 0;
-#pragma unroll
-        for (uint32_t gg = 0; gg < 8; ++gg) {
-            const uint32_t o = (uint32_t)__popcll(__ballot(incl <= base + gg));
-            owner = g == gg ? o : owner;
-        }
-        const uint32_t it = base + g;
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t it = base + lane;
         const bool valid = it < total;
-        const int src = (int)(valid ? owner : 0u);
+        // owner: the first lane whose inclusive count exceeds it (it < total =
+        // incl of lane 63, so the answer is in [0, 63]; 6 halvings of 64 candidates)
+        uint32_t lo = 0, hi = 63;
+#pragma unroll
+        for (int step = 0; step < 6; ++step) {
+            const uint32_t mid = (lo + hi) >> 1;
+            const uint32_t v = __shfl(incl, (int)mid, 64);
+            if (v > it) hi = mid;
+            else lo = mid + 1;
+        }
+        const int src = (int)(valid ? lo : 0u);
         uint32_t ct[3], dxu[3], dyu[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -254,39 +260,36 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
             dxu[k] = __shfl(e.dxu[k], src, 64);
             dyu[k] = __shfl(e.dyu[k], src, 64);
         }
-        const uint32_t pb = __shfl(pk_blk, src, 64), pc = __shfl(pk_cand, src, 64);
+        const uint32_t pc = __shfl(pk_cand, src, 64), pd = __shfl(pk_div, src, 64);
         const float zdx = __shfl(e.zdx, src, 64), zdy = __shfl(e.zdy, src, 64), zc0 = __shfl(e.zc0, src, 64);
-        const unsigned long long lo = (unsigned long long)__shfl(low, src, 64);
-        const uint32_t blk = it - __shfl(first, src, 64);
-        const uint32_t xs = pb & 127u, ys = (pb >> 7) & 127u, bnx = (pb >> 14) & 15u;
-        const bool tight = ((pb >> 18) & 1u) != 0;
-        const uint32_t by = (blk * (pb >> 19)) >> 10, bx = blk - by * bnx;
-        const uint32_t xb = xs + 8u * bx, yb = ys + 8u * by, x = xb + l;
-        const uint32_t cx0 = pc & 255u, cx1 = (pc >> 8) & 255u, cy0 = (pc >> 16) & 255u, cy1 = pc >> 24;
-        const bool col = valid && x >= cx0 && x < cx1;
-        const uint32_t r_lo = cy0 > yb ? cy0 - yb : 0u;
-        const uint32_t r_hi = cy1 > yb ? min(cy1 - yb, 8u) : 0u;
+        const unsigned long long lw = (unsigned long long)__shfl(low, src, 64);
+        const uint32_t k = it - __shfl(first, src, 64);
+        const uint32_t cx0 = pc & 127u, ccw = (pc >> 7) & 127u, cy0 = (pc >> 14) & 127u, cy1 = (pc >> 21) & 127u;
+        const bool tight = ((pc >> 28) & 1u) != 0;
+        const uint32_t brow = (k * (pd >> 7)) >> 16;
+        const uint32_t x = cx0 + (k - brow * ccw), yb = (pd & 127u) + 8u * brow;
+        const uint32_t xb = x & ~7u, l = x & 7u;
+        // the item lies in its owner's candidate rectangle (checked, not assumed)
+        const bool inside = valid && yb < cy1 && x - cx0 < ccw;
+        const uint32_t r_lo = inside && cy0 > yb ? cy0 - yb : 0u;
+        const uint32_t r_hi = inside ? min(cy1 - yb, 8u) : 0u;
         uint32_t cls = 1u;
-        if (valid && !tight) cls = block_class(ct, dxu, dyu, xb, yb);
+        if (inside && !tight) cls = block_class(ct, dxu, dyu, xb, yb);
         float z = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
+        for (uint32_t r = 0; r < r_lo; ++r) z = z + zdy;
         uint32_t E[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) E[k] = ct[k] + dyu[k] * x + dxu[k] * yb;
-        for (uint32_t r = 0; r < r_lo; ++r) {
-            z = z + zdy;
-#pragma unroll
-            for (int k = 0; k < 3; ++k) E[k] += dxu[k];
-        }
+        for (int kk = 0; kk < 3; ++kk) E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * (yb + r_lo);
         const uint32_t sxl = x - sx0;
         for (uint32_t r = r_lo; r < r_hi; ++r) {
-            bool cov = col && cls != 0u && (cls == 2u || (int32_t)(E[0] | E[1] | E[2]) >= 0);
+            bool cov = cls != 0u && (cls == 2u || (int32_t)(E[0] | E[1] | E[2]) >= 0);
             if (cov && tight && min(min(E[0], E[1]), E[2]) == 0u) cov = block_class(ct, dxu, dyu, xb, yb) != 0u;
             if (STATS) frags += cov ? 1u : 0u;
             if (cov && z > 0.0f)
-                atomicMax(&keys[key_slot<S>(sxl, yb + r - sy0)], ((unsigned long long)fbits(z) << 32) | lo);
+                atomicMax(&keys[key_slot<S>(sxl, yb + r - sy0)], ((unsigned long long)fbits(z) << 32) | lw);
             z = z + zdy;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) E[k] += dxu[k];
+            for (int kk = 0; kk < 3; ++kk) E[kk] += dxu[kk];
         }
     }
 }
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
                 order = r < F.ntris ? ((r << 3) | kFanUnclipped) : F.clip_order[r - F.ntris];
             }
             const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S);
-            raster_wave<S, STATS>(keys, e, has ? e.nbx * e.nby : 0u, sx0, sy0, frags);
+            raster_wave<S, STATS>(keys, e, has, sx0, sy0, frags);
         }
         __syncthreads();
 
