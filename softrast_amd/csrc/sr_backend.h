@@ -273,20 +273,31 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         const bool inside = valid && yb < cy1 && x - cx0 < ccw;
         const uint32_t r_lo = inside && cy0 > yb ? cy0 - yb : 0u;
         const uint32_t r_hi = inside ? min(cy1 - yb, 8u) : 0u;
-        uint32_t cls = 1u;
-        if (inside && !tight) cls = block_class(ct, dxu, dyu, xb, yb);
+        // per-pixel rule: covered = full || (test && all edge values >= 0)
+        bool full = false, test = true;
+        if (inside && !tight) {
+            const uint32_t cls = block_class(ct, dxu, dyu, xb, yb);
+            full = cls == 2u;
+            test = cls == 1u;
+        }
         float z = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
         for (uint32_t r = 0; r < r_lo; ++r) z = z + zdy;
         uint32_t E[3];
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * (yb + r_lo);
-        const uint32_t sxl = x - sx0;
+        // key_slot of (x - sx0, yb + r - sy0): rows of S keys, the in-group
+        // rotation alternating with the row's parity (yb is a multiple of 8)
+        const uint32_t sxl = x - sx0, ybl = yb - sy0;
+        const uint32_t t = sxl + 2u * (ybl >> 3) + (sxl >> 4);
+        const uint32_t a_even = ybl * S + (sxl & ~15u) + (t & 15u), a_odd = ybl * S + (sxl & ~15u) + ((t + 8u) & 15u);
         for (uint32_t r = r_lo; r < r_hi; ++r) {
-            bool cov = cls != 0u && (cls == 2u || (int32_t)(E[0] | E[1] | E[2]) >= 0);
-            if (cov && tight && min(min(E[0], E[1]), E[2]) == 0u) cov = block_class(ct, dxu, dyu, xb, yb) != 0u;
+            const bool pos = (int32_t)(E[0] | E[1] | E[2]) >= 0;
+            bool cov = full | (test & pos);
+            if (tight & pos & (min(min(E[0], E[1]), E[2]) == 0u))
+                cov = block_class(ct, dxu, dyu, xb, yb) != 0u;  // an edge through the pixel: the block test decides
             if (STATS) frags += cov ? 1u : 0u;
-            if (cov && z > 0.0f)
-                atomicMax(&keys[key_slot<S>(sxl, yb + r - sy0)], ((unsigned long long)fbits(z) << 32) | lw);
+            if (cov & (z > 0.0f))
+                atomicMax(&keys[((r & 1u) ? a_odd : a_even) + r * S], ((unsigned long long)fbits(z) << 32) | lw);
             z = z + zdy;
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk) E[kk] += dxu[kk];
