@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--sched-study", action="store_true",
+                    help="tile-order study on the clustered 3840x2160 / 4M-triangle workload instead of the default")
     ap.add_argument("--no-plain", action="store_true",
                     help="profiling aid: time only the batched path (--steps launches) and print its rate")
     return ap.parse_args()
@@ -128,6 +130,53 @@ def profile_batched(sr, scene, texs, args):
                       "warmup": args.warmup, "ms_per_step": round(ms, 4), "path": f"batched x{args.views} only"}))
 
 
+def sched_study(sr, args):
+    """--sched-study: one view of the clustered workload (3840x2160, 4,000,000
+    triangles, 90% in 3 clusters, 16 textures of 4096^2 built on the device),
+    rendered with each tile order; the learned order trains on the tile costs
+    of 64 warm-up frames first."""
+    from softrast_amd import scenes
+    t0 = time.time()
+    scene = scenes.clustered_scene()
+    print(f"[bench] clustered scene ready: {scene.num_tris} tris, {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    out = {"metric": "frames/s per tile order, clustered 3840x2160 / 4M tris (one view per EndFrame)",
+           "unit": "frames/s", "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "data": "synthetic",
+           "config": {"workload": "clustered_4k_4m", "width": scene.width, "height": scene.height,
+                      "triangles": scene.num_tris, "textures": "16x4096^2 RGBA8 + mips (device-built)"},
+           "orders": {}}
+    for order, name in ((sr.ORDER_NATURAL, "natural"), (sr.ORDER_TRICOUNT, "tricount"), (sr.ORDER_LEARNED, "learned")):
+        ctx = sr.RenderContext(flags=sr.CFG_TIMING, tile_order=order, bin_size=args.bin_size, sched_warmup_frames=64)
+        texs = [ctx.CreateTexture(t, t.shape[1], t.shape[0], True) for t in scene.textures]
+        fb = sr.FrameBuffer(ctx, scene.width, scene.height)
+        calls = make_calls(sr, fb, scene, texs, scene.mvp)
+
+        def frame():
+            ctx.BeginFrame()
+            ctx.ClearFrameBuffer(fb, 0)
+            for c in calls:
+                c.m_frameBuffer = (fb, fb.m_writePlane)
+                ctx.DrawIndexed(c)
+            ctx.EndFrame()
+        for _ in range(max(args.warmup, 70)):
+            frame()
+        st = []
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            frame()
+            st.append(ctx.frame_stats())
+        el = time.perf_counter() - t
+        out["orders"][name] = {"frames_per_s": round(args.steps / el, 2),
+                               "device_ms": round(statistics.mean(s["ms_total"] for s in st), 4),
+                               "backend_ms": round(statistics.mean(s["ms_backend"] for s in st), 4)}
+        if order == sr.ORDER_LEARNED:
+            out["orders"][name]["sched_err_log2"] = round(statistics.mean(s["sched_err"] for s in st), 4)
+            out["orders"][name]["trained"] = bool(st[-1]["sched_trained"])
+        print(f"[bench] {name}: {out['orders'][name]}", file=sys.stderr, flush=True)
+        fb.destroy()
+        ctx.Shutdown()
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -137,6 +186,8 @@ def main():
         sys.exit(2)
     import softrast_amd as sr
     from softrast_amd import scenes
+    if args.sched_study:
+        return sched_study(sr, args)
 
     t0 = time.time()
     scene = scenes.spheres_scene()

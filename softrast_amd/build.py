@@ -53,7 +53,7 @@ def build_gpu(extra=()):
 def build_scene():
     out = PKG / "libsrscene.so"
     tmp = out.with_suffix(".so.tmp")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", str(tmp), str(CSRC / "scene.cpp")])
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-o", str(tmp), str(CSRC / "scene.cpp")])
     os.replace(tmp, out)
     return out
 

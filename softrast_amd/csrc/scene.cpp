@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <vector>
 #include <algorithm>
 
@@ -121,22 +122,30 @@ void srs_value_noise_rgba8(uint64_t seed, uint32_t width, uint32_t height, uint8
             std::vector<float> lat((size_t)gw * gh);
             Pcg32 rng(mix2(seed, (uint64_t)(c * 16 + o)), (uint64_t)(c * 16 + o));
             for (auto& v : lat) v = (float)rng.uniform();
-            for (uint32_t y = 0; y < height; ++y) {
-                double fy = (double)y / period;
-                uint32_t y0 = (uint32_t)fy % gh, y1 = (y0 + 1) % gh;
-                double ty = fy - std::floor(fy);
-                ty = ty * ty * (3.0 - 2.0 * ty);
-                for (uint32_t x = 0; x < width; ++x) {
-                    double fx = (double)x / period;
-                    uint32_t x0 = (uint32_t)fx % gw, x1 = (x0 + 1) % gw;
-                    double tx = fx - std::floor(fx);
-                    tx = tx * tx * (3.0 - 2.0 * tx);
-                    double a = lat[(size_t)y0 * gw + x0], b = lat[(size_t)y0 * gw + x1];
-                    double cc = lat[(size_t)y1 * gw + x0], d = lat[(size_t)y1 * gw + x1];
-                    double v = (a + (b - a) * tx) + ((cc + (d - cc) * tx) - (a + (b - a) * tx)) * ty;
-                    acc[(size_t)y * width + x] += (float)(amp * v);
+            auto rows = [&](uint32_t ya, uint32_t yz) {
+                for (uint32_t y = ya; y < yz; ++y) {
+                    double fy = (double)y / period;
+                    uint32_t y0 = (uint32_t)fy % gh, y1 = (y0 + 1) % gh;
+                    double ty = fy - std::floor(fy);
+                    ty = ty * ty * (3.0 - 2.0 * ty);
+                    for (uint32_t x = 0; x < width; ++x) {
+                        double fx = (double)x / period;
+                        uint32_t x0 = (uint32_t)fx % gw, x1 = (x0 + 1) % gw;
+                        double tx = fx - std::floor(fx);
+                        tx = tx * tx * (3.0 - 2.0 * tx);
+                        double a = lat[(size_t)y0 * gw + x0], b = lat[(size_t)y0 * gw + x1];
+                        double cc = lat[(size_t)y1 * gw + x0], d = lat[(size_t)y1 * gw + x1];
+                        double v = (a + (b - a) * tx) + ((cc + (d - cc) * tx) - (a + (b - a) * tx)) * ty;
+                        acc[(size_t)y * width + x] += (float)(amp * v);
+                    }
                 }
-            }
+            };
+            // rows are independent: split them over up to 16 threads (same result)
+            const uint32_t nt = std::max(1u, std::min({ 16u, std::thread::hardware_concurrency(), height / 64u }));
+            std::vector<std::thread> pool;
+            for (uint32_t t = 1; t < nt; ++t) pool.emplace_back(rows, height * t / nt, height * (t + 1) / nt);
+            rows(0, height / nt);
+            for (auto& th : pool) th.join();
             total += amp;
             amp *= 0.5;
         }

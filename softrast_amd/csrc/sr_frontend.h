@@ -472,28 +472,52 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
             const uint32_t np = clip_polygon(poly, mor, pin);
             nfan = np >= 3 ? np - 2 : 0;
         }
+        __syncthreads();  // polygons are read by other lanes below
         const uint32_t base = wave_add_amount(&F.counters[C_CLIPREC], nfan);
         const bool fits = base + nfan <= F.clip_cap;
         if (nfan && !fits) atomicOr(&F.counters[C_FLAGS], FLAG_CLIP_OVERFLOW);
         if (nfan && fits) F.clip_base[g] = base;
-        const DrawDev& D = F.draws[d];
-        const ViewDev& V = F.views[D.view];
-        for (uint32_t j = 0; __ballot(j < nfan) != 0ull; ++j) {
+        // the wave's fan triangles, one per lane: lane j takes fan item t of
+        // the lane whose inclusive fan count first exceeds t
+        const uint32_t lane = lane_id();
+        uint32_t incl = fits ? nfan : 0u;
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        const uint32_t total = readlane_u32(incl, 63);
+        const uint32_t first = incl - (fits ? nfan : 0u);
+        for (uint32_t t0 = 0; t0 < total; t0 += 64) {
+            const uint32_t t = t0 + lane;
+            uint32_t lo = 0, hi = 63;
+#pragma unroll
+            for (int step = 0; step < 6; ++step) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (__shfl(incl, (int)mid, 64) > t) hi = mid;
+                else lo = mid + 1;
+            }
+            const int src = (int)(t < total ? lo : 0u);
+            const uint32_t og = __shfl(g, src, 64), od = __shfl(d, src, 64), opin = __shfl(pin, src, 64);
+            const uint32_t obase = __shfl(base, src, 64), j = t - __shfl(first, src, 64);
+            const DrawDev& D = F.draws[od];
+            const ViewDev& V = F.views[D.view];
             Prim s;
             bool ok = false;
-            if (j < nfan && fits) {
-                const float4 fv[3] = { poly.pos(pin, 0), poly.pos(pin, j + 1), poly.pos(pin, j + 2) };
+            if (t < total) {
+                const ClipPoly owner{ poly.lds, (uint32_t)src };
+                const float4 fv[3] = { owner.pos(opin, 0), owner.pos(opin, j + 1), owner.pos(opin, j + 2) };
                 float fa[3][kMaxSlots];
-                for (uint32_t t = 0; t < kMaxSlots; ++t) {
-                    fa[0][t] = poly.at(pin, 0, 4 + t);
-                    fa[1][t] = poly.at(pin, j + 1, 4 + t);
-                    fa[2][t] = poly.at(pin, j + 2, 4 + t);
+                for (uint32_t a = 0; a < kMaxSlots; ++a) {
+                    fa[0][a] = owner.at(opin, 0, 4 + a);
+                    fa[1][a] = owner.at(opin, j + 1, 4 + a);
+                    fa[2][a] = owner.at(opin, j + 2, 4 + a);
                 }
-                const uint32_t r = F.ntris + base + j;
+                const uint32_t r = F.ntris + obase + j;
                 ok = setup_prim(s, fv, fa, D.nslots, V.width, V.height);
                 if (ok) {
-                    write_record(F, r, s, d);
-                    F.clip_order[base + j] = (g << 3) | j;
+                    write_record(F, r, s, od);
+                    F.clip_order[obase + j] = (og << 3) | j;
                 } else {
                     F.rec_meta[r] = 0xFFFFFFFFu;
                 }
@@ -501,11 +525,12 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
             if (F.flags & FF_STATS) {
                 const uint32_t te = wave_sum(ok ? reference_tile_entries(s) : 0u);
                 const uint32_t pr = wave_sum(ok ? 1u : 0u);
-                if (lane_id() == 0 && te) atomicAdd(&F.counters[C_TILE_ENTRIES], te);
-                if (lane_id() == 0 && pr) atomicAdd(&F.counters[C_PRIMS], pr);
+                if (lane == 0 && te) atomicAdd(&F.counters[C_TILE_ENTRIES], te);
+                if (lane == 0 && pr) atomicAdd(&F.counters[C_PRIMS], pr);
             }
             walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0);
         }
+        __syncthreads();  // before the polygons are overwritten
     }
 }
 

@@ -37,6 +37,9 @@ struct SchedSample {
 
 __global__ __launch_bounds__(256) void k_bin_features(FrameDev F, float4* feat)
 {
+    // An overflowed frame has bin starts past the entry arena and no entry
+    // lists (k_emit skipped it); it is rendered again with larger arenas.
+    if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gb = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (gb >= F.total_bins) return;  // whole wave exits together

@@ -288,6 +288,35 @@ def test_clustered_scene_every_order(tile_order):
     ctx.Shutdown()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile_order", [0, 1, 2])
+def test_arena_overflow_retry(tile_order):
+    """A frame whose clip-fan or entry arena overflows is rendered again with
+    larger arenas, in every queue order. The overflowed attempt has bin starts
+    past the entry arena and no entry lists, so every kernel that reads them
+    (emit, back end, the learned order's features) must skip it. One-element
+    arenas overflow the first frame; a larger second scene overflows the grown
+    arenas again after the learned model has trained."""
+    import softrast_amd as sr
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=tile_order, max_prims=1,
+                           max_entries=1, sched_warmup_frames=1, sched_retrain_interval=1)
+    for name, sc, frames in (("small", helpers.random_scene(71, 3000, 320, 240, clip_heavy=True), 3),
+                             ("large", helpers.random_scene(72, 40000, 800, 600, clip_heavy=True), 2)):
+        tex = helpers.build_textures(sc)
+        ref, rstats = helpers.oracle_render(sc, tex, nthreads=8)
+        for frame in range(frames):
+            _say(f"[parity] case=overflow_retry order={tile_order} scene={name} frame={frame}")
+            col, dep, pid, st = helpers.gpu_render(sc, tex, ctx=ctx)
+            d = helpers.compare(ref, col, dep, pid)
+            assert not any(d.values()), (name, frame, d)
+            # re-runs of this frame: the first overflows, the grown arenas are kept
+            assert (st["retries"] > 0) == (frame == 0), (name, frame, st)
+        assert st["tris_clipped"] > 0 and st["frags_raster"] == rstats["frags_raster"]
+    if tile_order == sr.ORDER_LEARNED:
+        assert st["sched_trained"]
+    ctx.Shutdown()
+
+
 def _sliver_scene(seed, n, width, height):
     """Needle-thin triangles (0.0005-0.008 px wide, 2-40 px long) in clip
     space, both windings: mostly primitives without a tight pixel bound,
