@@ -58,8 +58,12 @@ enum srg_config_flags {
     SRG_CFG_STATS = 1u << 1,       /* count rasterised fragments and binned entries */
     SRG_CFG_TIMING = 1u << 2,      /* time each kernel with HIP events */
     SRG_CFG_TILE_COSTS = 1u << 3,  /* record per-tile s_memtime cost in the back end */
-    SRG_CFG_GRAPH = 1u << 4        /* capture the frame's launches in a hipGraph and replay it while
+    SRG_CFG_GRAPH = 1u << 4,       /* capture the frame's launches in a hipGraph and replay it while
                                       the draw list, views and arenas are unchanged */
+    SRG_CFG_AUTOTUNE = 1u << 5     /* pick bin_size x wg_per_tile per frame signature (view sizes and
+                                      draw triangle counts): each layout of srg_tuning is timed over
+                                      autotune_frames frames, then the fastest is kept. Every layout
+                                      gives the same image. */
 };
 
 typedef struct srg_config {
@@ -70,12 +74,16 @@ typedef struct srg_config {
     uint32_t max_prims;     /* initial primitive-record capacity, 0 = sized per frame */
     uint32_t max_entries;   /* initial tile-entry capacity, 0 = sized per frame */
     uint32_t wg_per_cu;     /* persistent back-end workgroups per CU, 0 = default */
-    uint32_t bin_size;      /* back-end work bin edge: 64, 32 or 16 pixels, 0 = 32. The
+    uint32_t bin_size;      /* back-end work bin edge: 64, 32, 16 or 8 pixels, 0 = 32. The
                                reference's 64x64 bin semantics are kept for every size. */
     uint32_t sched_warmup_frames;    /* SRG_ORDER_LEARNED: frames recorded before the first
                                         training (0 = 8); entry-count order meanwhile */
     uint32_t sched_retrain_interval; /* SRG_ORDER_LEARNED: retrain every N frames (0 = never) */
-    uint32_t reserved[6];
+    uint32_t wg_per_tile;   /* workgroups sharing one work bin, each owning a strip of
+                               bin_size / wg_per_tile rows: 1, 2 or 4 (0 = 1). Valid layouts
+                               (bin_size, wg_per_tile): (8,1) (16,1) (32,1|2|4) (64,1|2|4) */
+    uint32_t autotune_frames;        /* SRG_CFG_AUTOTUNE: frames timed per layout (0 = 3) */
+    uint32_t reserved[4];
 } srg_config;
 
 /* Mirrors DrawCall (Renderer.h:119-150). Buffers are registered handles plus
@@ -112,7 +120,22 @@ typedef struct srg_frame_stats {
     float sched_err;          /* SRG_ORDER_LEARNED: mean |log2 predicted - log2 measured| bin cost */
     uint32_t sched_trained;   /* SRG_ORDER_LEARNED: 1 once the model orders the queue */
     uint32_t sched_trainings; /* SRG_ORDER_LEARNED: training runs done by this frame */
+    uint32_t bin_size;        /* back-end layout of this frame: work-bin edge ... */
+    uint32_t wg_per_tile;     /* ... and workgroups per bin */
 } srg_frame_stats;
+
+/* SRG_CFG_AUTOTUNE state for the signature of the last frame. */
+#define SRG_TUNE_LAYOUTS 8
+typedef struct srg_tuning {
+    uint32_t state;           /* 0 = autotuning off or no frame yet, 1 = sweeping, 2 = layout chosen */
+    uint32_t bin_size;        /* layout chosen (state 2) */
+    uint32_t wg_per_tile;
+    uint32_t nlayouts;        /* SRG_TUNE_LAYOUTS */
+    struct {
+        uint32_t bin_size, wg_per_tile;
+        float ms;             /* best whole-frame device time measured, 0 = not yet */
+    } layouts[SRG_TUNE_LAYOUTS];
+} srg_tuning;
 
 /* ---- context: RenderContext ctor / Shutdown (Renderer.cpp:138-159) ---- */
 int srg_create(const srg_config* cfg, srg_ctx** out);
@@ -181,6 +204,10 @@ int srg_get_bin_stats(srg_ctx* ctx, uint32_t* counts, uint64_t* costs, uint32_t 
 #define SRG_SCHED_PARAMS 1249u
 int srg_sched_get_model(srg_ctx* ctx, float* params, uint32_t count);
 int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count);
+/* The autotuner's table for the last frame's signature (SRG_CFG_AUTOTUNE).
+ * No counterpart in the reference, whose bin size is a compile-time constant
+ * (Config.h:18-21) and whose tiles each take one task (Renderer.cpp:264-306). */
+int srg_get_tuning(srg_ctx* ctx, srg_tuning* out);
 
 /* ---- host helper: TextureData::CreateFromRGBA8 (Texture.cpp:119-199) ----
  * Tiles a power-of-two RGBA8 image (>= 32 in both dimensions) into 32x32

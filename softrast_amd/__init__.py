@@ -32,7 +32,9 @@ SHADER_UNLIT_DIFFUSE = 0
 SHADER_VISUALIZE_NORMALS = 1
 SHADER_VISUALIZE_UVS = 2
 ORDER_NATURAL, ORDER_TRICOUNT, ORDER_LEARNED = 0, 1, 2
-CFG_KEEP_PRIMID, CFG_STATS, CFG_TIMING, CFG_TILE_COSTS, CFG_GRAPH = 1, 2, 4, 8, 16
+CFG_KEEP_PRIMID, CFG_STATS, CFG_TIMING, CFG_TILE_COSTS, CFG_GRAPH, CFG_AUTOTUNE = 1, 2, 4, 8, 16, 32
+# back-end layouts (work-bin edge, workgroups per bin) the library accepts and the autotuner sweeps
+LAYOUTS = ((8, 1), (16, 1), (32, 1), (32, 2), (32, 4), (64, 1), (64, 2), (64, 4))
 FB_COLOUR, FB_DEPTH = 1, 2
 
 SRG_SYMBOLS = (
@@ -42,6 +44,7 @@ SRG_SYMBOLS = (
     "srg_fb_write_tiles", "srg_begin_frame", "srg_clear", "srg_draw_indexed", "srg_end_frame", "srg_blit",
     "srg_get_frame_stats", "srg_get_bin_stats", "srg_sched_get_model", "srg_sched_set_model",
     "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
+    "srg_get_tuning",
 )
 
 
@@ -56,7 +59,8 @@ class SrgConfig(ctypes.Structure):
                 ("tile_order", ctypes.c_uint32), ("max_prims", ctypes.c_uint32), ("max_entries", ctypes.c_uint32),
                 ("wg_per_cu", ctypes.c_uint32), ("bin_size", ctypes.c_uint32),
                 ("sched_warmup_frames", ctypes.c_uint32), ("sched_retrain_interval", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 6)]
+                ("wg_per_tile", ctypes.c_uint32), ("autotune_frames", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
 
 
 class SrgDraw(ctypes.Structure):
@@ -75,7 +79,20 @@ class SrgFrameStats(ctypes.Structure):
     _fields_ = [("tris_in", ctypes.c_uint64), ("tris_clipped", ctypes.c_uint64), ("prims", ctypes.c_uint64),
                 ("bin_entries", ctypes.c_uint64), ("work_entries", ctypes.c_uint64), ("frags_raster", ctypes.c_uint64), ("tiles_queued", ctypes.c_uint32), ("retries", ctypes.c_uint32), ("graph_captures", ctypes.c_uint32),
                 ("ms_frontend", ctypes.c_float), ("ms_backend", ctypes.c_float), ("ms_total", ctypes.c_float),
-                ("sched_err", ctypes.c_float), ("sched_trained", ctypes.c_uint32), ("sched_trainings", ctypes.c_uint32)]
+                ("sched_err", ctypes.c_float), ("sched_trained", ctypes.c_uint32), ("sched_trainings", ctypes.c_uint32),
+                ("bin_size", ctypes.c_uint32), ("wg_per_tile", ctypes.c_uint32)]
+
+
+TUNE_LAYOUTS = 8
+
+
+class SrgTuneLayout(ctypes.Structure):
+    _fields_ = [("bin_size", ctypes.c_uint32), ("wg_per_tile", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
+class SrgTuning(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint32), ("bin_size", ctypes.c_uint32), ("wg_per_tile", ctypes.c_uint32),
+                ("nlayouts", ctypes.c_uint32), ("layouts", SrgTuneLayout * TUNE_LAYOUTS)]
 
 
 _lib = None
@@ -122,6 +139,7 @@ def load_library(path=None):
         "srg_get_bin_stats": (i32, [vp, vp, vp, u32, ctypes.POINTER(u32)]),
         "srg_sched_get_model": (i32, [vp, vp, u32]),
         "srg_sched_set_model": (i32, [vp, vp, u32]),
+        "srg_get_tuning": (i32, [vp, ctypes.POINTER(SrgTuning)]),
         "srg_texture_build_rgba8": (i32, [vp, u32, u32, i32, vp, pu64, ctypes.POINTER(u32),
                                           ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     }
@@ -345,7 +363,8 @@ class RenderContext:
     """
 
     def __init__(self, device=0, flags=0, tile_order=ORDER_NATURAL, max_prims=0, max_entries=0, wg_per_cu=0,
-                 bin_size=0, sched_warmup_frames=0, sched_retrain_interval=0, lib_path=None):
+                 bin_size=0, sched_warmup_frames=0, sched_retrain_interval=0, wg_per_tile=0, autotune_frames=0,
+                 lib_path=None):
         self._lib = load_library(lib_path)
         cfg = SrgConfig()
         cfg.struct_size = ctypes.sizeof(SrgConfig)
@@ -353,6 +372,7 @@ class RenderContext:
         cfg.max_prims, cfg.max_entries, cfg.wg_per_cu = max_prims, max_entries, wg_per_cu
         cfg.bin_size = bin_size
         cfg.sched_warmup_frames, cfg.sched_retrain_interval = sched_warmup_frames, sched_retrain_interval
+        cfg.wg_per_tile, cfg.autotune_frames = wg_per_tile, autotune_frames
         h = ctypes.c_void_p()
         _check(self._lib.srg_create(ctypes.byref(cfg), ctypes.byref(h)), "RenderContext")
         self._ctx = h
@@ -511,6 +531,16 @@ class RenderContext:
     def set_sched_model(self, params):
         p = np.ascontiguousarray(params, np.float32)
         _check(self._lib.srg_sched_set_model(self._ctx, _ptr(p), p.size), "set_sched_model")
+
+    def tuning(self):
+        """Autotuner state for the last frame's signature: {"state": "off" |
+        "sweeping" | "chosen", "layout": (bin_size, wg_per_tile) or None,
+        "ms": {(bin_size, wg_per_tile): best device ms, 0 = not measured}}."""
+        t = SrgTuning()
+        _check(self._lib.srg_get_tuning(self._ctx, ctypes.byref(t)), "tuning")
+        return {"state": ("off", "sweeping", "chosen")[t.state],
+                "layout": (t.bin_size, t.wg_per_tile) if t.state == 2 else None,
+                "ms": {(l.bin_size, l.wg_per_tile): l.ms for l in t.layouts[:t.nlayouts]}}
 
     def Shutdown(self):
         if self._ctx:

@@ -1,8 +1,10 @@
 // sr_backend.h -- back end: per-bin coverage, early depth and resolve,
 // then interpolation and shading of the visible fragment of each pixel.
 //
-// k_backend<S> is persistent: each workgroup (4 waves) takes S x S bins from
-// the frame's queue. A 64-bit visibility key per pixel lives in LDS. Waves
+// k_backend<S, H> is persistent: each workgroup takes an S x H strip of an
+// S x S bin from the frame's queue (H = S: the whole bin; H = S/2, S/4: two
+// or four workgroups per bin, sharing its entry list). A 64-bit visibility
+// key per pixel lives in LDS. Waves
 // stride over the bin's entries 64 at a time, each lane loading one
 // primitive record so the loads overlap; the entries' 8x8 blocks then form a
 // queue that groups of 8 lanes take one block at a time, a lane per pixel
@@ -16,8 +18,8 @@
 
 namespace srg {
 
-constexpr uint32_t kBackendThreads = 256;
-constexpr uint32_t kBackendWaves = kBackendThreads / 64;
+// Workgroup size for an S x H strip: 4 waves, or one wave for an 8 x 8 bin.
+__host__ __device__ constexpr uint32_t backend_threads(uint32_t S, uint32_t H) { return S * H >= 256u ? 256u : 64u; }
 
 // The reference keeps a fragment iff z > 0 and z > D (ordered compare) and
 // shades every kept fragment in submission order, so the final pixel is the
@@ -130,9 +132,10 @@ __device__ __forceinline__ uint32_t block_class(const uint32_t ct[3], const uint
     return reject ? 0u : (full ? 2u : 1u);
 }
 
-// LDS slot of the key of bin-local pixel (x, y): rows of S keys, and inside
-// each group of 16 keys of a row the slot is rotated by 2*(y>>3) + (x>>4) +
-// 8*(y&1). Keys are 8 bytes, so 16 consecutive keys cover the LDS banks once.
+// LDS slot of the key of strip-local pixel (x, y): rows of S keys, and inside
+// each group of G = min(S, 16) keys of a row the slot is rotated by
+// 2*(y>>3) + (x>>4) + 8*(y&1) (mod G). Keys are 8 bytes, so 16 consecutive
+// keys cover the LDS banks once.
 // Without the rotation the 64 lanes of a wave hit the same few banks: one
 // block per wave puts its 8 rows on the same banks, and one block per lane
 // puts the same pixel of every lane's block on the same bank. With it, a
@@ -141,12 +144,15 @@ __device__ __forceinline__ uint32_t block_class(const uint32_t ct[3], const uint
 template <uint32_t S>
 __device__ __forceinline__ uint32_t key_slot(uint32_t x, uint32_t y)
 {
+    constexpr uint32_t G = S < 16u ? S : 16u;
     const uint32_t rot = 2u * (y >> 3) + (x >> 4) + 8u * (y & 1u);
-    return y * S + (x & ~15u) + ((x + rot) & 15u);
+    return y * S + (x & ~(G - 1u)) + ((x + rot) & (G - 1u));
 }
 
 // One primitive's tile-relative setup (Binning.cpp:423-451 re-basing) and its
-// block range inside the work bin at (sx0, sy0).
+// block range inside the S x H strip at (sx0, sy0). Strips start on 8-pixel
+// rows, so the blocks walked over a bin's strips are the blocks walked over
+// the bin.
 struct EntryRaster {
     uint32_t ct[3], dxu[3], dyu[3];
     uint32_t xs, xe, ys, ye;      // the reference's walk: blocks from (xs, ys) while xb < xe, yb < ye
@@ -159,7 +165,7 @@ struct EntryRaster {
 };
 
 __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2, uint4 q3, uint32_t order, int32_t X0,
-                                                    int32_t Y0, uint32_t sx0, uint32_t sy0, uint32_t S)
+                                                    int32_t Y0, uint32_t sx0, uint32_t sy0, uint32_t S, uint32_t H)
 {
     EntryRaster e;
     e.dxu[0] = q0.w; e.dxu[1] = q1.x; e.dxu[2] = q1.y;
@@ -173,7 +179,7 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
     e.xs = max(bminx & ~7u, sx0);
     e.xe = min(bmaxx, sx0 + S);
     e.ys = max(bminy & ~7u, sy0);
-    e.ye = min(bmaxy, sy0 + S);
+    e.ye = min(bmaxy, sy0 + H);
     // walked pixels: whole blocks from xs while xb < xe (xs, ys are multiples of 8)
     e.tight = ((q2.y >> 14) & 1u) != 0;
     const int32_t wx1 = e.xe > e.xs ? (int32_t)(e.xs + ((e.xe - e.xs + 7u) & ~7u)) : (int32_t)e.xs;
@@ -287,9 +293,11 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         for (int kk = 0; kk < 3; ++kk) E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * (yb + r_lo);
         // key_slot of (x - sx0, yb + r - sy0): rows of S keys, the in-group
         // rotation alternating with the row's parity (yb is a multiple of 8)
+        constexpr uint32_t G = S < 16u ? S : 16u;
         const uint32_t sxl = x - sx0, ybl = yb - sy0;
         const uint32_t t = sxl + 2u * (ybl >> 3) + (sxl >> 4);
-        const uint32_t a_even = ybl * S + (sxl & ~15u) + (t & 15u), a_odd = ybl * S + (sxl & ~15u) + ((t + 8u) & 15u);
+        const uint32_t a_even = ybl * S + (sxl & ~(G - 1u)) + (t & (G - 1u));
+        const uint32_t a_odd = ybl * S + (sxl & ~(G - 1u)) + ((t + 8u) & (G - 1u));
         for (uint32_t r = r_lo; r < r_hi; ++r) {
             const bool pos = (int32_t)(E[0] | E[1] | E[2]) >= 0;
             bool cov = full | (test & pos);
@@ -305,21 +313,25 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     }
 }
 
-template <uint32_t S, bool STATS>
-__global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
+template <uint32_t S, uint32_t H, bool STATS>
+__global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
 {
+    constexpr uint32_t T = backend_threads(S, H);
+    constexpr uint32_t W = T / 64u;
     constexpr uint32_t K = kBin / S;
-    constexpr uint32_t NPIX = S * S;
-    constexpr uint32_t PPT = NPIX / kBackendThreads;
+    constexpr uint32_t PARTS = S / H;  // workgroups per bin
+    constexpr uint32_t NPIX = S * H;
+    constexpr uint32_t PPT = NPIX / T;
     constexpr uint32_t BATCH = PPT < 4 ? PPT : 4;
-    static_assert(NPIX % kBackendThreads == 0, "bin must hold a whole number of pixels per thread");
+    static_assert(H % 8u == 0 && S % H == 0, "strips are whole 8-pixel block rows of the bin");
+    static_assert(NPIX % T == 0 && PPT % BATCH == 0, "strip must hold a whole number of pixels per thread");
     __shared__ unsigned long long keys[NPIX];
     __shared__ uint32_t s_q;
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
     unsigned long long frags = 0;
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
-    const uint32_t qlen = F.counters[C_QLEN];
+    const uint32_t qlen = F.counters[C_QLEN] * PARTS;  // strips of the queued bins, a bin's strips adjacent
 
     for (;;) {
         if (tid == 0) s_q = atomicAdd(&F.counters[C_QHEAD], 1u);
@@ -328,27 +340,27 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
         __syncthreads();
         if (qi >= qlen) break;
         const uint64_t t_start = (F.flags & FF_TILE_COSTS) ? clock64() : 0;
-        const uint32_t gb = F.queue[qi];
+        const uint32_t gb = F.queue[qi / PARTS];
         const ViewDev V = F.views[view_of_bin(F, gb)];
         const uint32_t lb = gb - V.bin_base;
         const uint32_t sbx = lb % V.bins_x, sby = lb / V.bins_x;
         const uint32_t tx = sbx / K, ty = sby / K;
-        const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S;
+        const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S + (qi % PARTS) * H;
         const int32_t X0 = (int32_t)(tx * kBin), Y0 = (int32_t)(ty * kBin);
         const size_t tofs = (size_t)(ty * V.tiles_x + tx) * kTilePixels;
         const bool clear_d = (V.flags & VF_CLEAR_DEPTH) != 0;
         const bool clear_c = (V.flags & VF_CLEAR_COLOUR) != 0;
 
-        for (uint32_t p = tid; p < NPIX; p += kBackendThreads) {
+        for (uint32_t p = tid; p < NPIX; p += T) {
             const uint32_t x = sx0 + p % S, y = sy0 + p / S;
             keys[key_slot<S>(p % S, p / S)] = clear_d ? 0ull : init_key(V.depth[tofs + y * kBin + x]);
         }
         __syncthreads();
 
         const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
-        for (uint32_t b0 = 0; b0 + wave < cnt; b0 += kBackendThreads) {
-            // lane j of wave w holds entry b0 + w + 4j
-            const uint32_t my = b0 + wave + kBackendWaves * lane;
+        for (uint32_t b0 = 0; b0 + wave < cnt; b0 += T) {
+            // lane j of wave w holds entry b0 + w + W*j
+            const uint32_t my = b0 + wave + W * lane;
             const bool has = my < cnt;
             uint4 q0 = make_uint4(0, 0, 0, 0), q1 = q0, q2 = q0, q3 = q0;
             uint32_t order = 0;
@@ -358,7 +370,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
                 q0 = R[0]; q1 = R[1]; q2 = R[2]; q3 = R[3];
                 order = r < F.ntris ? ((r << 3) | kFanUnclipped) : F.clip_order[r - F.ntris];
             }
-            const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S);
+            const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S, H);
             raster_wave<S, STATS>(keys, e, has, sx0, sy0, frags);
         }
         __syncthreads();
@@ -369,7 +381,7 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
             bool win[BATCH];
 #pragma unroll
             for (uint32_t i = 0; i < BATCH; ++i) {
-                const uint32_t p = tid + (p0 + i) * kBackendThreads;
+                const uint32_t p = tid + (p0 + i) * T;
                 kk[i] = keys[key_slot<S>(p % S, p / S)];
                 const uint32_t lo = (uint32_t)kk[i];
                 win[i] = lo != 0xFFFFFFFFu && kk[i] != 0ull;
@@ -377,13 +389,13 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
             }
 #pragma unroll
             for (uint32_t i = 0; i < BATCH; ++i) {
-                const uint32_t p = tid + (p0 + i) * kBackendThreads;
+                const uint32_t p = tid + (p0 + i) * T;
                 col[i] = (win[i] && (V.flags & VF_HAS_COLOUR)) ? shade_pixel(F, rec[i], X0, Y0, sx0 + p % S, sy0 + p / S)
                                                                : V.clear_colour;
             }
 #pragma unroll
             for (uint32_t i = 0; i < BATCH; ++i) {
-                const uint32_t p = tid + (p0 + i) * kBackendThreads;
+                const uint32_t p = tid + (p0 + i) * T;
                 const size_t o = tofs + (sy0 + p / S) * kBin + sx0 + p % S;
                 if (win[i]) {
                     if (V.flags & VF_HAS_COLOUR) V.colour[o] = col[i];
@@ -395,7 +407,9 @@ __global__ __launch_bounds__(kBackendThreads) void k_backend(FrameDev F)
                 }
             }
         }
-        if ((F.flags & FF_TILE_COSTS) && tid == 0) F.bin_cost[gb] = clock64() - t_start;
+        // a bin's cost is the sum over its strips (zeroed at frame start)
+        if ((F.flags & FF_TILE_COSTS) && tid == 0)
+            atomicAdd((unsigned long long*)&F.bin_cost[gb], (unsigned long long)(clock64() - t_start));
         __syncthreads();
     }
     if (STATS) {

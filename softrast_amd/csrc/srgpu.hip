@@ -32,14 +32,16 @@
 
 namespace srg {
 
-// Frame-start fill: counters and per-bin counts / cursors (graph-safe).
-__global__ void k_frame_init(uint32_t* counters, uint32_t* bin_count, uint32_t* bin_cursor, uint32_t nbins)
+// Frame-start fill: counters and per-bin counts / cursors / costs (graph-safe).
+__global__ void k_frame_init(uint32_t* counters, uint32_t* bin_count, uint32_t* bin_cursor, uint64_t* bin_cost,
+                             uint32_t nbins)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < C_NCOUNTERS) counters[i] = 0;
     for (uint32_t t = i; t < nbins; t += gridDim.x * blockDim.x) {
         bin_count[t] = 0;
         bin_cursor[t] = 0;
+        bin_cost[t] = 0;
     }
 }
 
@@ -129,6 +131,22 @@ struct BlitJob {
     void* user = nullptr;
 };
 
+// Back-end work layout: bin edge in pixels and workgroups per bin.
+struct BackendLayout {
+    uint32_t bin_size, parts;
+};
+
+// Autotuner: every layout of the sweep is timed over a few frames of the same
+// frame signature (views and triangle counts), then the fastest is kept until
+// the signature changes. Layouts change only the schedule, never a pixel.
+struct TuneState {
+    bool sweeping = false, locked = false;
+    uint64_t sig = 0;
+    uint32_t cand = 0, frame = 0;
+    float best_ms[8] = {};
+    BackendLayout chosen{ 32, 1 };
+};
+
 struct PlaneState {
     uint32_t* colour = nullptr;
     float* depth = nullptr;
@@ -191,6 +209,9 @@ struct srg_ctx {
     DevArray<float> sched_pred;
     bool sched_trained = false;
     uint32_t sched_frames = 0, sched_last_train = 0;
+    uint32_t sched_bin_size = 0;  // bin size the recorded samples and the model belong to
+    std::unordered_map<uint64_t, TuneState> tunes;  // autotuner, per frame signature
+    uint64_t tune_sig = 0;                          // signature of the last frame
     std::vector<uint8_t> graph_sig;
 
     hipEvent_t ev[4] = {};
@@ -315,6 +336,20 @@ int ensure_sched(srg_ctx* ctx)
     return SRG_OK;
 }
 
+// A new bin size: the recorded samples and the model describe other bins, so
+// the model starts again from its initial weights and an empty ring.
+int reset_sched(srg_ctx* ctx)
+{
+    hipLaunchKernelGGL(k_mlp_init, dim3((kMlpParams + 255) / 256), dim3(256), 0, ctx->stream,
+                       (MlpParams*)ctx->sched_params, (AdamState*)ctx->sched_adam, 0x5EED5C4Du);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->sched_head, 0u, (size_t)1);
+    HIP_TRY(hipGetLastError());
+    ctx->sched_trained = false;
+    ctx->sched_frames = 0;
+    ctx->sched_last_train = 0;
+    return SRG_OK;
+}
+
 // After a frame of SRG_ORDER_LEARNED: report the model's error and train it
 // once enough frames are recorded (and again every retrain interval).
 int sched_after_frame(srg_ctx* ctx)
@@ -349,35 +384,91 @@ int sched_after_frame(srg_ctx* ctx)
     return SRG_OK;
 }
 
-uint32_t bin_k_of(const srg_config& cfg)
+// Back-end layouts: work-bin edge S and workgroups per bin (each owning a
+// strip of S x S/k pixels). An 8x8 bin is one wave; every other strip holds
+// at least one pixel per lane of four waves.
+constexpr BackendLayout kLayouts[] = { { 8, 1 }, { 16, 1 }, { 32, 1 }, { 32, 2 }, { 32, 4 }, { 64, 1 }, { 64, 2 }, { 64, 4 } };
+constexpr uint32_t kNumLayouts = sizeof(kLayouts) / sizeof(kLayouts[0]);
+
+bool layout_valid(uint32_t bin_size, uint32_t parts)
 {
-    switch (cfg.bin_size) {
-    case 64: return 1;
-    case 16: return 4;
-    default: return 2;  // 32x32 work bins
-    }
+    for (const BackendLayout& l : kLayouts)
+        if (l.bin_size == bin_size && l.parts == parts) return true;
+    return false;
 }
 
-int launch_backend(srg_ctx* ctx, uint32_t k, const FrameDev& F)
+template <uint32_t S, uint32_t H>
+void launch_backend_t(srg_ctx* ctx, uint32_t wg, bool stats, const FrameDev& F)
 {
-    const uint32_t wg = ctx->cfg.wg_per_cu ? ctx->cfg.wg_per_cu : (k == 1 ? 4u : 8u);
-    const dim3 grid((uint32_t)ctx->num_cus * wg), block(kBackendThreads);
+    const dim3 grid((uint32_t)ctx->num_cus * wg), block(backend_threads(S, H));
+    if (stats) hipLaunchKernelGGL((k_backend<S, H, true>), grid, block, 0, ctx->stream, F);
+    else hipLaunchKernelGGL((k_backend<S, H, false>), grid, block, 0, ctx->stream, F);
+}
+
+int launch_backend(srg_ctx* ctx, BackendLayout L, const FrameDev& F)
+{
+    // resident workgroups per CU: LDS holds 8 bytes per strip pixel
+    const uint32_t wg = ctx->cfg.wg_per_cu ? ctx->cfg.wg_per_cu
+                      : L.bin_size * L.bin_size / L.parts >= 4096u ? 4u : (L.bin_size == 8 ? 16u : 8u);
     const bool stats = (F.flags & FF_STATS) != 0;
-    switch (k) {
-    case 1:
-        if (stats) hipLaunchKernelGGL((k_backend<64, true>), grid, block, 0, ctx->stream, F);
-        else hipLaunchKernelGGL((k_backend<64, false>), grid, block, 0, ctx->stream, F);
-        break;
-    case 4:
-        if (stats) hipLaunchKernelGGL((k_backend<16, true>), grid, block, 0, ctx->stream, F);
-        else hipLaunchKernelGGL((k_backend<16, false>), grid, block, 0, ctx->stream, F);
-        break;
-    default:
-        if (stats) hipLaunchKernelGGL((k_backend<32, true>), grid, block, 0, ctx->stream, F);
-        else hipLaunchKernelGGL((k_backend<32, false>), grid, block, 0, ctx->stream, F);
-        break;
+    switch (L.bin_size * 8u + L.parts) {
+    case 8 * 8 + 1: launch_backend_t<8, 8>(ctx, wg, stats, F); break;
+    case 16 * 8 + 1: launch_backend_t<16, 16>(ctx, wg, stats, F); break;
+    case 32 * 8 + 1: launch_backend_t<32, 32>(ctx, wg, stats, F); break;
+    case 32 * 8 + 2: launch_backend_t<32, 16>(ctx, wg, stats, F); break;
+    case 32 * 8 + 4: launch_backend_t<32, 8>(ctx, wg, stats, F); break;
+    case 64 * 8 + 1: launch_backend_t<64, 64>(ctx, wg, stats, F); break;
+    case 64 * 8 + 2: launch_backend_t<64, 32>(ctx, wg, stats, F); break;
+    case 64 * 8 + 4: launch_backend_t<64, 16>(ctx, wg, stats, F); break;
+    default: return SRG_ERR_INVALID;
     }
     return SRG_OK;
+}
+
+BackendLayout config_layout(const srg_config& cfg)
+{
+    return { cfg.bin_size ? cfg.bin_size : 32u, cfg.wg_per_tile ? cfg.wg_per_tile : 1u };
+}
+
+uint64_t mix64(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    return h * 0xFF51AFD7ED558CCDull;
+}
+
+// The autotuner's layout for a frame of signature sig: the next layout of
+// the sweep, or the chosen one.
+BackendLayout tune_layout(srg_ctx* ctx, uint64_t sig)
+{
+    auto it = ctx->tunes.find(sig);
+    if (it == ctx->tunes.end()) {
+        if (ctx->tunes.size() >= 64) ctx->tunes.clear();
+        TuneState t;
+        t.sweeping = true;
+        t.sig = sig;
+        it = ctx->tunes.emplace(sig, t).first;
+    }
+    return it->second.sweeping ? kLayouts[it->second.cand] : it->second.chosen;
+}
+
+// After a frame of the sweep: keep the layout's best device time; after the
+// last layout, choose the fastest.
+void tune_after_frame(srg_ctx* ctx, uint64_t sig, float ms)
+{
+    TuneState& t = ctx->tunes[sig];
+    if (!t.sweeping) return;
+    float& b = t.best_ms[t.cand];
+    if (b == 0.0f || ms < b) b = ms;
+    const uint32_t frames = ctx->cfg.autotune_frames ? ctx->cfg.autotune_frames : 3u;
+    if (++t.frame < frames) return;
+    t.frame = 0;
+    if (++t.cand < kNumLayouts) return;
+    uint32_t best = 0;
+    for (uint32_t i = 1; i < kNumLayouts; ++i)
+        if (t.best_ms[i] < t.best_ms[best]) best = i;
+    t.chosen = kLayouts[best];
+    t.sweeping = false;
+    t.locked = true;
 }
 
 int run_frame(srg_ctx* ctx)
@@ -390,13 +481,28 @@ int run_frame(srg_ctx* ctx)
         views.push_back({ fb, plane });
         return (uint32_t)views.size() - 1;
     };
-    for (auto& d : ctx->draws) view_of(d.fb, d.plane);
+    std::vector<uint32_t> draw_view;
+    draw_view.reserve(ctx->draws.size());
+    for (auto& d : ctx->draws) draw_view.push_back(view_of(d.fb, d.plane));
     for (srg_fb* fb : ctx->fbs)
         for (uint32_t p = 0; p < 2; ++p)
             if (fb->planes[p].pending) view_of(fb, p);
     if (views.empty()) return SRG_OK;
 
-    const uint32_t K = bin_k_of(ctx->cfg);
+    // back-end layout: the configured one, or the autotuner's for this
+    // frame's signature (view sizes, per-draw triangle counts)
+    const bool autotune = (ctx->cfg.flags & SRG_CFG_AUTOTUNE) != 0;
+    uint64_t sig = 0;
+    BackendLayout L = config_layout(ctx->cfg);
+    if (autotune) {
+        sig = mix64(sig, views.size());
+        for (const ViewKey& v : views) sig = mix64(mix64(sig, v.fb->width), v.fb->height);
+        for (size_t i = 0; i < ctx->draws.size(); ++i) sig = mix64(mix64(sig, ctx->draws[i].index_num), draw_view[i]);
+        L = tune_layout(ctx, sig);
+        ctx->tune_sig = sig;
+    }
+    const bool sweeping = autotune && ctx->tunes[sig].sweeping;
+    const uint32_t K = kBin / L.bin_size;
     std::vector<ViewDev> vdev(views.size());
     uint64_t total_bins = 0;
     for (size_t i = 0; i < views.size(); ++i) {
@@ -549,17 +655,23 @@ int run_frame(srg_ctx* ctx)
         const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((TB + 255) / 256, 1024));
         const uint32_t scan_blocks = (TB + kScanThreads - 1) / kScanThreads;
         const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
-        const bool learned = ctx->cfg.tile_order == SRG_ORDER_LEARNED;
+        // while the autotuner sweeps, the learned order falls back to entry
+        // counts and records nothing (its samples belong to one bin size)
+        const bool learned = ctx->cfg.tile_order == SRG_ORDER_LEARNED && !sweeping;
         if (learned) {
             if ((st = ensure_sched(ctx)) || (st = ctx->sched_feat.ensure(TB)) || (st = ctx->sched_pred.ensure(TB)))
                 return st;
+            if (ctx->sched_bin_size != L.bin_size) {
+                if (ctx->sched_bin_size && (st = reset_sched(ctx))) return st;
+                ctx->sched_bin_size = L.bin_size;
+            }
             F.flags |= FF_TILE_COSTS;
         }
         // The whole frame on the stream: tables, six kernels, counters back.
         auto enqueue = [&](bool split_events) -> int {
             HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
             hipLaunchKernelGGL(k_frame_init, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->counters.ptr,
-                               ctx->bin_count.ptr, ctx->bin_cursor.ptr, TB);
+                               ctx->bin_count.ptr, ctx->bin_cursor.ptr, ctx->bin_cost.ptr, TB);
             if (ctx->cfg.flags & SRG_CFG_KEEP_PRIMID)
                 for (auto& v : vdev)
                     if (v.primid)
@@ -586,7 +698,7 @@ int run_frame(srg_ctx* ctx)
                                    (const uint32_t*)ctx->queue2.ptr,
                                    (const float*)(learned && ctx->sched_trained ? ctx->sched_pred.ptr : nullptr));
             if (split_events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-            launch_backend(ctx, K, F);
+            if (const int lst = launch_backend(ctx, L, F)) return lst;
             if (learned)
                 hipLaunchKernelGGL(k_mlp_record, dim3((TB + 255) / 256), dim3(256), 0, ctx->stream, F,
                                    (const float4*)ctx->sched_feat.ptr,
@@ -598,7 +710,7 @@ int run_frame(srg_ctx* ctx)
             return SRG_OK;
         };
 
-        const bool timing = (ctx->cfg.flags & SRG_CFG_TIMING) != 0;
+        const bool timing = (ctx->cfg.flags & SRG_CFG_TIMING) != 0 || sweeping;
         bool split = false;
         if (ctx->cfg.flags & SRG_CFG_GRAPH) {
             // Replay the captured frame when nothing it depends on changed:
@@ -607,7 +719,7 @@ int run_frame(srg_ctx* ctx)
             std::vector<uint8_t> sig(ctx->h_tables, ctx->h_tables + off);
             const uint64_t extra[] = { (uint64_t)(uintptr_t)ctx->h_tables, (uint64_t)(uintptr_t)ctx->tables.ptr,
                                        (uint64_t)(uintptr_t)ctx->h_counters, nblocks, emit_blocks, clip_blocks,
-                                       init_blocks, TB, K, ctx->cfg.flags, ctx->cfg.wg_per_cu,
+                                       init_blocks, TB, K, L.parts, ctx->cfg.flags, ctx->cfg.wg_per_cu,
                                        (uint64_t)ctx->sched_trained, ctx->cfg.tile_order,
                                        (uint64_t)(uintptr_t)ctx->sched_feat.ptr, (uint64_t)(uintptr_t)ctx->sched_pred.ptr };
             sig.insert(sig.end(), (const uint8_t*)&F, (const uint8_t*)&F + sizeof(F));
@@ -667,6 +779,9 @@ int run_frame(srg_ctx* ctx)
             }
             (void)hipEventElapsedTime(&ctx->stats.ms_total, ctx->ev[0], ctx->ev[2]);
         }
+        if (sweeping) tune_after_frame(ctx, sig, ctx->stats.ms_total);
+        ctx->stats.bin_size = L.bin_size;
+        ctx->stats.wg_per_tile = L.parts;
         break;
     }
     for (auto& v : views) v.fb->planes[v.plane].pending = false;
@@ -701,6 +816,10 @@ int srg_create(const srg_config* cfg, srg_ctx** out)
 {
     if (!out) return SRG_ERR_INVALID;
     *out = nullptr;
+    if (cfg && (cfg->struct_size != sizeof(srg_config) ||
+                !layout_valid(cfg->bin_size ? cfg->bin_size : 32u, cfg->wg_per_tile ? cfg->wg_per_tile : 1u) ||
+                cfg->tile_order > SRG_ORDER_LEARNED))
+        return SRG_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SRG_ERR_NO_DEVICE;
     srg_ctx* ctx = new (std::nothrow) srg_ctx();
@@ -1101,6 +1220,29 @@ int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out)
 {
     if (!ctx || !out) return SRG_ERR_INVALID;
     *out = ctx->stats;
+    return SRG_OK;
+}
+
+int srg_get_tuning(srg_ctx* ctx, srg_tuning* out)
+{
+    if (!ctx || !out) return SRG_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    out->nlayouts = kNumLayouts;
+    static_assert(kNumLayouts == SRG_TUNE_LAYOUTS, "tuning table size");
+    const auto it = ctx->tunes.find(ctx->tune_sig);
+    const TuneState* t = ((ctx->cfg.flags & SRG_CFG_AUTOTUNE) && it != ctx->tunes.end()) ? &it->second : nullptr;
+    for (uint32_t i = 0; i < kNumLayouts; ++i) {
+        out->layouts[i].bin_size = kLayouts[i].bin_size;
+        out->layouts[i].wg_per_tile = kLayouts[i].parts;
+        out->layouts[i].ms = t ? t->best_ms[i] : 0.0f;
+    }
+    if (t) {
+        out->state = t->sweeping ? 1u : 2u;
+        if (t->locked) {
+            out->bin_size = t->chosen.bin_size;
+            out->wg_per_tile = t->chosen.parts;
+        }
+    }
     return SRG_OK;
 }
 

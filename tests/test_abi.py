@@ -59,6 +59,35 @@ def test_no_device_is_an_error_not_a_fallback():
         sr.RenderContext()
 
 
+@pytest.mark.parametrize("bin_size,wg_per_tile", [(16, 2), (8, 2), (8, 4), (16, 4), (48, 1), (32, 3), (128, 1)])
+def test_invalid_layout_rejected_before_device(bin_size, wg_per_tile):
+    """A work bin split into strips must keep whole 8-pixel block rows and at
+    least one pixel per lane; other layouts are refused at creation."""
+    lib = sr.load_library()
+    cfg = sr.SrgConfig()
+    cfg.struct_size = ctypes.sizeof(cfg)
+    cfg.bin_size, cfg.wg_per_tile = bin_size, wg_per_tile
+    h = ctypes.c_void_p()
+    assert lib.srg_create(ctypes.byref(cfg), ctypes.byref(h)) == -1  # SRG_ERR_INVALID
+    assert not h.value
+
+
+def test_valid_layouts_pass_validation():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    lib = sr.load_library()
+    for bin_size, wg_per_tile in sr.LAYOUTS:
+        cfg = sr.SrgConfig()
+        cfg.struct_size = ctypes.sizeof(cfg)
+        cfg.bin_size, cfg.wg_per_tile = bin_size, wg_per_tile
+        h = ctypes.c_void_p()
+        assert lib.srg_create(ctypes.byref(cfg), ctypes.byref(h)) == -8  # past validation: SRG_ERR_NO_DEVICE
+
+
 def test_null_arguments_rejected():
     lib = sr.load_library()
     assert lib.srg_begin_frame(None) == -1

@@ -140,11 +140,85 @@ def test_spheres_1080p_1m_tris():
     _check(sc, "spheres_1m", nthreads=16)
 
 
-@pytest.mark.parametrize("bin_size", [16, 32, 64])
-def test_bin_sizes_same_image(bin_size):
-    """Work-bin size is a scheduling choice: every size reproduces the oracle."""
+# (work-bin edge, workgroups per bin): every layout the library accepts
+LAYOUTS = [(8, 1), (16, 1), (32, 1), (32, 2), (32, 4), (64, 1), (64, 2), (64, 4)]
+
+
+@pytest.mark.parametrize("bin_size,wg_per_tile", LAYOUTS)
+def test_layouts_same_image(bin_size, wg_per_tile):
+    """Work-bin size and workgroups per bin are scheduling choices: every
+    layout reproduces the oracle."""
     sc = helpers.random_scene(40, 4000, 517, 333, clip_heavy=True, ndraws=2, shaders=(0, 1))
-    _check(sc, f"bin_size_{bin_size}", bin_size=bin_size)
+    _check(sc, f"layout_{bin_size}x{wg_per_tile}", bin_size=bin_size, wg_per_tile=wg_per_tile)
+
+
+@pytest.mark.parametrize("bin_size,wg_per_tile", [(8, 1), (32, 4), (64, 2), (64, 4)])
+def test_layouts_clustered_learned(bin_size, wg_per_tile):
+    """Strips and one-wave bins under the learned order: the image is the
+    oracle's, and every non-empty bin has a recorded cost (summed over its
+    strips)."""
+    import softrast_amd as sr
+    from softrast_amd import scenes
+    sc = scenes.clustered_scene(ntris=60000, width=960, height=544, ndraws=2, tex_size=128)
+    tex = helpers.build_textures(sc)
+    ref, rstats = helpers.oracle_render(sc, tex, nthreads=8)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=sr.ORDER_LEARNED,
+                           bin_size=bin_size, wg_per_tile=wg_per_tile, sched_warmup_frames=1)
+    for frame in range(3):
+        _say(f"[parity] case=layout_clustered {bin_size}x{wg_per_tile} frame={frame}")
+        col, dep, pid, st = helpers.gpu_render(sc, tex, ctx=ctx)
+        d = helpers.compare(ref, col, dep, pid)
+        assert not any(d.values()), (frame, d)
+        assert (st["bin_size"], st["wg_per_tile"]) == (bin_size, wg_per_tile)
+    assert st["frags_raster"] == rstats["frags_raster"] and st["sched_trained"]
+    counts, costs = ctx.bin_stats()
+    assert len(counts) == ((sc.width + 63) // 64) * ((sc.height + 63) // 64) * (64 // bin_size) ** 2
+    assert np.all(costs[counts > 0] > 0)
+    ctx.Shutdown()
+
+
+def test_autotune_sweeps_every_layout():
+    """SRG_CFG_AUTOTUNE times each layout over autotune_frames frames of one
+    frame signature, keeps the fastest, and starts a new sweep for a new
+    signature; every frame of the sweep is the oracle's image. The learned
+    order records nothing during the sweep and trains on the chosen layout."""
+    import softrast_amd as sr
+    a = helpers.random_scene(40, 4000, 517, 333, clip_heavy=True, ndraws=2, shaders=(0, 1))
+    b = helpers.random_scene(41, 2000, 300, 200, clip_heavy=True)
+    refs = {}
+    for name, sc in (("a", a), ("b", b)):
+        tex = helpers.build_textures(sc)
+        refs[name] = (sc, tex, helpers.oracle_render(sc, tex, nthreads=8)[0])
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS | sr.CFG_AUTOTUNE, tile_order=sr.ORDER_LEARNED,
+                           autotune_frames=2, sched_warmup_frames=2)
+
+    def frame(name):
+        sc, tex, ref = refs[name]
+        col, dep, pid, st = helpers.gpu_render(sc, tex, ctx=ctx)
+        d = helpers.compare(ref, col, dep, pid)
+        assert not any(d.values()), (name, d)
+        return st
+
+    used = []
+    for i in range(2 * len(LAYOUTS)):
+        _say(f"[parity] case=autotune sweep frame={i}")
+        st = frame("a")
+        used.append((st["bin_size"], st["wg_per_tile"]))
+        assert not st["sched_trained"]
+    assert used == [l for l in LAYOUTS for _ in range(2)]
+    t = ctx.tuning()
+    assert t["state"] == "chosen" and all(v > 0 for v in t["ms"].values())
+    chosen = min(t["ms"], key=t["ms"].get)
+    assert t["layout"] == chosen
+    for i in range(3):
+        st = frame("a")
+        assert (st["bin_size"], st["wg_per_tile"]) == chosen
+    assert st["sched_trained"]
+    st = frame("b")  # a new signature starts its own sweep
+    assert (st["bin_size"], st["wg_per_tile"]) == LAYOUTS[0] and ctx.tuning()["state"] == "sweeping"
+    st = frame("a")  # the first signature keeps its choice
+    assert (st["bin_size"], st["wg_per_tile"]) == chosen and ctx.tuning()["layout"] == chosen
+    ctx.Shutdown()
 
 
 def _multi_view_scene(seed):
@@ -345,7 +419,7 @@ def _sliver_scene(seed, n, width, height):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("bin_size", [16, 32, 64])
-def test_needle_slivers(bin_size):
+@pytest.mark.parametrize("bin_size,wg_per_tile", LAYOUTS)
+def test_needle_slivers(bin_size, wg_per_tile):
     sc = _sliver_scene(61, 20000, 400, 300)
-    _check(sc, f"slivers_bin{bin_size}", bin_size=bin_size)
+    _check(sc, f"slivers_{bin_size}x{wg_per_tile}", bin_size=bin_size, wg_per_tile=wg_per_tile)
