@@ -54,6 +54,11 @@ def parse():
                     help="tile-order study on the clustered 3840x2160 / 4M-triangle workload instead of the default")
     ap.add_argument("--no-plain", action="store_true",
                     help="profiling aid: time only the batched path (--steps launches) and print its rate")
+    ap.add_argument("--workload", choices=("spheres", "mixed", "combined"), default="spheres",
+                    help="mixed: 256 mixed-resolution views per EndFrame under the autotuner; combined: 64 views "
+                         "of 2560x1440 x 2M triangles, learned vs tri-count order (the default is the 1080p/1M "
+                         "workload)")
+    ap.add_argument("--nviews", type=int, default=0, help="mixed/combined: views per batch (0 = the workload's)")
     return ap.parse_args()
 
 
@@ -177,6 +182,138 @@ def sched_study(sr, args):
     print(json.dumps(out))
 
 
+def viewset_calls(sr, vs, fbs, texs):
+    calls = []
+    shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader}
+    for v, fb in zip(vs.views, fbs):
+        for d in v.draws:
+            c = sr.DrawCall().SetFrameBuffer(fb).SetMVP(v.mvp)
+            c.SetIndexBuffer(d.indices, d.indices.dtype.itemsize, len(d.indices))
+            c.SetPositionBuffer(d.vertices, 32, len(d.vertices))
+            c.SetAttributeBuffer(d.vertices, 32, len(d.vertices), 6)
+            c.SetPixelShader(shader[d.shader], texs[d.texture])
+            calls.append((fb, c))
+    return calls
+
+
+def viewset_frame(sr, ctx, fbs, calls):
+    ctx.BeginFrame()
+    for fb in fbs:
+        ctx.ClearFrameBuffer(fb, 0)
+    for fb, c in calls:
+        c.m_frameBuffer = (fb, fb.m_writePlane)
+        ctx.DrawIndexed(c)
+    ctx.EndFrame()
+
+
+def batch_workload(sr, args):
+    """--workload mixed | combined: every view of the batch in one EndFrame
+    (one hipGraph-captured launch sequence); frames/s counts views.
+
+    mixed     256 views, resolutions uniform over {256^2, 512^2, 1024x768,
+              1920x1080}, 1k-500k triangles each (log-uniform), sphere/cube
+              mixes, 8 x 1024^2 textures; tri-count order; the autotuner sweeps
+              bin size {8,16,32,64} x workgroups per bin {1,2,4} first
+    combined  64 views of 2560x1440 x 2,000,000 triangles, alternating uniform
+              and 90/10 clustered layouts, 8 x 2048^2 textures; tri-count
+              order, then the learned order (trained after 16 frames,
+              retrained every 16)
+    """
+    from softrast_amd import scenes
+    t0 = time.time()
+    if args.workload == "mixed":
+        vs = scenes.mixed_views(nviews=args.nviews or 256)
+        runs = [("autotuned", dict(flags=sr.CFG_TIMING | sr.CFG_GRAPH | sr.CFG_AUTOTUNE, tile_order=sr.ORDER_TRICOUNT,
+                                   autotune_frames=2))]
+        config = {"workload": f"mixed_views_{len(vs.views)}", "views": len(vs.views),
+                  "resolutions": "uniform over 256x256, 512x512, 1024x768, 1920x1080",
+                  "triangles_per_view": "log-uniform 1k-500k", "textures": "8x1024^2 RGBA8 + mips (device-built)"}
+    else:
+        vs = scenes.combined_views(nviews=args.nviews or 64)
+        runs = [("tricount", dict(flags=sr.CFG_TIMING | sr.CFG_GRAPH, tile_order=sr.ORDER_TRICOUNT)),
+                ("learned", dict(flags=sr.CFG_TIMING | sr.CFG_GRAPH, tile_order=sr.ORDER_LEARNED,
+                                 sched_warmup_frames=16, sched_retrain_interval=16))]
+        config = {"workload": f"combined_{len(vs.views)}x2560x1440_2m", "views": len(vs.views),
+                  "width": 2560, "height": 1440, "triangles_per_view": vs.views[0].num_tris,
+                  "layouts": "even views uniform, odd views 90% in 3 clusters",
+                  "textures": "8x2048^2 RGBA8 + mips (device-built)"}
+    ntris = vs.num_tris
+    print(f"[bench] {args.workload} views ready: {len(vs.views)} views, {ntris} tris, {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    out = {"metric": f"views/s, {config['workload']} (all views in one EndFrame)", "unit": "views/s",
+           "higher_is_better": True, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "scaling": "weak",
+           "vs_baseline": None, "dtype": "fp32+int32", "data": "synthetic (seeded, see scenes.py)",
+           "config": config, "runs": {}}
+    best = None
+    for name, kw in runs:
+        ctx = sr.RenderContext(bin_size=args.bin_size, **kw)
+        texs = [ctx.CreateTexture(t, t.shape[1], t.shape[0], True) for t in vs.textures]
+        fbs = [sr.FrameBuffer(ctx, v.width, v.height) for v in vs.views]
+        calls = viewset_calls(sr, vs, fbs, texs)
+        t1 = time.time()
+        viewset_frame(sr, ctx, fbs, calls)  # uploads every buffer, grows the arenas
+        print(f"[bench] {name}: first batch {time.time() - t1:.1f}s", file=sys.stderr, flush=True)
+        run = {}
+        if kw["flags"] & sr.CFG_AUTOTUNE:
+            sweep = 1
+            while ctx.tuning()["state"] == "sweeping":
+                viewset_frame(sr, ctx, fbs, calls)
+                sweep += 1
+            t = ctx.tuning()
+            run["autotune"] = {"frames": sweep, "chosen": {"bin_size": t["layout"][0], "wg_per_tile": t["layout"][1]},
+                               "device_ms": {f"bin{b}_wg{w}": round(ms, 3) for (b, w), ms in t["ms"].items()}}
+            print(f"[bench] {name}: autotuner chose {t['layout']} after {sweep} batches", file=sys.stderr, flush=True)
+        warm = args.warmup
+        if kw.get("tile_order") == sr.ORDER_LEARNED:
+            warm = max(warm, kw["sched_warmup_frames"] + 2)
+        for _ in range(warm):
+            viewset_frame(sr, ctx, fbs, calls)
+        st = []
+        t1 = time.perf_counter()  # EndFrame is synchronous: the device is idle here
+        for _ in range(args.steps):
+            viewset_frame(sr, ctx, fbs, calls)
+            st.append(ctx.frame_stats())
+        el = time.perf_counter() - t1
+        run.update({"views_per_s": round(len(vs.views) * args.steps / el, 2), "ms_per_batch": round(1e3 * el / args.steps, 3),
+                    "device_ms_per_batch": round(statistics.mean(s["ms_total"] for s in st), 3),
+                    "layout": {"bin_size": st[-1]["bin_size"], "wg_per_tile": st[-1]["wg_per_tile"]}})
+        if kw.get("tile_order") == sr.ORDER_LEARNED:
+            run["sched_err_log2"] = round(statistics.mean(s["sched_err"] for s in st), 4)
+            run["trainings_in_timed_batches"] = sum(s["sched_trainings"] for s in st)
+        out["runs"][name] = run
+        print(f"[bench] {name}: {run}", file=sys.stderr, flush=True)
+        if best is None or run["views_per_s"] > out["runs"][best]["views_per_s"]:
+            best = name
+        layout = (st[-1]["bin_size"], st[-1]["wg_per_tile"])
+        for fb in fbs:
+            fb.destroy()
+        ctx.Shutdown()
+
+    # untimed frame with statistics on the best run's layout: fragments and covered pixels
+    sctx = sr.RenderContext(flags=sr.CFG_STATS, bin_size=layout[0], wg_per_tile=layout[1])
+    texs = [sctx.CreateTexture(t, t.shape[1], t.shape[0], True) for t in vs.textures]
+    fbs = [sr.FrameBuffer(sctx, v.width, v.height) for v in vs.views]
+    viewset_frame(sr, sctx, fbs, viewset_calls(sr, vs, fbs, texs))
+    sst = sctx.frame_stats()
+    covered = 0
+    for fb in fbs:
+        covered += int(np.count_nonzero(fb.read_tiles()[1].reshape(-1).view(np.uint32) != 0))
+        fb.destroy()
+    sctx.Shutdown()
+    value = out["runs"][best]["views_per_s"]
+    batches = value / len(vs.views)
+    algo = sum(d.indices.dtype.itemsize * len(d.indices) + 32 * len(d.vertices) for v in vs.views for d in v.draws)
+    algo += sum(8 * v.width * v.height for v in vs.views) + 16 * covered
+    out.update({"value": value, "ms_per_step": out["runs"][best]["ms_per_batch"], "best_run": best,
+                "mtris_per_s": round(ntris * batches / 1e6, 1), "frags_raster": sst["frags_raster"],
+                "frags_source": "gpu stats frame", "gfrags_per_s": round(sst["frags_raster"] * batches / 1e9, 3),
+                "roofline": {"bound": "hbm", "kernel": "whole batch", "achieved": round(algo * batches / 1e9, 2),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(algo * batches / 1e9 / HBM_PEAK_GBS, 5), "algorithmic_bytes": algo,
+                             "traffic": None}})
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -188,6 +325,8 @@ def main():
     from softrast_amd import scenes
     if args.sched_study:
         return sched_study(sr, args)
+    if args.workload != "spheres":
+        return batch_workload(sr, args)
 
     t0 = time.time()
     scene = scenes.spheres_scene()

@@ -277,6 +277,80 @@ uint32_t srs_spheres(uint64_t seed, uint32_t nspheres, uint32_t slices, uint32_t
     return (uint32_t)vtx.size();
 }
 
+// Cubes (24 vertices with per-face normals, 12 triangles) with log-normal
+// edge lengths exp(N(0, sigma)), centres uniform in the view frustum of a
+// camera at the origin looking down +z (as srs_spheres), random yaw and
+// pitch. The size scale is chosen so that the summed mean projected area
+// (1.5 s^2 for edge s, a convex body's surface area / 4) is
+// `depth_complexity` times the screen area. UVs 0..1 per face plus a per-cube
+// offset in [0, 4). u32 indices. Returns the vertex count; call with null
+// buffers to size.
+uint32_t srs_cubes(uint64_t seed, uint32_t ncubes, double sigma, double fov_y_rad, double aspect, double zmin,
+                   double zmax, uint32_t width, uint32_t height, double depth_complexity, Vertex* vout,
+                   uint32_t* iout)
+{
+    if (!vout) return ncubes * 24u;
+    Pcg32 rng(seed, 4);
+    std::vector<V3> centre(ncubes);
+    std::vector<double> edge(ncubes), yaw(ncubes), pitch(ncubes), uo(ncubes), vo(ncubes);
+    const double tan_y = std::tan(fov_y_rad * 0.5), tan_x = tan_y * aspect;
+    const double focal = 0.5 * height / tan_y;
+    double sum = 0.0;
+    for (uint32_t c = 0; c < ncubes; ++c) {
+        const double u = rng.uniform();
+        const double z = std::cbrt(zmin * zmin * zmin + u * (zmax * zmax * zmax - zmin * zmin * zmin));
+        centre[c] = { (2.0 * rng.uniform() - 1.0) * tan_x * z, (2.0 * rng.uniform() - 1.0) * tan_y * z, z };
+        edge[c] = std::exp(sigma * rng.normal());
+        yaw[c] = 6.283185307179586 * rng.uniform();
+        pitch[c] = 3.141592653589793 * (rng.uniform() - 0.5);
+        uo[c] = 4.0 * rng.uniform();
+        vo[c] = 4.0 * rng.uniform();
+        const double ps = edge[c] * focal / z;
+        sum += 1.5 * ps * ps;
+    }
+    const double scale = ncubes ? std::sqrt(depth_complexity * (double)width * height / sum) : 1.0;
+    std::vector<Vertex> vtx;
+    vtx.reserve((size_t)ncubes * 24);
+    std::vector<uint32_t> idx;
+    idx.reserve((size_t)ncubes * 36);
+    static const int faces[6][3] = { { 1, 0, 0 }, { -1, 0, 0 }, { 0, 1, 0 }, { 0, -1, 0 }, { 0, 0, 1 }, { 0, 0, -1 } };
+    static const double corner[4][2] = { { -1, -1 }, { 1, -1 }, { 1, 1 }, { -1, 1 } };
+    for (uint32_t c = 0; c < ncubes; ++c) {
+        const double h = 0.5 * edge[c] * scale;
+        const double cyw = std::cos(yaw[c]), syw = std::sin(yaw[c]), cp = std::cos(pitch[c]), sp = std::sin(pitch[c]);
+        // rotation: pitch about x, then yaw about y
+        auto rot = [&](V3 p) {
+            const V3 q = { p.x, cp * p.y - sp * p.z, sp * p.y + cp * p.z };
+            return V3{ cyw * q.x + syw * q.z, q.y, -syw * q.x + cyw * q.z };
+        };
+        for (int f = 0; f < 6; ++f) {
+            const V3 n = { (double)faces[f][0], (double)faces[f][1], (double)faces[f][2] };
+            const V3 a = std::fabs(n.y) > 0.5 ? V3{ 1, 0, 0 } : V3{ 0, 1, 0 };
+            const V3 b = cross(n, a);
+            const uint32_t base = (uint32_t)vtx.size();
+            const V3 nr = rot(n);
+            for (int k = 0; k < 4; ++k) {
+                const V3 p = rot({ h * (n.x + corner[k][0] * a.x + corner[k][1] * b.x),
+                                   h * (n.y + corner[k][0] * a.y + corner[k][1] * b.y),
+                                   h * (n.z + corner[k][0] * a.z + corner[k][1] * b.z) });
+                Vertex v;
+                v.pos[0] = (float)(centre[c].x + p.x);
+                v.pos[1] = (float)(centre[c].y + p.y);
+                v.pos[2] = (float)(centre[c].z + p.z);
+                v.nrm[0] = (float)nr.x; v.nrm[1] = (float)nr.y; v.nrm[2] = (float)nr.z;
+                v.uv[0] = (float)(uo[c] + 0.5 * (corner[k][0] + 1.0));
+                v.uv[1] = (float)(vo[c] + 0.5 * (corner[k][1] + 1.0));
+                vtx.push_back(v);
+            }
+            push_tri(idx, vtx, base, base + 1, base + 2, nr);
+            push_tri(idx, vtx, base, base + 2, base + 3, nr);
+        }
+    }
+    std::memcpy(vout, vtx.data(), sizeof(Vertex) * vtx.size());
+    std::memcpy(iout, idx.data(), sizeof(uint32_t) * idx.size());
+    return (uint32_t)vtx.size();
+}
+
 // Independent triangles for a width x height target, given directly in clip
 // space (draw them with the identity MVP): a fraction `clustered` of the
 // triangles around 3 Gaussian cluster centres (sigma = sigma_frac * width,

@@ -41,6 +41,8 @@ def _load():
         lib.srs_spheres.restype = u32
         lib.srs_clustered_tris.argtypes = [u64, u32, u32, u32, d, d, d, d, vp]
         lib.srs_clustered_tris.restype = None
+        lib.srs_cubes.argtypes = [u64, u32, d, d, d, d, d, u32, u32, d, vp, vp]
+        lib.srs_cubes.restype = u32
         _lib = lib
     return _lib
 
@@ -146,3 +148,126 @@ def clustered_scene(seed=SEED_C3, ntris=4_000_000, width=3840, height=2160, ndra
         draws.append(Draw(vv, np.arange((t1 - t0) * 3, dtype=np.uint32), k if tex_size else -1))
     texs = [value_noise_rgba8(seed ^ (0x7E470000 + k), tex_size, tex_size) for k in range(ndraws)] if tex_size else []
     return Scene(width, height, mvp, draws, texs, {"config": "clustered"})
+
+
+def _splitmix64(state):
+    """One SplitMix64 step: (new state, 64-bit output)."""
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+class _Stream:
+    """Uniform doubles in [0, 1) from SplitMix64 of (master seed, index): the
+    per-view parameters of the multi-view workloads."""
+
+    def __init__(self, seed, index):
+        self.state = (seed * 0x100000001B3 + index * 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+
+    def uniform(self):
+        self.state, z = _splitmix64(self.state)
+        return (z >> 11) * (1.0 / (1 << 53))
+
+    def seed(self):
+        self.state, z = _splitmix64(self.state)
+        return z
+
+
+@dataclass
+class ViewSet:
+    """Several views rendered in one frame (one EndFrame): each view is a
+    Scene (its own size, MVP and draws) whose draws index the shared
+    `textures`."""
+    views: list
+    textures: list
+    info: dict = field(default_factory=dict)
+
+    @property
+    def num_tris(self):
+        return sum(v.num_tris for v in self.views)
+
+
+def _index_array(idx, nverts):
+    return idx.astype(np.uint16) if nverts <= 0x10000 else idx.astype(np.uint32)
+
+
+def shape_mix_view(seed, ntris, width, height, sphere_share, textures, fov=60.0, depth_complexity=4.0,
+                   zmin=20.0, zmax=80.0):
+    """One view of UV spheres (25 x 6, 250 tris, log-normal radii as the
+    spheres scene) and cubes (12 tris, log-normal edges) in the frustum of a
+    camera at the origin looking down +z: about ntris triangles, sphere_share
+    of them in spheres, summed projected area depth_complexity x the screen.
+    Two draws (spheres, cubes), u16 indices where the vertices fit."""
+    lib = _load()
+    aspect = width / height
+    nsph = int(round(sphere_share * ntris / 250.0))
+    ncub = max(1, int(round((ntris - 250 * nsph) / 12.0)))
+    draws = []
+    if nsph:
+        slices, stacks = 25, 6
+        share = 250.0 * nsph / (250.0 * nsph + 12.0 * ncub)
+        nv = lib.srs_spheres(seed, nsph, slices, stacks, 0.8, math.radians(fov), aspect, zmin, zmax, width, height,
+                             depth_complexity * share, None, None, None)
+        v = np.empty(nv, VERTEX_DTYPE)
+        i = np.empty(nsph * 250 * 3, np.uint32)
+        lib.srs_spheres(seed, nsph, slices, stacks, 0.8, math.radians(fov), aspect, zmin, zmax, width, height,
+                        depth_complexity * share, v.ctypes.data, i.ctypes.data, None)
+        draws.append(Draw(v, _index_array(i, nv), textures[0]))
+    else:
+        share = 0.0
+    nv = lib.srs_cubes(seed ^ 0xC0BE, ncub, 0.6, math.radians(fov), aspect, zmin, zmax, width, height,
+                       depth_complexity * (1.0 - share), None, None)
+    v = np.empty(nv, VERTEX_DTYPE)
+    i = np.empty(ncub * 36, np.uint32)
+    lib.srs_cubes(seed ^ 0xC0BE, ncub, 0.6, math.radians(fov), aspect, zmin, zmax, width, height,
+                  depth_complexity * (1.0 - share), v.ctypes.data, i.ctypes.data)
+    draws.append(Draw(v, _index_array(i, nv), textures[1]))
+    mvp = camera_mvp((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), fov, aspect)
+    return Scene(width, height, mvp, draws, [], {"spheres": nsph, "cubes": ncub})
+
+
+MIXED_RESOLUTIONS = ((256, 256), (512, 512), (1024, 768), (1920, 1080))
+
+
+def mixed_views(seed=SEED_C4, nviews=256, resolutions=MIXED_RESOLUTIONS, tri_min=1000, tri_max=500_000, ntex=8,
+                tex_size=1024):
+    """Batched workload: nviews independent views, resolution uniform over
+    `resolutions`, triangle count log-uniform in [tri_min, tri_max], sphere
+    share of the triangles uniform in [0, 1], each view's two draws on two of
+    the ntex tex_size^2 value-noise textures (with mips)."""
+    views = []
+    for k in range(nviews):
+        st = _Stream(seed, k)
+        w, h = resolutions[min(len(resolutions) - 1, int(st.uniform() * len(resolutions)))]
+        ntris = int(round(math.exp(math.log(tri_min) + st.uniform() * (math.log(tri_max) - math.log(tri_min)))))
+        share = st.uniform()
+        views.append(shape_mix_view(st.seed(), ntris, w, h, share, (k % ntex, (k + 1) % ntex)))
+    texs = [value_noise_rgba8(seed ^ (0x7E470000 + t), tex_size, tex_size) for t in range(ntex)]
+    return ViewSet(views, texs, {"config": "mixed_views", "nviews": nviews})
+
+
+def combined_views(seed=SEED_C5, nviews=64, width=2560, height=1440, ntris=2_000_000, ntex=8, tex_size=2048,
+                   threads=16):
+    """Combined workload: nviews views of width x height, each of ntris
+    independent triangles in clip space (identity MVP) laid out as the
+    clustered scene: even views uniform over the screen, odd views 90% in 3
+    Gaussian clusters. One draw per view on texture view % ntex; the views
+    share one u32 index buffer (0, 1, 2, ...)."""
+    from concurrent.futures import ThreadPoolExecutor
+    lib = _load()
+    idx = np.arange(ntris * 3, dtype=np.uint32)
+    mvp = np.eye(4, dtype=np.float32).reshape(-1)
+
+    def make(k):
+        st = _Stream(seed, k)
+        v = np.empty(ntris * 3, VERTEX_DTYPE)
+        lib.srs_clustered_tris(st.seed(), ntris, width, height, 0.9 if k % 2 else 0.0, 0.05, 0.5, 2000.0,
+                               v.ctypes.data)
+        return Scene(width, height, mvp, [Draw(v, idx, k % ntex)], [], {"clustered": 0.9 if k % 2 else 0.0})
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        views = list(ex.map(make, range(nviews)))
+        texs = list(ex.map(lambda t: value_noise_rgba8(seed ^ (0x7E470000 + t), tex_size, tex_size), range(ntex)))
+    return ViewSet(views, texs, {"config": "combined_views", "nviews": nviews})

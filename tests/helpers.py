@@ -142,3 +142,44 @@ def compare(oracle_plane, colour, depth, primid=None):
     out["colour"] = int(np.count_nonzero(diff))
     out["colour_gt1"] = int(np.count_nonzero(diff > 1))
     return out
+
+
+def oracle_render_views(viewset, texdata, nthreads=8, clear=0):
+    """Every view of a scenes.ViewSet in one oracle frame, one plane per view
+    (the order key runs over all views' draws, as in one EndFrame)."""
+    from oracle.oracle import Oracle, Plane, make_texture
+    o = Oracle(nthreads=nthreads)
+    planes = [Plane(v.width, v.height) for v in viewset.views]
+    for p in planes:
+        o.clear(p, clear)
+    stex = [make_texture(t) for t in texdata]
+    draws = []
+    for vi, v in enumerate(viewset.views):
+        for d in v.draws:
+            vf = d.vertices.view(np.float32).reshape(-1, 8)
+            draws.append(dict(shader=d.shader, tex=stex[d.texture][0] if d.texture >= 0 else None, idx=d.indices,
+                              idx_stride=d.indices.dtype.itemsize, pos=vf, pos_stride=ATTR_STRIDE, pos_num=len(vf),
+                              attr=vf, attr_stride=ATTR_STRIDE, attr_num=len(vf), uv_offset=d.uv_offset, plane=vi,
+                              mvp=v.mvp))
+    stats = o.render(draws, planes)
+    o.close()
+    return planes, stats
+
+
+def gpu_render_views(viewset, texdata, ctx, fbs, clear=0):
+    """Every view of a scenes.ViewSet in one EndFrame, view k into fbs[k];
+    `texdata` holds TextureData or device textures by ViewSet texture index."""
+    import softrast_amd as sr
+    shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader}
+    ctx.BeginFrame()
+    for v, fb in zip(viewset.views, fbs):
+        ctx.ClearFrameBuffer(fb, clear)
+        for d in v.draws:
+            call = sr.DrawCall().SetFrameBuffer(fb).SetMVP(v.mvp)
+            call.SetIndexBuffer(d.indices, d.indices.dtype.itemsize, len(d.indices))
+            call.SetPositionBuffer(d.vertices, ATTR_STRIDE, len(d.vertices))
+            call.SetAttributeBuffer(d.vertices, ATTR_STRIDE, len(d.vertices), d.uv_offset)
+            call.SetPixelShader(shader[d.shader], texdata[d.texture] if d.texture >= 0 else None)
+            ctx.DrawIndexed(call)
+    ctx.EndFrame()
+    return ctx.frame_stats()

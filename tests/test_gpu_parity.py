@@ -423,3 +423,66 @@ def _sliver_scene(seed, n, width, height):
 def test_needle_slivers(bin_size, wg_per_tile):
     sc = _sliver_scene(61, 20000, 400, 300)
     _check(sc, f"slivers_{bin_size}x{wg_per_tile}", bin_size=bin_size, wg_per_tile=wg_per_tile)
+
+
+def _viewset_textures(vs):
+    from softrast_amd import TextureData
+    return [TextureData().CreateFromRGBA8(t, t.shape[1], t.shape[0], True) for t in vs.textures]
+
+
+def _compare_views(vs, planes, fbs, what):
+    for vi, (fb, p) in enumerate(zip(fbs, planes)):
+        col, dep, pid = fb.read_tiles(primid=True)
+        d = helpers.compare(p, col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
+        assert not any(d.values()), (what, vi, vs.views[vi].width, vs.views[vi].height, d)
+
+
+def test_mixed_views_batch_autotuned():
+    """The mixed-views batch at reduced triangle counts: 12 views of the four
+    resolutions (sphere/cube mixes, u16 and u32 indices) in one
+    hipGraph-captured EndFrame while the autotuner sweeps every layout; every
+    view of every frame is the oracle's image."""
+    import softrast_amd as sr
+    from softrast_amd import scenes
+    vs = scenes.mixed_views(nviews=12, tri_max=20000, tex_size=128)
+    tex = _viewset_textures(vs)
+    planes, rstats = helpers.oracle_render_views(vs, tex)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS | sr.CFG_GRAPH | sr.CFG_AUTOTUNE,
+                           tile_order=sr.ORDER_TRICOUNT, autotune_frames=1)
+    fbs = [sr.FrameBuffer(ctx, v.width, v.height) for v in vs.views]
+    for frame in range(len(LAYOUTS) + 2):
+        _say(f"[parity] case=mixed_views frame={frame} tris={vs.num_tris}")
+        st = helpers.gpu_render_views(vs, tex, ctx, fbs)
+        _compare_views(vs, planes, fbs, frame)
+        assert st["frags_raster"] == rstats["frags_raster"]
+    t = ctx.tuning()
+    assert t["state"] == "chosen" and (st["bin_size"], st["wg_per_tile"]) == t["layout"]
+    for fb in fbs:
+        fb.destroy()
+    ctx.Shutdown()
+
+
+def test_combined_views_learned_retrain():
+    """The combined workload at reduced size: 4 views alternating uniform and
+    90/10 clustered layouts in one EndFrame, learned order trained after 2
+    frames and retrained every 2; every view of every frame is the oracle's
+    image."""
+    import softrast_amd as sr
+    from softrast_amd import scenes
+    vs = scenes.combined_views(nviews=4, width=640, height=360, ntris=30000, tex_size=128, threads=4)
+    tex = _viewset_textures(vs)
+    planes, rstats = helpers.oracle_render_views(vs, tex)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=sr.ORDER_LEARNED,
+                           sched_warmup_frames=2, sched_retrain_interval=2)
+    fbs = [sr.FrameBuffer(ctx, v.width, v.height) for v in vs.views]
+    trainings = 0
+    for frame in range(7):
+        _say(f"[parity] case=combined_views frame={frame}")
+        st = helpers.gpu_render_views(vs, tex, ctx, fbs)
+        _compare_views(vs, planes, fbs, frame)
+        assert st["frags_raster"] == rstats["frags_raster"]
+        trainings += st["sched_trainings"]
+    assert st["sched_trained"] and trainings >= 3
+    for fb in fbs:
+        fb.destroy()
+    ctx.Shutdown()
