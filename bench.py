@@ -149,8 +149,11 @@ def sched_study(sr, args):
            "config": {"workload": "clustered_4k_4m", "width": scene.width, "height": scene.height,
                       "triangles": scene.num_tris, "textures": "16x4096^2 RGBA8 + mips (device-built)"},
            "orders": {}}
-    for order, name in ((sr.ORDER_NATURAL, "natural"), (sr.ORDER_TRICOUNT, "tricount"), (sr.ORDER_LEARNED, "learned")):
-        ctx = sr.RenderContext(flags=sr.CFG_TIMING, tile_order=order, bin_size=args.bin_size, sched_warmup_frames=64)
+    runs = ((sr.ORDER_NATURAL, "natural", 0), (sr.ORDER_TRICOUNT, "tricount", 0), (sr.ORDER_LEARNED, "learned", 0),
+            (sr.ORDER_TRICOUNT, "tricount_autotuned", sr.CFG_AUTOTUNE))
+    for order, name, extra in runs:
+        ctx = sr.RenderContext(flags=sr.CFG_TIMING | extra, tile_order=order, bin_size=args.bin_size,
+                               sched_warmup_frames=64, autotune_frames=2)
         texs = [ctx.CreateTexture(t, t.shape[1], t.shape[0], True) for t in scene.textures]
         fb = sr.FrameBuffer(ctx, scene.width, scene.height)
         calls = make_calls(sr, fb, scene, texs, scene.mvp)
@@ -172,7 +175,11 @@ def sched_study(sr, args):
         el = time.perf_counter() - t
         out["orders"][name] = {"frames_per_s": round(args.steps / el, 2),
                                "device_ms": round(statistics.mean(s["ms_total"] for s in st), 4),
-                               "backend_ms": round(statistics.mean(s["ms_backend"] for s in st), 4)}
+                               "backend_ms": round(statistics.mean(s["ms_backend"] for s in st), 4),
+                               "layout": {"bin_size": st[-1]["bin_size"], "wg_per_tile": st[-1]["wg_per_tile"]}}
+        if extra & sr.CFG_AUTOTUNE:
+            t = ctx.tuning()
+            out["orders"][name]["autotune_device_ms"] = {f"bin{b}_wg{w}": round(ms, 3) for (b, w), ms in t["ms"].items()}
         if order == sr.ORDER_LEARNED:
             out["orders"][name]["sched_err_log2"] = round(statistics.mean(s["sched_err"] for s in st), 4)
             out["orders"][name]["trained"] = bool(st[-1]["sched_trained"])

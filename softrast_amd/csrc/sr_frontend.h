@@ -195,17 +195,6 @@ __device__ __forceinline__ BinWalk bin_walk(const ViewDev& V, uint32_t bin_k, ui
     return w;
 }
 
-// Candidate i of the walk: returns the global bin index, or ~0 if its tile
-// fails the corner test.
-__device__ __forceinline__ uint32_t bin_candidate(const BinWalk& w, uint32_t i, const int32_t c[3], const int32_t dx[3],
-                                                  const int32_t dy[3])
-{
-    const uint32_t sbx = w.sbx0 + i % w.nx, sby = w.sby0 + i / w.nx;
-    const uint32_t tx = sbx / w.k, ty = sby / w.k;
-    if (w.check && !tile_accepts(c, dx, dy, tx * kBin, ty * kBin)) return 0xFFFFFFFFu;
-    return w.bin_base + sby * w.view_bins_x + sbx;
-}
-
 // Number of 64x64 tiles the reference bins the primitive into (statistics).
 __device__ __forceinline__ uint32_t reference_tile_entries(const Prim& s)
 {
@@ -221,22 +210,71 @@ __device__ __forceinline__ uint32_t reference_tile_entries(const Prim& s)
 // Counts (emit = false) or appends (emit = true) a primitive's bin entries,
 // wave-aggregated. Uniform control flow required; `active` lanes carry a
 // primitive described by (c, dx, dy, bbox) in view V.
+//
+// The wave's (primitive, candidate bin) pairs are compacted: an inclusive
+// prefix of the candidate counts numbers them, and each pass hands 64 of them
+// to the 64 lanes (the owner of pair `it` is the first lane whose prefix
+// exceeds it), so a wave takes ceil(sum / 64) passes rather than as many as
+// its largest primitive has candidates.
 __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, bool active, const int32_t c[3],
                                           const int32_t dx[3], const int32_t dy[3], int32_t xmin, int32_t xmax,
                                           int32_t ymin, int32_t ymax, bool emit, uint32_t rec)
 {
     BinWalk w;
-    if (active) w = bin_walk(V, F.bin_k, F.bin_size, xmin, xmax, ymin, ymax);
-    else w.ncand = 0;
-    for (uint32_t i = 0; __ballot(i < w.ncand) != 0ull; ++i) {
-        uint32_t key = 0xFFFFFFFFu;
-        if (i < w.ncand) key = bin_candidate(w, i, c, dx, dy);
-        const bool act = key != 0xFFFFFFFFu;
+    if (active) {
+        w = bin_walk(V, F.bin_k, F.bin_size, xmin, xmax, ymin, ymax);
+    } else {
+        w.sbx0 = w.sby0 = w.nx = w.ncand = 0;
+        w.view_bins_x = w.bin_base = 0;
+        w.check = false;
+    }
+    const uint32_t lane = lane_id();
+    uint32_t incl = w.ncand;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    const uint32_t total = readlane_u32(incl, 63);
+    const uint32_t first = incl - w.ncand;
+    const uint32_t pk_org = w.sbx0 | (w.sby0 << 16);                  // bin coordinates < 2^16
+    const uint32_t pk_nx = max(w.nx, 1u) | ((w.check ? 1u : 0u) << 31);  // nx < 2^31
+    const bool any_check = __ballot(w.check && w.ncand != 0) != 0ull;
+    const uint32_t K = F.bin_k;
+    for (uint32_t base = 0; base < total; base += 64) {
+        const uint32_t it = base + lane;
+        const bool valid = it < total;
+        uint32_t lo = 0, hi = 63;
+#pragma unroll
+        for (int step = 0; step < 6; ++step) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (__shfl(incl, (int)mid, 64) > it) hi = mid;
+            else lo = mid + 1;
+        }
+        const int src = (int)(valid ? lo : 0u);
+        const uint32_t org = __shfl(pk_org, src, 64), pn = __shfl(pk_nx, src, 64);
+        const uint32_t i = it - __shfl(first, src, 64);
+        const uint32_t bb = __shfl(w.bin_base, src, 64), vbx = __shfl(w.view_bins_x, src, 64);
+        const uint32_t orec = __shfl(rec, src, 64);
+        const uint32_t nx = pn & 0x7FFFFFFFu;
+        const uint32_t sbx = (org & 0xFFFFu) + i % nx, sby = (org >> 16) + i / nx;
+        bool act = valid;
+        if (any_check) {  // the reference's corner test for primitives spanning 3+ bin rows
+            int32_t oc[3], odx[3], ody[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                oc[k] = __shfl(c[k], src, 64);
+                odx[k] = __shfl(dx[k], src, 64);
+                ody[k] = __shfl(dy[k], src, 64);
+            }
+            if (act && (pn >> 31) && !tile_accepts(oc, odx, ody, (sbx / K) * kBin, (sby / K) * kBin)) act = false;
+        }
+        const uint32_t key = act ? bb + sby * vbx + sbx : 0u;
         if (!emit) {
-            wave_count(F.bin_count, act ? key : 0u, act);
+            wave_count(F.bin_count, key, act);
         } else {
-            const uint32_t slot = wave_add_slot(F.bin_cursor, act ? key : 0u, act);
-            if (act) F.entries[F.bin_start[key] + slot] = rec;
+            const uint32_t slot = wave_add_slot(F.bin_cursor, key, act);
+            if (act) F.entries[F.bin_start[key] + slot] = orec;
         }
     }
 }

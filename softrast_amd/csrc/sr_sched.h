@@ -7,7 +7,9 @@
 //   k_bin_features  per work bin, from its entry list: x0 = log2(1 + entries),
 //                   x1 = log2(1 + walked 8x8 blocks),
 //                   x2 = log2(1 + walked blocks of primitives without a tight bound),
-//                   x3 = log2(1 + summed bbox-in-bin area / bin area)
+//                   x3 = log2(1 + summed bbox-in-bin area / bin area);
+//                   the sums are estimated from at most 64 entries spread
+//                   evenly over the list, scaled by entries / sampled
 //   k_mlp_infer     4 -> 32 -> 32 -> 1 ReLU MLP (fp32) -> predicted log2 cost
 //   k_mlp_record    after the back end: (features, log2 measured s_memtime
 //                   cost) samples into a ring buffer, and the model's error
@@ -49,8 +51,10 @@ __global__ __launch_bounds__(256) void k_bin_features(FrameDev F, float4* feat)
     const int32_t X0 = (int32_t)((sbx / K) * kBin), Y0 = (int32_t)((sby / K) * kBin);
     const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S;
     const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
+    const uint32_t ns = cnt < 64u ? cnt : 64u;  // sampled entries: lane j takes entry j * cnt / ns
     uint32_t blocks = 0, open_blocks = 0, area = 0;
-    for (uint32_t e = lane; e < cnt; e += 64) {
+    if (lane < ns) {
+        const uint32_t e = (uint32_t)(((uint64_t)lane * cnt) / ns);
         const uint4 q2 = F.rec_raster[(size_t)F.entries[beg + e] * 4 + 2];
         const int32_t xmin = q2.y & 0x3FFF, xmax = (q2.y >> 16) & 0x3FFF, ymin = q2.z & 0x3FFF, ymax = (q2.z >> 16) & 0x3FFF;
         const uint32_t xs = max((uint32_t)clampi(xmin - X0, 0, 64) & ~7u, sx0);
@@ -67,9 +71,11 @@ __global__ __launch_bounds__(256) void k_bin_features(FrameDev F, float4* feat)
     blocks = wave_sum(blocks);
     open_blocks = wave_sum(open_blocks);
     area = wave_sum(area);
+    const float scale = ns ? (float)cnt / (float)ns : 0.0f;
     if (lane == 0)
-        feat[gb] = make_float4(__log2f(1.0f + (float)cnt), __log2f(1.0f + (float)blocks),
-                               __log2f(1.0f + (float)open_blocks), __log2f(1.0f + (float)area / (float)(S * S)));
+        feat[gb] = make_float4(__log2f(1.0f + (float)cnt), __log2f(1.0f + scale * (float)blocks),
+                               __log2f(1.0f + scale * (float)open_blocks),
+                               __log2f(1.0f + scale * (float)area / (float)(S * S)));
 }
 
 __device__ __forceinline__ float mlp_forward(const MlpParams& P, const float x[kFeat], float h1[kHid], float h2[kHid])
