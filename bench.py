@@ -10,7 +10,10 @@ starts. Two paths are timed:
   plain    one view per EndFrame: clear + 8 DrawIndexed + synchronous EndFrame
   batched  --views V views of the same scene (camera yaw spread 3 degrees per
            view, one framebuffer each) in one EndFrame, launched as a replayed
-           hipGraph, tile queue ordered by --tile-order
+           hipGraph, work-bin layout chosen by the library's autotuner, tile
+           queue ordered by --tile-order (default: the learned cost model,
+           trained on the chosen layout before timing); the same path with
+           the tri-count order is timed beside it
 
 A step is one EndFrame; frames/s counts views completed per second
 (value = the batched path). Also reported:
@@ -43,7 +46,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--views", type=int, default=8, help="views per EndFrame in the batched path")
-    ap.add_argument("--tile-order", type=int, default=1, help="batched path: 0 natural, 1 tri-count, 2 learned")
+    ap.add_argument("--tile-order", type=int, default=2, help="batched path: 0 natural, 1 tri-count, 2 learned")
+    ap.add_argument("--no-autotune", action="store_true", help="batched path: library default layout")
     ap.add_argument("--bin-size", type=int, default=0, help="work-bin edge 16/32/64 (0 = library default)")
     ap.add_argument("--cpu-frames", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(host cores, 16)")
@@ -82,8 +86,10 @@ def make_calls(sr, fb, scene, texs, mvp):
 
 
 def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup):
-    """Returns (frames/s, ms per EndFrame, per-frame stats, ctx, framebuffers)."""
-    ctx = sr.RenderContext(flags=flags, tile_order=tile_order, bin_size=bin_size)
+    """Returns (frames/s, ms per EndFrame, per-frame stats, ctx, framebuffers).
+    Untimed frames first: `warmup`, and with the autotuner or the learned
+    order, until the layout is chosen and the model trained on it."""
+    ctx = sr.RenderContext(flags=flags, tile_order=tile_order, bin_size=bin_size, autotune_frames=3)
     fbs, calls = [], []
     for v in range(views):
         fb = sr.FrameBuffer(ctx, scene.width, scene.height)
@@ -101,6 +107,12 @@ def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup
         ctx.EndFrame()
 
     for _ in range(warmup):  # also uploads and caches every buffer and texture
+        frame()
+    for _ in range(200):
+        tuned = not (flags & sr.CFG_AUTOTUNE) or ctx.tuning()["state"] == "chosen"
+        trained = tile_order != sr.ORDER_LEARNED or ctx.frame_stats()["sched_trained"]
+        if tuned and trained:
+            break
         frame()
     stats = []
     t0 = time.perf_counter()  # EndFrame is synchronous: the device is idle here
@@ -126,8 +138,9 @@ def load_traffic(name):
 
 def profile_batched(sr, scene, texs, args):
     """--no-plain: the batched path alone (for a profiler); prints its rate only."""
-    fps, ms, _, ctx, fbs = time_path(sr, scene, texs, args.views, sr.CFG_TIMING | sr.CFG_GRAPH, args.tile_order,
-                                     args.bin_size, args.steps, args.warmup)
+    flags = sr.CFG_TIMING | sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
+    fps, ms, _, ctx, fbs = time_path(sr, scene, texs, args.views, flags, args.tile_order, args.bin_size, args.steps,
+                                     args.warmup)
     for fb in fbs:
         fb.destroy()
     ctx.Shutdown()
@@ -365,16 +378,30 @@ def main():
 
     # ---- batched path ----
     fps_batched = ms_batched = dev_batched = None
+    batched_runs = {}
     if not args.no_batched:
-        fps_batched, ms_batched, st_b, bctx, bfbs = time_path(sr, scene, texs, args.views,
-                                                              sr.CFG_TIMING | sr.CFG_GRAPH, args.tile_order,
-                                                              args.bin_size, max(20, args.steps // args.views),
-                                                              args.warmup)
-        dev_batched = statistics.mean(s["ms_total"] for s in st_b)
-        for fb in bfbs:
-            fb.destroy()
-        bctx.Shutdown()
-        print(f"[bench] batched x{args.views}: {fps_batched:.1f} frames/s", file=sys.stderr, flush=True)
+        bflags = sr.CFG_TIMING | sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
+        orders = [args.tile_order] + ([sr.ORDER_TRICOUNT] if args.tile_order != sr.ORDER_TRICOUNT else [])
+        for order in orders:
+            fps, ms, st_b, bctx, bfbs = time_path(sr, scene, texs, args.views, bflags, order, args.bin_size,
+                                                  max(20, args.steps // args.views), args.warmup)
+            run = {"tile_order": ("natural", "tricount", "learned")[order], "frames_per_s": round(fps, 2),
+                   "ms_per_launch": round(ms, 4),
+                   "device_ms_per_launch": round(statistics.mean(s["ms_total"] for s in st_b), 4),
+                   "layout": {"bin_size": st_b[-1]["bin_size"], "wg_per_tile": st_b[-1]["wg_per_tile"]}}
+            if bflags & sr.CFG_AUTOTUNE:
+                t = bctx.tuning()
+                run["autotune_device_ms"] = {f"bin{b}_wg{w}": round(v, 4) for (b, w), v in t["ms"].items()}
+            if order == sr.ORDER_LEARNED:
+                run["sched_err_log2"] = round(statistics.mean(s["sched_err"] for s in st_b), 4)
+            batched_runs[run["tile_order"]] = run
+            if order == args.tile_order:
+                fps_batched, ms_batched, dev_batched = fps, ms, run["device_ms_per_launch"]
+            for fb in bfbs:
+                fb.destroy()
+            bctx.Shutdown()
+            print(f"[bench] batched x{args.views} {run['tile_order']}: {fps:.1f} frames/s, layout {run['layout']}",
+                  file=sys.stderr, flush=True)
 
     value = fps_batched if fps_batched else fps_plain
     out = {
@@ -393,7 +420,9 @@ def main():
         "config": {"workload": "spheres_1080p_1m", "width": scene.width, "height": scene.height,
                    "triangles_per_view": scene.num_tris, "draws_per_view": len(scene.draws),
                    "textures": "8x2048^2 RGBA8 + mips",
-                   "path": (f"batched x{args.views} views, hipGraph, tile order {args.tile_order}"
+                   "path": (f"batched x{args.views} views, hipGraph, "
+                            f"{'autotuned layout, ' if not args.no_autotune else ''}"
+                            f"{('natural', 'tri-count', 'learned')[args.tile_order]} tile order"
                             if fps_batched else "plain single-view")},
         "plain": {"frames_per_s": round(fps_plain, 2), "ms_per_frame": round(ms_plain, 4),
                   "device_ms": {"frontend": round(fe_ms, 4), "backend": round(be_ms, 4), "total": round(dev_ms, 4)}},
@@ -406,7 +435,7 @@ def main():
     if fps_batched:
         out["batched"] = {"views": args.views, "frames_per_s": round(fps_batched, 2),
                           "ms_per_launch": round(ms_batched, 4), "device_ms_per_launch": round(dev_batched, 4),
-                          "vs_plain": round(fps_batched / fps_plain, 3)}
+                          "vs_plain": round(fps_batched / fps_plain, 3), "runs": batched_runs}
 
     if not args.no_parity or not args.no_cpu_baseline:
         from oracle.oracle import Oracle, Plane, make_texture

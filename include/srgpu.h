@@ -122,6 +122,8 @@ typedef struct srg_frame_stats {
     uint32_t sched_trainings; /* SRG_ORDER_LEARNED: training runs done by this frame */
     uint32_t bin_size;        /* back-end layout of this frame: work-bin edge ... */
     uint32_t wg_per_tile;     /* ... and workgroups per bin */
+    uint32_t sched_fallback;  /* SRG_ORDER_LEARNED: 1 while the model's last measured error
+                                 (sched_err) exceeds 1.0 and the entry-count order is used */
 } srg_frame_stats;
 
 /* SRG_CFG_AUTOTUNE state for the signature of the last frame. */

@@ -80,7 +80,7 @@ class SrgFrameStats(ctypes.Structure):
                 ("bin_entries", ctypes.c_uint64), ("work_entries", ctypes.c_uint64), ("frags_raster", ctypes.c_uint64), ("tiles_queued", ctypes.c_uint32), ("retries", ctypes.c_uint32), ("graph_captures", ctypes.c_uint32),
                 ("ms_frontend", ctypes.c_float), ("ms_backend", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("sched_err", ctypes.c_float), ("sched_trained", ctypes.c_uint32), ("sched_trainings", ctypes.c_uint32),
-                ("bin_size", ctypes.c_uint32), ("wg_per_tile", ctypes.c_uint32)]
+                ("bin_size", ctypes.c_uint32), ("wg_per_tile", ctypes.c_uint32), ("sched_fallback", ctypes.c_uint32)]
 
 
 TUNE_LAYOUTS = 8

@@ -4,13 +4,15 @@
 // expensive bins first shortens the frame's tail. Every kernel here changes
 // only the order bins are processed in, never the arithmetic of a pixel.
 //
-//   k_bin_features  per work bin, from its entry list: x0 = log2(1 + entries),
+//   k_bin_predict   one wave per work bin: the features, from its entry list:
+//                   x0 = log2(1 + entries),
 //                   x1 = log2(1 + walked 8x8 blocks),
 //                   x2 = log2(1 + walked blocks of primitives without a tight bound),
 //                   x3 = log2(1 + summed bbox-in-bin area / bin area);
 //                   the sums are estimated from at most 64 entries spread
-//                   evenly over the list, scaled by entries / sampled
-//   k_mlp_infer     4 -> 32 -> 32 -> 1 ReLU MLP (fp32) -> predicted log2 cost
+//                   evenly over the list, scaled by entries / sampled. Then,
+//                   once trained, the 4 -> 32 -> 32 -> 1 ReLU MLP (fp32) ->
+//                   predicted log2 cost, one hidden unit per lane
 //   k_mlp_record    after the back end: (features, log2 measured s_memtime
 //                   cost) samples into a ring buffer, and the model's error
 //   k_mlp_train     one workgroup, mini-batch Adam on the recorded samples
@@ -37,11 +39,16 @@ struct SchedSample {
     float y;  // log2 measured cost
 };
 
-__global__ __launch_bounds__(256) void k_bin_features(FrameDev F, float4* feat)
+__global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, const MlpParams* params, float* pred)
 {
+    __shared__ MlpParams P;
     // An overflowed frame has bin starts past the entry arena and no entry
     // lists (k_emit skipped it); it is rendered again with larger arenas.
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
+    if (params) {
+        for (uint32_t i = threadIdx.x; i < kMlpParams; i += 256) ((float*)&P)[i] = ((const float*)params)[i];
+        __syncthreads();
+    }
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t gb = blockIdx.x * 4u + (threadIdx.x >> 6);
     if (gb >= F.total_bins) return;  // whole wave exits together
@@ -72,10 +79,26 @@ __global__ __launch_bounds__(256) void k_bin_features(FrameDev F, float4* feat)
     open_blocks = wave_sum(open_blocks);
     area = wave_sum(area);
     const float scale = ns ? (float)cnt / (float)ns : 0.0f;
-    if (lane == 0)
-        feat[gb] = make_float4(__log2f(1.0f + (float)cnt), __log2f(1.0f + scale * (float)blocks),
-                               __log2f(1.0f + scale * (float)open_blocks),
-                               __log2f(1.0f + scale * (float)area / (float)(S * S)));
+    const float x[kFeat] = { __log2f(1.0f + (float)cnt), __log2f(1.0f + scale * (float)blocks),
+                             __log2f(1.0f + scale * (float)open_blocks),
+                             __log2f(1.0f + scale * (float)area / (float)(S * S)) };
+    if (lane == 0) feat[gb] = make_float4(x[0], x[1], x[2], x[3]);
+    if (!params) return;
+    // hidden unit j = lane & 31 (the upper half repeats it); layer 2 reads
+    // unit i of layer 1 from lane i
+    const uint32_t j = lane & 31u;
+    float a = P.b1[j];
+#pragma unroll
+    for (uint32_t k = 0; k < kFeat; ++k) a += P.w1[j][k] * x[k];
+    const float h1 = a > 0.0f ? a : 0.0f;
+    float b = P.b2[j];
+#pragma unroll
+    for (uint32_t i = 0; i < kHid; ++i) b += P.w2[j][i] * __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, h1), (int)i));
+    const float h2 = b > 0.0f ? b : 0.0f;
+    float o = lane < kHid ? P.w3[j] * h2 : 0.0f;
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) o += __shfl_xor(o, m, 64);
+    if (lane == 0) pred[gb] = cnt ? exp2f(fminf(P.b3 + o, 120.0f)) : 0.0f;
 }
 
 __device__ __forceinline__ float mlp_forward(const MlpParams& P, const float x[kFeat], float h1[kHid], float h2[kHid])
@@ -100,25 +123,6 @@ __device__ __forceinline__ float mlp_forward(const MlpParams& P, const float x[k
     return o;
 }
 
-// Predicted cost per bin (2^out); bins without entries predict 0.
-__global__ __launch_bounds__(256) void k_mlp_infer(FrameDev F, const float4* feat, const MlpParams* params, float* pred)
-{
-    __shared__ MlpParams P;
-    for (uint32_t i = threadIdx.x; i < kMlpParams; i += 256) ((float*)&P)[i] = ((const float*)params)[i];
-    __syncthreads();
-    const uint32_t gb = blockIdx.x * 256u + threadIdx.x;
-    if (gb >= F.total_bins) return;
-    if (F.bin_count[gb] == 0) {
-        pred[gb] = 0.0f;
-        return;
-    }
-    const float4 f = feat[gb];
-    const float x[kFeat] = { f.x, f.y, f.z, f.w };
-    float h1[kHid], h2[kHid];
-    const float o = mlp_forward(P, x, h1, h2);
-    pred[gb] = exp2f(fminf(o, 120.0f));
-}
-
 // After the back end: append (features, log2 cost) of every non-empty bin to
 // the ring buffer and accumulate |predicted - measured| in log2 units.
 __global__ __launch_bounds__(256) void k_mlp_record(FrameDev F, const float4* feat, const float* pred,
@@ -130,15 +134,20 @@ __global__ __launch_bounds__(256) void k_mlp_record(FrameDev F, const float4* fe
     const bool ok = clean && gb < F.total_bins && F.bin_count[gb] > 0 && F.bin_cost[gb] > 0;
     const uint32_t slot = wave_add_amount(ring_head, ok ? 1u : 0u);
     const uint32_t nok = wave_sum(ok ? 1u : 0u);
-    if (lane_id() == 0 && nok) atomicAdd(&F.counters[C_SCHED_N], nok);
+    const float y = ok ? __log2f((float)F.bin_cost[gb]) : 0.0f;
+    float err = (ok && pred) ? fabsf(__log2f(fmaxf(pred[gb], 1.0f)) - y) : 0.0f;
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) err += __shfl_xor(err, m, 64);
+    if (lane_id() == 0 && nok) {  // one atomic per wave, not per bin
+        atomicAdd(&F.counters[C_SCHED_N], nok);
+        if (pred) atomicAdd((float*)&F.counters[C_SCHED_ERR], err);
+    }
     if (!ok) return;
     const float4 f = feat[gb];
-    const float y = __log2f((float)F.bin_cost[gb]);
     SchedSample s;
     s.x[0] = f.x; s.x[1] = f.y; s.x[2] = f.z; s.x[3] = f.w;
     s.y = y;
     ring[slot % ring_cap] = s;
-    if (pred) atomicAdd((float*)&F.counters[C_SCHED_ERR], fabsf(__log2f(fmaxf(pred[gb], 1.0f)) - y));
 }
 
 struct AdamState {

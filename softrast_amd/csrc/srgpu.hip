@@ -208,6 +208,7 @@ struct srg_ctx {
     DevArray<float4> sched_feat;
     DevArray<float> sched_pred;
     bool sched_trained = false;
+    bool sched_fallback = false;  // last measured error above kSchedFallbackErr: entry-count order
     uint32_t sched_frames = 0, sched_last_train = 0;
     uint32_t sched_bin_size = 0;  // bin size the recorded samples and the model belong to
     std::unordered_map<uint64_t, TuneState> tunes;  // autotuner, per frame signature
@@ -336,6 +337,10 @@ int ensure_sched(srg_ctx* ctx)
     return SRG_OK;
 }
 
+// Mean |log2 predicted - log2 measured| bin cost above which the learned
+// order gives way to the entry-count order (a factor of 2 off on average).
+constexpr float kSchedFallbackErr = 1.0f;
+
 // A new bin size: the recorded samples and the model describe other bins, so
 // the model starts again from its initial weights and an empty ring.
 int reset_sched(srg_ctx* ctx)
@@ -345,6 +350,7 @@ int reset_sched(srg_ctx* ctx)
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->sched_head, 0u, (size_t)1);
     HIP_TRY(hipGetLastError());
     ctx->sched_trained = false;
+    ctx->sched_fallback = false;
     ctx->sched_frames = 0;
     ctx->sched_last_train = 0;
     return SRG_OK;
@@ -358,6 +364,7 @@ int sched_after_frame(srg_ctx* ctx)
     float err;
     std::memcpy(&err, &ctx->h_counters[C_SCHED_ERR], sizeof(err));
     ctx->stats.sched_err = (ctx->sched_trained && n) ? err / (float)n : 0.0f;
+    if (ctx->sched_trained && n) ctx->sched_fallback = ctx->stats.sched_err > kSchedFallbackErr;
     ctx->sched_frames++;
     const uint32_t warm = ctx->cfg.sched_warmup_frames ? ctx->cfg.sched_warmup_frames : 8u;
     uint32_t steps = 0;
@@ -381,6 +388,7 @@ int sched_after_frame(srg_ctx* ctx)
         ctx->stats.sched_trainings++;
     }
     ctx->stats.sched_trained = ctx->sched_trained ? 1u : 0u;
+    ctx->stats.sched_fallback = ctx->sched_fallback ? 1u : 0u;
     return SRG_OK;
 }
 
@@ -667,6 +675,9 @@ int run_frame(srg_ctx* ctx)
             }
             F.flags |= FF_TILE_COSTS;
         }
+        // the model orders the queue once trained, unless its last measured
+        // error was too large: then the entry-count order is the fallback
+        const bool use_model = ctx->sched_trained && !ctx->sched_fallback;
         // The whole frame on the stream: tables, six kernels, counters back.
         auto enqueue = [&](bool split_events) -> int {
             HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
@@ -687,16 +698,14 @@ int run_frame(srg_ctx* ctx)
                                (const uint32_t*)ctx->scan_blk.ptr);
             hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
             if (learned) {
-                hipLaunchKernelGGL(k_bin_features, dim3((TB + 3) / 4), dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr);
-                if (ctx->sched_trained)
-                    hipLaunchKernelGGL(k_mlp_infer, dim3((TB + 255) / 256), dim3(256), 0, ctx->stream, F,
-                                       (const float4*)ctx->sched_feat.ptr, (const MlpParams*)ctx->sched_params,
-                                       ctx->sched_pred.ptr);
+                hipLaunchKernelGGL(k_bin_predict, dim3((TB + 3) / 4), dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr,
+                                   (const MlpParams*)(ctx->sched_trained ? ctx->sched_params : nullptr),
+                                   ctx->sched_pred.ptr);
             }
             if (ordered)
                 hipLaunchKernelGGL(k_order, dim3(1), dim3(kScanThreads), 0, ctx->stream, F,
                                    (const uint32_t*)ctx->queue2.ptr,
-                                   (const float*)(learned && ctx->sched_trained ? ctx->sched_pred.ptr : nullptr));
+                                   (const float*)(learned && use_model ? ctx->sched_pred.ptr : nullptr));
             if (split_events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
             if (const int lst = launch_backend(ctx, L, F)) return lst;
             if (learned)
@@ -720,7 +729,7 @@ int run_frame(srg_ctx* ctx)
             const uint64_t extra[] = { (uint64_t)(uintptr_t)ctx->h_tables, (uint64_t)(uintptr_t)ctx->tables.ptr,
                                        (uint64_t)(uintptr_t)ctx->h_counters, nblocks, emit_blocks, clip_blocks,
                                        init_blocks, TB, K, L.parts, ctx->cfg.flags, ctx->cfg.wg_per_cu,
-                                       (uint64_t)ctx->sched_trained, ctx->cfg.tile_order,
+                                       (uint64_t)ctx->sched_trained | ((uint64_t)ctx->sched_fallback << 1), ctx->cfg.tile_order,
                                        (uint64_t)(uintptr_t)ctx->sched_feat.ptr, (uint64_t)(uintptr_t)ctx->sched_pred.ptr };
             sig.insert(sig.end(), (const uint8_t*)&F, (const uint8_t*)&F + sizeof(F));
             sig.insert(sig.end(), (const uint8_t*)extra, (const uint8_t*)extra + sizeof(extra));
@@ -1264,6 +1273,7 @@ int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count)
     HIP_TRY(hipMemcpyAsync(ctx->sched_params, params, sizeof(MlpParams), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->sched_trained = true;
+    ctx->sched_fallback = false;
     return SRG_OK;
 }
 

@@ -486,3 +486,23 @@ def test_combined_views_learned_retrain():
     for fb in fbs:
         fb.destroy()
     ctx.Shutdown()
+
+
+def test_learned_order_falls_back_on_large_error():
+    """A model whose measured error exceeds one log2 unit hands the queue
+    back to the entry-count order (the fallback); the image never changes."""
+    import softrast_amd as sr
+    sc = helpers.random_scene(42, 3000, 400, 300, clip_heavy=True)
+    tex = helpers.build_textures(sc)
+    ref, _ = helpers.oracle_render(sc, tex, nthreads=8)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=sr.ORDER_LEARNED)
+    bad = np.zeros(ctx.SCHED_PARAMS, np.float32)
+    bad[-1] = 40.0  # output bias: every bin predicted at 2^40 ticks
+    ctx.set_sched_model(bad)
+    for frame in range(3):
+        _say(f"[parity] case=sched_fallback frame={frame}")
+        col, dep, pid, st = helpers.gpu_render(sc, tex, ctx=ctx)
+        d = helpers.compare(ref, col, dep, pid)
+        assert not any(d.values()), (frame, d)
+        assert st["sched_trained"] and st["sched_err"] > 1.0 and st["sched_fallback"] == 1, st
+    ctx.Shutdown()
