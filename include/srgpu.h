@@ -100,7 +100,8 @@ typedef struct srg_draw {
     srg_handle attribute_buffer; uint64_t attribute_offset; uint32_t attribute_stride; uint32_t attribute_num;
     uint32_t uv_offset;
     srg_fb* fb;                   /* target framebuffer */
-    uint32_t plane;               /* its plane (srg_fb_write_plane at SetFrameBuffer time) */
+    uint32_t plane;               /* its plane (srg_fb_write_plane at SetFrameBuffer time), or
+                                     SRG_PLANE_WRITE: the write plane when the draw is submitted */
     float mvp[16];
 } srg_draw;
 
@@ -174,6 +175,13 @@ int srg_fb_write_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, const uint32_t*
 int srg_begin_frame(srg_ctx* ctx);
 int srg_clear(srg_ctx* ctx, srg_fb* fb, uint32_t colour, int clear_colour, int clear_depth);
 int srg_draw_indexed(srg_ctx* ctx, const srg_draw* draw);
+/* n draws in one call, each as srg_draw_indexed; returns the first failing
+ * status (the draws before it are recorded). No counterpart in the reference,
+ * whose DrawIndexed takes one DrawCall (Renderer.cpp:161-166): a binding
+ * that pays per foreign call submits a prepared draw list with it, and
+ * SRG_PLANE_WRITE keeps such a list valid across SwapPlanes. */
+#define SRG_PLANE_WRITE 0xFFFFFFFFu
+int srg_draw_indexed_n(srg_ctx* ctx, const srg_draw* draws, uint32_t n);
 /* Synchronous like the reference: on return every tile is rastered and shaded. */
 int srg_end_frame(srg_ctx* ctx);
 /* Copies the write plane's colour to linear RGBA8 rows (width*height*4 bytes),

@@ -44,8 +44,9 @@ SRG_SYMBOLS = (
     "srg_fb_write_tiles", "srg_begin_frame", "srg_clear", "srg_draw_indexed", "srg_end_frame", "srg_blit",
     "srg_get_frame_stats", "srg_get_bin_stats", "srg_sched_get_model", "srg_sched_set_model",
     "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
-    "srg_get_tuning",
+    "srg_get_tuning", "srg_draw_indexed_n",
 )
+PLANE_WRITE = 0xFFFFFFFF
 
 
 class SrgError(RuntimeError):
@@ -140,6 +141,7 @@ def load_library(path=None):
         "srg_sched_get_model": (i32, [vp, vp, u32]),
         "srg_sched_set_model": (i32, [vp, vp, u32]),
         "srg_get_tuning": (i32, [vp, ctypes.POINTER(SrgTuning)]),
+        "srg_draw_indexed_n": (i32, [vp, vp, u32]),
         "srg_texture_build_rgba8": (i32, [vp, u32, u32, i32, vp, pu64, ctypes.POINTER(u32),
                                           ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     }
@@ -355,6 +357,28 @@ class FrameBuffer:
             self._h = None
 
 
+class DrawList:
+    """DrawCalls prepared once (buffers registered, srg_draw records built)
+    and submitted each frame with one library call. Each call draws to its
+    framebuffer's write plane at submission (SRG_PLANE_WRITE), so the list
+    stays valid across SwapPlanes / Blit. Build a new list after changing a
+    call or invalidating one of its buffers."""
+
+    def __init__(self, ctx, calls):
+        self._calls = list(calls)  # keeps the calls' arrays alive
+        self._n = len(self._calls)
+        self._arr = (SrgDraw * max(1, self._n))()
+        for k, c in enumerate(self._calls):
+            if c.m_indexBuffer.m_ptr is None:
+                raise SrgError(-1, "DrawList: no index buffer bound")
+            d = ctx._prepare_draw(c, np.asarray(c.m_mvp, np.float32).reshape(-1))
+            d.fb, d.plane = c.m_frameBuffer[0]._h, PLANE_WRITE
+            self._arr[k] = d
+
+    def __len__(self):
+        return self._n
+
+
 class RenderContext:
     """RenderContext (Renderer.h:153-177) over libsrgpu.
 
@@ -452,6 +476,16 @@ class RenderContext:
         _check(self._lib.srg_draw_indexed(self._ctx, ctypes.byref(d)), "DrawIndexed")
         call.m_drawCallIdx = len(self._draws)
         self._draws.append(call)
+
+    def PrepareDrawList(self, calls):
+        """A DrawList of these DrawCalls (see DrawList)."""
+        return DrawList(self, calls)
+
+    def DrawIndexedList(self, draws):
+        """DrawIndexed for every call of a DrawList, in order, with one
+        library call (srg_draw_indexed_n)."""
+        _check(self._lib.srg_draw_indexed_n(self._ctx, ctypes.byref(draws._arr), draws._n), "DrawIndexedList")
+        self._draws.extend(draws._calls)
 
     def _prepare_draw(self, call, mvp):
         d = SrgDraw()

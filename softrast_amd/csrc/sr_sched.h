@@ -39,19 +39,21 @@ struct SchedSample {
     float y;  // log2 measured cost
 };
 
-__global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, const MlpParams* params, float* pred)
+// The MLP as k_bin_predict holds it in LDS: layer 2 transposed, so that lane
+// j reading weight (j, i) for every i walks consecutive banks.
+struct MlpLds {
+    float w1[kHid][kFeat];
+    float b1[kHid];
+    float w2t[kHid][kHid];  // w2t[i][j] = w2[j][i]
+    float b2[kHid];
+    float w3[kHid];
+    float b3;
+};
+
+// Features of bin gb and, once trained, its predicted cost (one wave).
+__device__ __forceinline__ void bin_predict_one(const FrameDev& F, const MlpLds& P, bool trained, uint32_t gb,
+                                                uint32_t lane, float4* feat, float* pred)
 {
-    __shared__ MlpParams P;
-    // An overflowed frame has bin starts past the entry arena and no entry
-    // lists (k_emit skipped it); it is rendered again with larger arenas.
-    if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
-    if (params) {
-        for (uint32_t i = threadIdx.x; i < kMlpParams; i += 256) ((float*)&P)[i] = ((const float*)params)[i];
-        __syncthreads();
-    }
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t gb = blockIdx.x * 4u + (threadIdx.x >> 6);
-    if (gb >= F.total_bins) return;  // whole wave exits together
     const ViewDev V = F.views[view_of_bin(F, gb)];
     const uint32_t S = F.bin_size, K = F.bin_k;
     const uint32_t lb = gb - V.bin_base, sbx = lb % V.bins_x, sby = lb / V.bins_x;
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, c
                              __log2f(1.0f + scale * (float)open_blocks),
                              __log2f(1.0f + scale * (float)area / (float)(S * S)) };
     if (lane == 0) feat[gb] = make_float4(x[0], x[1], x[2], x[3]);
-    if (!params) return;
+    if (!trained) return;
     // hidden unit j = lane & 31 (the upper half repeats it); layer 2 reads
     // unit i of layer 1 from lane i
     const uint32_t j = lane & 31u;
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, c
     const float h1 = a > 0.0f ? a : 0.0f;
     float b = P.b2[j];
 #pragma unroll
-    for (uint32_t i = 0; i < kHid; ++i) b += P.w2[j][i] * __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, h1), (int)i));
+    for (uint32_t i = 0; i < kHid; ++i) b += P.w2t[i][j] * __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, h1), (int)i));
     const float h2 = b > 0.0f ? b : 0.0f;
     float o = lane < kHid ? P.w3[j] * h2 : 0.0f;
 #pragma unroll
@@ -121,6 +123,30 @@ __device__ __forceinline__ float mlp_forward(const MlpParams& P, const float x[k
 #pragma unroll
     for (uint32_t i = 0; i < kHid; ++i) o += P.w3[i] * h2[i];
     return o;
+}
+
+// Grid-stride over bins, one wave per bin at a time, so each workgroup loads
+// the parameters once.
+__global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, const MlpParams* params, float* pred)
+{
+    __shared__ MlpLds P;
+    // An overflowed frame has bin starts past the entry arena and no entry
+    // lists (k_emit skipped it); it is rendered again with larger arenas.
+    if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
+    if (params) {
+        for (uint32_t i = threadIdx.x; i < kHid * kFeat; i += 256) (&P.w1[0][0])[i] = (&params->w1[0][0])[i];
+        for (uint32_t i = threadIdx.x; i < kHid * kHid; i += 256) P.w2t[i % kHid][i / kHid] = (&params->w2[0][0])[i];
+        if (threadIdx.x < kHid) {
+            P.b1[threadIdx.x] = params->b1[threadIdx.x];
+            P.b2[threadIdx.x] = params->b2[threadIdx.x];
+            P.w3[threadIdx.x] = params->w3[threadIdx.x];
+        }
+        if (threadIdx.x == 0) P.b3 = params->b3;
+        __syncthreads();
+    }
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t gb = blockIdx.x * 4u + (threadIdx.x >> 6); gb < F.total_bins; gb += gridDim.x * 4u)
+        bin_predict_one(F, P, params != nullptr, gb, lane, feat, pred);
 }
 
 // After the back end: append (features, log2 cost) of every non-empty bin to

@@ -698,7 +698,8 @@ int run_frame(srg_ctx* ctx)
                                (const uint32_t*)ctx->scan_blk.ptr);
             hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
             if (learned) {
-                hipLaunchKernelGGL(k_bin_predict, dim3((TB + 3) / 4), dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr,
+                hipLaunchKernelGGL(k_bin_predict, dim3(std::min<uint32_t>((TB + 3) / 4, 4u * (uint32_t)ctx->num_cus)),
+                                   dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr,
                                    (const MlpParams*)(ctx->sched_trained ? ctx->sched_params : nullptr),
                                    ctx->sched_pred.ptr);
             }
@@ -1126,7 +1127,7 @@ int srg_draw_indexed(srg_ctx* ctx, const srg_draw* d)
 {
     if (!ctx || !d) return SRG_ERR_INVALID;
     if (!ctx->in_frame) return SRG_ERR_STATE;
-    if (!d->fb || d->plane > 1) return SRG_ERR_INVALID;
+    if (!d->fb || (d->plane > 1 && d->plane != SRG_PLANE_WRITE)) return SRG_ERR_INVALID;
     if (std::find(ctx->fbs.begin(), ctx->fbs.end(), d->fb) == ctx->fbs.end()) return SRG_ERR_HANDLE;
     if (!valid_handle(ctx, d->index_buffer) || !valid_handle(ctx, d->position_buffer) ||
         !valid_handle(ctx, d->attribute_buffer))
@@ -1151,6 +1152,15 @@ int srg_draw_indexed(srg_ctx* ctx, const srg_draw* d)
         d->attribute_offset + (uint64_t)(d->attribute_num - 1) * d->attribute_stride + d->attribute_stride > ab.bytes)
         return SRG_ERR_INVALID;
     ctx->draws.push_back(*d);
+    if (d->plane == SRG_PLANE_WRITE) ctx->draws.back().plane = d->fb->write_plane;
+    return SRG_OK;
+}
+
+int srg_draw_indexed_n(srg_ctx* ctx, const srg_draw* draws, uint32_t n)
+{
+    if (!ctx || (!draws && n)) return SRG_ERR_INVALID;
+    for (uint32_t i = 0; i < n; ++i)
+        if (const int st = srg_draw_indexed(ctx, draws + i)) return st;
     return SRG_OK;
 }
 

@@ -506,3 +506,44 @@ def test_learned_order_falls_back_on_large_error():
         assert not any(d.values()), (frame, d)
         assert st["sched_trained"] and st["sched_err"] > 1.0 and st["sched_fallback"] == 1, st
     ctx.Shutdown()
+
+
+def test_draw_list_follows_write_plane():
+    """A DrawList (one srg_draw_indexed_n call per frame) renders the oracle's
+    image, and after Blit swaps the planes it draws into the new write plane
+    (SRG_PLANE_WRITE), leaving the blitted plane as it was."""
+    import softrast_amd as sr
+    sc = helpers.random_scene(43, 3000, 320, 240, clip_heavy=True, ndraws=3, shaders=(0, 1, 2))
+    tex = helpers.build_textures(sc)
+    refs = [helpers.oracle_render(sc, tex, nthreads=8, clear=c)[0] for c in (0, 0x40)]
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS)
+    fb = sr.FrameBuffer(ctx, sc.width, sc.height)
+    shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader}
+    calls = []
+    for d in sc.draws:
+        c = sr.DrawCall().SetFrameBuffer(fb).SetMVP(sc.mvp)
+        c.SetIndexBuffer(d.indices, d.indices.dtype.itemsize, len(d.indices))
+        c.SetPositionBuffer(d.vertices, 32, len(d.vertices)).SetAttributeBuffer(d.vertices, 32, len(d.vertices), 6)
+        c.SetPixelShader(shader[d.shader], tex[d.texture] if d.texture >= 0 else None)
+        calls.append(c)
+    draws = ctx.PrepareDrawList(calls)
+    assert len(draws) == len(sc.draws)
+    shown = None
+    for frame in range(2):
+        _say(f"[parity] case=draw_list frame={frame}")
+        plane = fb.m_writePlane
+        ctx.BeginFrame()
+        ctx.ClearFrameBuffer(fb, (0, 0x40)[frame])  # the two frames differ in the background
+        ctx.DrawIndexedList(draws)
+        ctx.EndFrame()
+        col, dep, pid = fb.read_tiles(primid=True)
+        d = helpers.compare(refs[frame], col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
+        assert not any(d.values()), (frame, d)
+        if shown is not None:  # the plane blitted last frame was not drawn into
+            assert np.array_equal(fb.read_tiles(plane=plane ^ 1)[0], shown)
+            assert not np.array_equal(col, shown)
+        shown = col.copy()
+        ctx.Blit(fb)
+        assert fb.m_writePlane == plane ^ 1
+    fb.destroy()
+    ctx.Shutdown()
