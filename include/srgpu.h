@@ -219,6 +219,40 @@ int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count);
  * (Config.h:18-21) and whose tiles each take one task (Renderer.cpp:264-306). */
 int srg_get_tuning(srg_ctx* ctx, srg_tuning* out);
 
+/* ---- scene ingest: Obj::Model::Load (Viewer/Obj.cpp:374-567) ----
+ * Host code (no device needed). Reads a Wavefront OBJ as the reference's
+ * viewer does: v / vt / vn / f (triangles and quads, 1-based or negative
+ * indices), one vertex per distinct (position, uv, normal) triple of a mesh,
+ * "g" starts a new mesh, usemtl / mtllib / newmtl / map_Kd name materials.
+ * Meshes come out as Obj::Mesh (Obj.h:24-42): Obj::Vertex {pos3, norm3, uv2}
+ * and u16 indices, u32 above 65,535 vertices. The flags are Obj::LoadFlags
+ * (Obj.h:54-60). The reference's "<path>.bin" cache (kt serializer) is not
+ * written or read. Parse errors and unreadable files return SRG_ERR_INVALID. */
+enum srg_obj_flags {
+    SRG_OBJ_FLIP_WINDING = 1u, /* swap the 2nd and 3rd index of every triangle */
+    SRG_OBJ_FLIP_UVS = 4u      /* v -> 1 - v */
+};
+typedef struct srg_obj_model srg_obj_model;
+typedef struct srg_obj_mesh {
+    uint32_t index_stride;     /* 2 (IndexType::u16) or 4 (IndexType::u32) */
+    uint32_t num_indices;
+    const void* indices;       /* owned by the model */
+    uint32_t num_vertices;
+    const float* vertices;     /* 8 floats per vertex, owned by the model */
+    uint32_t material;         /* index into the model's materials (0 without any) */
+} srg_obj_mesh;
+int srg_obj_load(const char* path, uint32_t flags, srg_obj_model** out);
+/* The same from text in memory; mtllib names are read relative to dir
+ * (NULL: the working directory). */
+int srg_obj_parse(const char* text, size_t bytes, const char* dir, uint32_t flags, srg_obj_model** out);
+uint32_t srg_obj_mesh_count(const srg_obj_model* model);
+int srg_obj_get_mesh(const srg_obj_model* model, uint32_t index, srg_obj_mesh* out);
+uint32_t srg_obj_material_count(const srg_obj_model* model);
+const char* srg_obj_material_name(const srg_obj_model* model, uint32_t index);
+/* The map_Kd path of a material, joined to the OBJ's directory ("" if none). */
+const char* srg_obj_material_diffuse(const srg_obj_model* model, uint32_t index);
+void srg_obj_free(srg_obj_model* model);
+
 /* ---- host helper: TextureData::CreateFromRGBA8 (Texture.cpp:119-199) ----
  * Tiles a power-of-two RGBA8 image (>= 32 in both dimensions) into 32x32
  * Morton-swizzled tiles and, if calc_mips, appends the mip chain. Call with

@@ -44,8 +44,11 @@ SRG_SYMBOLS = (
     "srg_fb_write_tiles", "srg_begin_frame", "srg_clear", "srg_draw_indexed", "srg_end_frame", "srg_blit",
     "srg_get_frame_stats", "srg_get_bin_stats", "srg_sched_get_model", "srg_sched_set_model",
     "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
-    "srg_get_tuning", "srg_draw_indexed_n",
+    "srg_get_tuning", "srg_draw_indexed_n", "srg_obj_load", "srg_obj_parse", "srg_obj_mesh_count",
+    "srg_obj_get_mesh", "srg_obj_material_count", "srg_obj_material_name", "srg_obj_material_diffuse",
+    "srg_obj_free",
 )
+OBJ_FLIP_WINDING, OBJ_FLIP_UVS = 1, 4  # Obj::LoadFlags (Viewer/Obj.h:54-60)
 PLANE_WRITE = 0xFFFFFFFF
 
 
@@ -82,6 +85,11 @@ class SrgFrameStats(ctypes.Structure):
                 ("ms_frontend", ctypes.c_float), ("ms_backend", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("sched_err", ctypes.c_float), ("sched_trained", ctypes.c_uint32), ("sched_trainings", ctypes.c_uint32),
                 ("bin_size", ctypes.c_uint32), ("wg_per_tile", ctypes.c_uint32), ("sched_fallback", ctypes.c_uint32)]
+
+
+class SrgObjMesh(ctypes.Structure):
+    _fields_ = [("index_stride", ctypes.c_uint32), ("num_indices", ctypes.c_uint32), ("indices", ctypes.c_void_p),
+                ("num_vertices", ctypes.c_uint32), ("vertices", ctypes.c_void_p), ("material", ctypes.c_uint32)]
 
 
 TUNE_LAYOUTS = 8
@@ -142,6 +150,14 @@ def load_library(path=None):
         "srg_sched_set_model": (i32, [vp, vp, u32]),
         "srg_get_tuning": (i32, [vp, ctypes.POINTER(SrgTuning)]),
         "srg_draw_indexed_n": (i32, [vp, vp, u32]),
+        "srg_obj_load": (i32, [ctypes.c_char_p, u32, ctypes.POINTER(vp)]),
+        "srg_obj_parse": (i32, [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, u32, ctypes.POINTER(vp)]),
+        "srg_obj_mesh_count": (u32, [vp]),
+        "srg_obj_get_mesh": (i32, [vp, u32, ctypes.POINTER(SrgObjMesh)]),
+        "srg_obj_material_count": (u32, [vp]),
+        "srg_obj_material_name": (ctypes.c_char_p, [vp, u32]),
+        "srg_obj_material_diffuse": (ctypes.c_char_p, [vp, u32]),
+        "srg_obj_free": (None, [vp]),
         "srg_texture_build_rgba8": (i32, [vp, u32, u32, i32, vp, pu64, ctypes.POINTER(u32),
                                           ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     }
@@ -355,6 +371,57 @@ class FrameBuffer:
         if self._h:
             self._ctx._lib.srg_fb_destroy(self._ctx._ctx, self._h)
             self._h = None
+
+
+class ObjMesh:
+    """Obj::Mesh (Viewer/Obj.h:24-42): `vertices` in Obj::Vertex layout
+    (pos3, norm3, uv2; scenes.VERTEX_DTYPE), `indices` uint16 or uint32."""
+
+    def __init__(self, vertices, indices, material):
+        self.vertices, self.indices, self.material = vertices, indices, material
+
+
+class ObjModel:
+    """Obj::Model::Load (Viewer/Obj.cpp:374-567) through srg_obj_load /
+    srg_obj_parse: `meshes` (ObjMesh) and `materials` ((name, map_Kd path))."""
+
+    def __init__(self, handle, lib):
+        from . import scenes
+        try:
+            self.meshes = []
+            for i in range(lib.srg_obj_mesh_count(handle)):
+                m = SrgObjMesh()
+                _check(lib.srg_obj_get_mesh(handle, i, ctypes.byref(m)), "obj mesh")
+                dt = np.uint16 if m.index_stride == 2 else np.uint32
+                idx = np.ctypeslib.as_array((ctypes.c_uint8 * (m.num_indices * m.index_stride)).from_address(m.indices)) \
+                    .view(dt).copy() if m.num_indices else np.zeros(0, dt)
+                vtx = np.ctypeslib.as_array((ctypes.c_uint8 * (m.num_vertices * 32)).from_address(m.vertices)) \
+                    .view(scenes.VERTEX_DTYPE).copy() if m.num_vertices else np.zeros(0, scenes.VERTEX_DTYPE)
+                self.meshes.append(ObjMesh(vtx, idx, m.material))
+            self.materials = [(lib.srg_obj_material_name(handle, i).decode(),
+                               lib.srg_obj_material_diffuse(handle, i).decode())
+                              for i in range(lib.srg_obj_material_count(handle))]
+        finally:
+            lib.srg_obj_free(handle)
+
+
+def obj_load(path, flags=0):
+    """Loads a Wavefront OBJ file as the reference's viewer does (flags:
+    OBJ_FLIP_WINDING, OBJ_FLIP_UVS). Raises SrgError on a parse error."""
+    lib = load_library()
+    h = ctypes.c_void_p()
+    _check(lib.srg_obj_load(str(path).encode(), flags, ctypes.byref(h)), "obj_load")
+    return ObjModel(h, lib)
+
+
+def obj_parse(text, flags=0, directory=None):
+    """obj_load on OBJ text in memory; mtllib files are read from `directory`."""
+    lib = load_library()
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    h = ctypes.c_void_p()
+    _check(lib.srg_obj_parse(data, len(data), str(directory).encode() if directory else None, flags,
+                             ctypes.byref(h)), "obj_parse")
+    return ObjModel(h, lib)
 
 
 class DrawList:

@@ -29,6 +29,7 @@
 #include "srgpu.h"
 #include "sr_sched.h"
 #include "sr_texture.h"
+#include "sr_obj.h"
 
 namespace srg {
 
@@ -1611,5 +1612,88 @@ int srg_texture_read(srg_ctx* ctx, srg_handle h, uint8_t* out, uint64_t capacity
     if (t.bytes) HIP_TRY(hipMemcpy(out, t.dev, t.bytes, hipMemcpyDeviceToHost));
     return SRG_OK;
 }
+
+}  // extern "C"
+
+struct srg_obj_model {
+    srg_obj::Model model;
+};
+
+namespace {
+
+std::string dir_of(const char* path)
+{
+    const char* slash = std::strrchr(path, '/');
+    return slash ? std::string(path, (size_t)(slash - path) + 1) : std::string();
+}
+
+int obj_parse(const char* text, size_t bytes, std::string dir, uint32_t flags, srg_obj_model** out)
+{
+    srg_obj_model* m = new (std::nothrow) srg_obj_model();
+    if (!m) return SRG_ERR_NOMEM;
+    srg_obj::Parser parser(m->model, flags, std::move(dir));
+    if (!srg_obj::parse_text(parser, text, bytes)) {
+        delete m;
+        return SRG_ERR_INVALID;
+    }
+    *out = m;
+    return SRG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int srg_obj_parse(const char* text, size_t bytes, const char* dir, uint32_t flags, srg_obj_model** out)
+{
+    if (!out || (!text && bytes)) return SRG_ERR_INVALID;
+    *out = nullptr;
+    std::string d = dir ? std::string(dir) : std::string();
+    if (!d.empty() && d.back() != '/') d += '/';
+    return obj_parse(text, bytes, std::move(d), flags, out);
+}
+
+int srg_obj_load(const char* path, uint32_t flags, srg_obj_model** out)
+{
+    if (!out || !path) return SRG_ERR_INVALID;
+    *out = nullptr;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return SRG_ERR_INVALID;
+    std::vector<char> text;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = std::fread(buf, 1, sizeof(buf), f)) > 0) text.insert(text.end(), buf, buf + n);
+    std::fclose(f);
+    return obj_parse(text.data(), text.size(), dir_of(path), flags, out);
+}
+
+uint32_t srg_obj_mesh_count(const srg_obj_model* m) { return m ? (uint32_t)m->model.meshes.size() : 0u; }
+
+int srg_obj_get_mesh(const srg_obj_model* m, uint32_t index, srg_obj_mesh* out)
+{
+    if (!m || !out || index >= m->model.meshes.size()) return SRG_ERR_INVALID;
+    const srg_obj::Mesh& s = m->model.meshes[index];
+    out->index_stride = s.index_stride;
+    out->num_indices = s.num_indices;
+    out->indices = s.index_data.data();
+    out->num_vertices = (uint32_t)s.vertices.size();
+    out->vertices = s.vertices.empty() ? nullptr : s.vertices[0].pos;
+    out->material = s.material;
+    return SRG_OK;
+}
+
+uint32_t srg_obj_material_count(const srg_obj_model* m) { return m ? (uint32_t)m->model.materials.size() : 0u; }
+
+const char* srg_obj_material_name(const srg_obj_model* m, uint32_t index)
+{
+    return (m && index < m->model.materials.size()) ? m->model.materials[index].name.c_str() : nullptr;
+}
+
+const char* srg_obj_material_diffuse(const srg_obj_model* m, uint32_t index)
+{
+    return (m && index < m->model.materials.size()) ? m->model.materials[index].diffuse.c_str() : nullptr;
+}
+
+void srg_obj_free(srg_obj_model* m) { delete m; }
 
 }  // extern "C"

@@ -85,16 +85,48 @@ class Scene:
         return sum(len(d.indices) // 3 for d in self.draws)
 
 
-def cube_grid_scene(seed=SEED_C1, gx=32, gz=32, width=1280, height=720, tex_size=512):
-    """1280x720, 32x32 unit cubes (12,288 tris), one 512^2 value-noise texture."""
+def cube_grid_mesh(seed=SEED_C1, gx=32, gz=32):
+    """The cube grid as generated: (vertices, u16 indices), front faces wound
+    as SoftRast keeps them."""
     lib = _load()
     nv = lib.srs_cube_grid(seed, gx, gz, None, None)
     v = np.empty(nv, VERTEX_DTYPE)
     i = np.empty(gx * gz * 36, np.uint16)
     lib.srs_cube_grid(seed, gx, gz, v.ctypes.data, i.ctypes.data)
+    return v, i
+
+
+def obj_text(vertices, indices):
+    """Wavefront OBJ text of one triangle mesh, triangles only and no "g"
+    lines: one v / vt / vn line per vertex (9 significant digits, which %f
+    reads back to the same float32) and faces "f a/a/a b/b/b c/c/c" wound
+    the other way round, so that loading with FlipWinding (as
+    Viewer/Main.cpp:30-35 loads its models) restores the winding."""
+    out = []
+    for p in vertices["pos"]:
+        out.append("v %.9g %.9g %.9g" % (p[0], p[1], p[2]))
+    for t in vertices["uv"]:
+        out.append("vt %.9g %.9g" % (t[0], t[1]))
+    for n in vertices["nrm"]:
+        out.append("vn %.9g %.9g %.9g" % (n[0], n[1], n[2]))
+    tri = indices.reshape(-1, 3).astype(np.int64) + 1
+    for a, b, c in tri:
+        out.append(f"f {a}/{a}/{a} {c}/{c}/{c} {b}/{b}/{b}")
+    return "\n".join(out) + "\n"
+
+
+def cube_grid_scene(seed=SEED_C1, gx=32, gz=32, width=1280, height=720, tex_size=512):
+    """1280x720, 32x32 unit cubes (12,288 tris), one 512^2 value-noise
+    texture. The mesh is emitted as OBJ text and loaded back with
+    FlipWinding through the library's OBJ reader, the path the reference's
+    viewer takes (so the reference can render the same file)."""
+    from . import obj_parse, OBJ_FLIP_WINDING
+    v, i = cube_grid_mesh(seed, gx, gz)
+    mesh = obj_parse(obj_text(v, i), OBJ_FLIP_WINDING).meshes[0]
     mvp = camera_mvp((0.0, 8.0, -20.0), (0.0, 0.0, 0.0), 60.0, width / height)
     tex = value_noise_rgba8(seed ^ 0x7E47, tex_size, tex_size)
-    return Scene(width, height, mvp, [Draw(v, i, 0)], [tex], {"config": "cube_grid"})
+    return Scene(width, height, mvp, [Draw(mesh.vertices, mesh.indices, 0)], [tex],
+                 {"config": "cube_grid", "source": "obj"})
 
 
 def spheres_scene(seed=SEED_C2, nspheres=4000, ndraws=8, width=1920, height=1080, tex_size=2048,
