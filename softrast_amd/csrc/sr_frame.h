@@ -147,6 +147,77 @@ __device__ __forceinline__ void wave_count(uint32_t* counters, uint32_t key, boo
     if (count) atomicAdd(&counters[key], count);
 }
 
+// Per-wave LDS hash for aggregating atomics by key, without a serial pass
+// per distinct key: kHashSlots keys, then their counts, then their bases.
+// Every slot is empty (key kHashEmpty, count 0) between uses. A wave has at
+// most 64 distinct keys, so linear probing in 128 slots always ends.
+constexpr uint32_t kHashSlots = 128, kHashWords = 3 * kHashSlots, kHashEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void wave_hash_init(uint32_t* t)
+{
+    for (uint32_t i = lane_id(); i < kHashSlots; i += 64) {
+        t[i] = kHashEmpty;
+        t[kHashSlots + i] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// Slot of `key` in the wave's table (claimed by compare-and-swap, probing
+// onward past other keys) and this lane's rank among the lanes adding to it.
+// Keys must differ from kHashEmpty. The probe count is bounded: a lane that
+// finds no slot (impossible while every pass empties the slots it used)
+// gets kHashSlots and adds on its own.
+__device__ __forceinline__ uint32_t wave_hash_insert(uint32_t* t, uint32_t key, uint32_t& rank)
+{
+    uint32_t h = (key * 0x9E3779B1u) >> 25;
+    for (uint32_t probe = 0; probe < kHashSlots; ++probe) {
+        const uint32_t prev = atomicCAS(&t[h], kHashEmpty, key);
+        if (prev == kHashEmpty || prev == key) {
+            rank = atomicAdd(&t[kHashSlots + h], 1u);
+            return h;
+        }
+        h = (h + 1u) & (kHashSlots - 1u);
+    }
+    rank = 0;
+    return kHashSlots;
+}
+
+// A wave's LDS operations complete in issue order, so once every lane's
+// increment is issued, the rank-0 lane of a key reads the key's full count.
+__device__ __forceinline__ uint32_t wave_add_slot_lds(uint32_t* t, uint32_t* counters, uint32_t key, bool active)
+{
+    uint32_t h = kHashSlots, rank = 0;
+    if (active) h = wave_hash_insert(t, key, rank);
+    const bool in_table = h < kHashSlots;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t own = 0;
+    if (active && !in_table) own = atomicAdd(&counters[key], 1u);
+    if (in_table && rank == 0) t[2 * kHashSlots + h] = atomicAdd(&counters[key], t[kHashSlots + h]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t slot = in_table ? t[2 * kHashSlots + h] + rank : own;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (in_table && rank == 0) {
+        t[h] = kHashEmpty;
+        t[kHashSlots + h] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    return slot;
+}
+
+__device__ __forceinline__ void wave_count_lds(uint32_t* t, uint32_t* counters, uint32_t key, bool active)
+{
+    uint32_t h = kHashSlots, rank = 0;
+    if (active) h = wave_hash_insert(t, key, rank);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (active && h == kHashSlots) atomicAdd(&counters[key], 1u);
+    if (h < kHashSlots && rank == 0) {
+        atomicAdd(&counters[key], t[kHashSlots + h]);
+        t[h] = kHashEmpty;
+        t[kHashSlots + h] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
 // Sum over the wave (result in every lane).
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {

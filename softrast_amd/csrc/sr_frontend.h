@@ -218,7 +218,7 @@ __device__ __forceinline__ uint32_t reference_tile_entries(const Prim& s)
 // its largest primitive has candidates.
 __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, bool active, const int32_t c[3],
                                           const int32_t dx[3], const int32_t dy[3], int32_t xmin, int32_t xmax,
-                                          int32_t ymin, int32_t ymax, bool emit, uint32_t rec)
+                                          int32_t ymin, int32_t ymax, bool emit, uint32_t rec, uint32_t* ht)
 {
     BinWalk w;
     if (active) {
@@ -271,9 +271,9 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
         }
         const uint32_t key = act ? bb + sby * vbx + sbx : 0u;
         if (!emit) {
-            wave_count(F.bin_count, key, act);
+            wave_count_lds(ht, F.bin_count, key, act);
         } else {
-            const uint32_t slot = wave_add_slot(F.bin_cursor, key, act);
+            const uint32_t slot = wave_add_slot_lds(ht, F.bin_cursor, key, act);
             if (act) F.entries[F.bin_start[key] + slot] = orec;
         }
     }
@@ -358,6 +358,9 @@ __device__ __forceinline__ void block_stats_flush(const FrameDev& F, BlockStats*
 __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
 {
     __shared__ BlockStats bs;
+    __shared__ uint32_t s_hash[kSetupThreads / 64 * kHashWords];
+    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kHashWords;
+    wave_hash_init(ht);
     const bool stats = (F.flags & FF_STATS) != 0;
     if (stats && threadIdx.x == 0) bs = BlockStats{ 0, 0 };
     if (stats) __syncthreads();
@@ -392,7 +395,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
     if (clipped) F.clip_queue[q] = g;  // at most one slot per triangle: never overflows
 
     if (stats) block_stats_add(&bs, survive ? reference_tile_entries(s) : 0u, survive ? 1u : 0u);
-    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0);
+    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0, ht);
     if (stats) {
         __syncthreads();
         block_stats_flush(F, &bs);
@@ -485,6 +488,9 @@ __device__ __forceinline__ uint32_t draw_of_triangle(const FrameDev& F, uint32_t
 __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
 {
     __shared__ float s_poly[2 * 9 * kClipComps * kClipThreads];
+    __shared__ uint32_t s_hash[kClipThreads / 64 * kHashWords];
+    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kHashWords;
+    wave_hash_init(ht);
     const uint32_t n = F.counters[C_CLIPQ];
     const uint32_t stride = gridDim.x * blockDim.x;
     const ClipPoly poly{ s_poly, threadIdx.x };
@@ -566,7 +572,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
                 if (lane == 0 && te) atomicAdd(&F.counters[C_TILE_ENTRIES], te);
                 if (lane == 0 && pr) atomicAdd(&F.counters[C_PRIMS], pr);
             }
-            walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0);
+            walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0, ht);
         }
         __syncthreads();  // before the polygons are overwritten
     }
@@ -730,7 +736,10 @@ __global__ __launch_bounds__(kScanThreads) void k_order(FrameDev F, const uint32
 
 __global__ __launch_bounds__(256) void k_emit(FrameDev F)
 {
+    __shared__ uint32_t s_hash[256 / 64 * kHashWords];
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
+    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kHashWords;
+    wave_hash_init(ht);
     const uint32_t nrec = F.ntris + F.counters[C_CLIPREC];
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base_r = blockIdx.x * blockDim.x; base_r < nrec; base_r += stride) {
@@ -750,7 +759,7 @@ __global__ __launch_bounds__(256) void k_emit(FrameDev F)
             xmin = q2.y & 0x3FFF; xmax = (q2.y >> 16) & 0x3FFF; ymin = q2.z & 0x3FFF; ymax = (q2.z >> 16) & 0x3FFF;
             view = F.draws[d].view;
         }
-        walk_bins(F, F.views[view], active, c, dx, dy, xmin, xmax, ymin, ymax, true, r);
+        walk_bins(F, F.views[view], active, c, dx, dy, xmin, xmax, ymin, ymax, true, r, ht);
     }
 }
 
