@@ -78,15 +78,25 @@ struct FrameDev {
     uint32_t* clip_queue;  // global triangle ids routed to the clipper
     uint32_t* clip_base;   // per global triangle: first fan record (relative to ntris)
     uint32_t* clip_order;  // per fan record: order key
-    uint32_t* bin_count;
+    uint32_t* bin_count;   // entries per bin (written by the scan)
     uint32_t* bin_start;   // total_bins + 1
-    uint32_t* bin_cursor;
+    uint32_t* part_count;  // kBinParts x total_bins: entries per (part, bin)
+    uint32_t* part_cursor; // kBinParts x total_bins: append cursors
+    uint32_t* part_off;    // kBinParts x total_bins: first entry of a part inside its bin's list
     uint32_t* entries;
     uint32_t* queue;
     uint64_t* bin_cost;    // optional per bin (s_memtime ticks)
     uint32_t* counters;
     uint32_t flags;
 };
+
+// A bin's counter and append cursor are split into kBinParts parts, the part
+// chosen by the entry's record (64 consecutive records share one), so the
+// atomics of a bin that many primitives land in spread over kBinParts
+// addresses. The scan sums the parts and lays them out one after another
+// inside the bin's entry list.
+constexpr uint32_t kBinParts = 8;
+__host__ __device__ constexpr uint32_t part_of(uint32_t rec) { return (rec >> 6) & (kBinParts - 1u); }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 

@@ -270,11 +270,12 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
             if (act && (pn >> 31) && !tile_accepts(oc, odx, ody, (sbx / K) * kBin, (sby / K) * kBin)) act = false;
         }
         const uint32_t key = act ? bb + sby * vbx + sbx : 0u;
+        const uint32_t pkey = part_of(orec) * F.total_bins + key;  // the (part, bin) counter
         if (!emit) {
-            wave_count_lds(ht, F.bin_count, key, act);
+            wave_count_lds(ht, F.part_count, pkey, act);
         } else {
-            const uint32_t slot = wave_add_slot_lds(ht, F.bin_cursor, key, act);
-            if (act) F.entries[F.bin_start[key] + slot] = orec;
+            const uint32_t slot = wave_add_slot_lds(ht, F.part_cursor, pkey, act);
+            if (act) F.entries[F.bin_start[key] + F.part_off[pkey] + slot] = orec;
         }
     }
 }
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
     if (clipped) F.clip_queue[q] = g;  // at most one slot per triangle: never overflows
 
     if (stats) block_stats_add(&bs, survive ? reference_tile_entries(s) : 0u, survive ? 1u : 0u);
-    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0, ht);
+    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, g, ht);
     if (stats) {
         __syncthreads();
         block_stats_flush(F, &bs);
@@ -548,6 +549,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
             const ViewDev& V = F.views[D.view];
             Prim s;
             bool ok = false;
+            uint32_t r = 0;
             if (t < total) {
                 const ClipPoly owner{ poly.lds, (uint32_t)src };
                 const float4 fv[3] = { owner.pos(opin, 0), owner.pos(opin, j + 1), owner.pos(opin, j + 2) };
@@ -557,7 +559,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
                     fa[1][a] = owner.at(opin, j + 1, 4 + a);
                     fa[2][a] = owner.at(opin, j + 2, 4 + a);
                 }
-                const uint32_t r = F.ntris + obase + j;
+                r = F.ntris + obase + j;
                 ok = setup_prim(s, fv, fa, D.nslots, V.width, V.height);
                 if (ok) {
                     write_record(F, r, s, od);
@@ -572,7 +574,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
                 if (lane == 0 && te) atomicAdd(&F.counters[C_TILE_ENTRIES], te);
                 if (lane == 0 && pr) atomicAdd(&F.counters[C_PRIMS], pr);
             }
-            walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, 0, ht);
+            walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, r, ht);
         }
         __syncthreads();  // before the polygons are overwritten
     }
@@ -637,7 +639,8 @@ __device__ __forceinline__ void scan_inputs(const FrameDev& F, uint32_t t, uint3
     c = 0;
     q = 0;
     if (t < F.total_bins) {
-        c = F.bin_count[t];
+#pragma unroll
+        for (uint32_t k = 0; k < kBinParts; ++k) c += F.part_count[k * F.total_bins + t];
         q = (c || view_cleared(F, t)) ? 1u : 0u;
     }
 }
@@ -692,6 +695,13 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(FrameDev F, const u
     if (t < F.total_bins) {
         F.bin_start[t] = blk[2 * blockIdx.x] + c;
         if (enq) F.queue[blk[2 * blockIdx.x + 1] + q] = t;
+        uint32_t off = 0;  // the bin's parts, one after another
+#pragma unroll
+        for (uint32_t k = 0; k < kBinParts; ++k) {
+            F.part_off[k * F.total_bins + t] = off;
+            off += F.part_count[k * F.total_bins + t];
+        }
+        F.bin_count[t] = off;
     }
 }
 

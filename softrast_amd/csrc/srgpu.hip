@@ -33,16 +33,17 @@
 
 namespace srg {
 
-// Frame-start fill: counters and per-bin counts / cursors / costs (graph-safe).
-__global__ void k_frame_init(uint32_t* counters, uint32_t* bin_count, uint32_t* bin_cursor, uint64_t* bin_cost,
+// Frame-start fill: counters, per-(part, bin) counts and cursors, per-bin
+// costs (graph-safe).
+__global__ void k_frame_init(uint32_t* counters, uint32_t* part_count, uint32_t* part_cursor, uint64_t* bin_cost,
                              uint32_t nbins)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < C_NCOUNTERS) counters[i] = 0;
-    for (uint32_t t = i; t < nbins; t += gridDim.x * blockDim.x) {
-        bin_count[t] = 0;
-        bin_cursor[t] = 0;
-        bin_cost[t] = 0;
+    for (uint32_t t = i; t < nbins * kBinParts; t += gridDim.x * blockDim.x) {
+        part_count[t] = 0;
+        part_cursor[t] = 0;
+        if (t < nbins) bin_cost[t] = 0;
     }
 }
 
@@ -193,7 +194,7 @@ struct srg_ctx {
     DevArray<uint4> rec_raster;
     DevArray<float4> rec_shade;
     DevArray<uint32_t> rec_meta, clip_queue, clip_base, clip_order;
-    DevArray<uint32_t> bin_count, bin_start, bin_cursor, queue, queue2, entries, scan_blk;
+    DevArray<uint32_t> bin_count, bin_start, part_count, part_cursor, part_off, queue, queue2, entries, scan_blk;
     DevArray<uint64_t> bin_cost;
     DevArray<uint32_t> counters;
     DevArray<uint32_t> blit_staging;
@@ -536,7 +537,7 @@ int run_frame(srg_ctx* ctx)
         }
         total_bins += (uint64_t)v.bins_x * v.bins_y;
     }
-    if (total_bins >= 0xFFFFFFF0ull) return SRG_ERR_INVALID;
+    if (total_bins * kBinParts >= 0xFFFFFFF0ull) return SRG_ERR_INVALID;  // (part, bin) indices are 32-bit
 
     // ---- draws, textures, front-end blocks ----
     std::vector<DrawDev> ddev(ctx->draws.size());
@@ -606,7 +607,9 @@ int run_frame(srg_ctx* ctx)
 
     // ---- arenas ----
     const uint32_t TB = (uint32_t)std::max<uint64_t>(total_bins, 1);
-    if ((st = ctx->bin_count.ensure(TB)) || (st = ctx->bin_start.ensure(TB + 1)) || (st = ctx->bin_cursor.ensure(TB)) ||
+    if ((st = ctx->bin_count.ensure(TB)) || (st = ctx->bin_start.ensure(TB + 1)) ||
+        (st = ctx->part_count.ensure((size_t)kBinParts * TB)) || (st = ctx->part_cursor.ensure((size_t)kBinParts * TB)) ||
+        (st = ctx->part_off.ensure((size_t)kBinParts * TB)) ||
         (st = ctx->queue.ensure(TB)) || (st = ctx->queue2.ensure(TB)) ||
         (st = ctx->scan_blk.ensure(2 * ((TB + kScanThreads - 1) / kScanThreads))) || (st = ctx->bin_cost.ensure(TB)) || (st = ctx->counters.ensure(C_NCOUNTERS)) ||
         (st = ctx->clip_queue.ensure(std::max<uint32_t>(NT, 1))) || (st = ctx->clip_base.ensure(std::max<uint32_t>(NT, 1))))
@@ -650,7 +653,9 @@ int run_frame(srg_ctx* ctx)
         F.clip_order = ctx->clip_order.ptr;
         F.bin_count = ctx->bin_count.ptr;
         F.bin_start = ctx->bin_start.ptr;
-        F.bin_cursor = ctx->bin_cursor.ptr;
+        F.part_count = ctx->part_count.ptr;
+        F.part_cursor = ctx->part_cursor.ptr;
+        F.part_off = ctx->part_off.ptr;
         F.entries = ctx->entries.ptr;
         F.queue = ctx->queue.ptr;
         F.bin_cost = ctx->bin_cost.ptr;
@@ -661,7 +666,7 @@ int run_frame(srg_ctx* ctx)
 
         const uint32_t emit_blocks = std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 16384));
         const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + kClipThreads - 1) / kClipThreads, 2048));
-        const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((TB + 255) / 256, 1024));
+        const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((kBinParts * TB + 255) / 256, 1024));
         const uint32_t scan_blocks = (TB + kScanThreads - 1) / kScanThreads;
         const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
         // while the autotuner sweeps, the learned order falls back to entry
@@ -683,7 +688,7 @@ int run_frame(srg_ctx* ctx)
         auto enqueue = [&](bool split_events) -> int {
             HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
             hipLaunchKernelGGL(k_frame_init, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->counters.ptr,
-                               ctx->bin_count.ptr, ctx->bin_cursor.ptr, ctx->bin_cost.ptr, TB);
+                               ctx->part_count.ptr, ctx->part_cursor.ptr, ctx->bin_cost.ptr, TB);
             if (ctx->cfg.flags & SRG_CFG_KEEP_PRIMID)
                 for (auto& v : vdev)
                     if (v.primid)
@@ -882,7 +887,9 @@ void srg_destroy(srg_ctx* ctx)
     ctx->clip_order.release();
     ctx->bin_count.release();
     ctx->bin_start.release();
-    ctx->bin_cursor.release();
+    ctx->part_count.release();
+    ctx->part_cursor.release();
+    ctx->part_off.release();
     ctx->queue.release();
     ctx->queue2.release();
     ctx->scan_blk.release();
