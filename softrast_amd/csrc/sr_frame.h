@@ -95,7 +95,7 @@ struct FrameDev {
 // atomics of a bin that many primitives land in spread over kBinParts
 // addresses. The scan sums the parts and lays them out one after another
 // inside the bin's entry list.
-constexpr uint32_t kBinParts = 8;
+constexpr uint32_t kBinParts = 16;
 __host__ __device__ constexpr uint32_t part_of(uint32_t rec) { return (rec >> 6) & (kBinParts - 1u); }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
