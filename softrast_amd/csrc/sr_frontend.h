@@ -705,12 +705,17 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(FrameDev F, const u
     }
 }
 
-// Longest-first ordering of the back end's queue (one workgroup): bins are
-// bucketed by a log-scaled cost key (16 buckets per octave) -- the entry
-// count, or the learned model's predicted cost -- and written out from the
-// largest bucket down. The order inside a bucket is arbitrary; the image does
-// not depend on the order bins are processed in.
+// Longest-first ordering of the back end's queue: bins are bucketed by a
+// log-scaled cost key (16 buckets per octave) -- the entry count, or the
+// learned model's predicted cost -- and written out from the largest bucket
+// down. The order inside a bucket is arbitrary; the image does not depend on
+// the order bins are processed in. Three passes, so that a queue of any
+// length is ordered by many workgroups: per block of kOrderBlock queue
+// entries a bucket histogram (k_order_hist), per bucket the offset of every
+// block's run (k_order_offsets, buckets descending, blocks ascending inside a
+// bucket), and the scatter (k_order_scatter).
 constexpr uint32_t kOrderBuckets = 512;
+constexpr uint32_t kOrderBlock = 1024;
 
 __device__ __forceinline__ uint32_t cost_bucket(float c)
 {
@@ -719,29 +724,63 @@ __device__ __forceinline__ uint32_t cost_bucket(float c)
     return b >= (float)(kOrderBuckets - 1) ? kOrderBuckets - 1 : 1u + (uint32_t)b;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_order(FrameDev F, const uint32_t* in_queue, const float* pred)
+__global__ __launch_bounds__(kOrderBlock) void k_order_hist(FrameDev F, const uint32_t* in_queue, const float* pred,
+                                                           uint16_t* bkt, uint32_t* hist)
 {
-    __shared__ uint32_t hist[kOrderBuckets];
-    const uint32_t n = F.counters[C_QLEN];
-    for (uint32_t i = threadIdx.x; i < kOrderBuckets; i += kScanThreads) hist[i] = 0;
+    __shared__ uint32_t h[kOrderBuckets];
+    const uint32_t n = F.counters[C_QLEN], base = blockIdx.x * kOrderBlock;
+    if (base >= n) return;  // the whole workgroup
+    if (threadIdx.x < kOrderBuckets) h[threadIdx.x] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
+    const uint32_t i = base + threadIdx.x;
+    if (i < n) {
         const uint32_t gb = in_queue[i];
-        atomicAdd(&hist[cost_bucket(pred ? pred[gb] : (float)F.bin_count[gb])], 1u);
+        const uint32_t b = cost_bucket(pred ? pred[gb] : (float)F.bin_count[gb]);
+        bkt[i] = (uint16_t)b;
+        atomicAdd(&h[b], 1u);
     }
     __syncthreads();
-    {  // descending exclusive scan: largest bucket first
-        const uint32_t bkt = kOrderBuckets - 1u - threadIdx.x;
-        uint32_t c = threadIdx.x < kOrderBuckets ? hist[bkt] : 0u, unused = 0u, tc, tu;
-        block_scan2(c, unused, tc, tu);
-        if (threadIdx.x < kOrderBuckets) hist[bkt] = c;
+    if (threadIdx.x < kOrderBuckets) hist[blockIdx.x * kOrderBuckets + threadIdx.x] = h[threadIdx.x];
+}
+
+// One workgroup, a thread per bucket: bucket totals over the blocks, their
+// exclusive scan from the largest bucket down, then each block's offset.
+__global__ __launch_bounds__(kOrderBuckets) void k_order_offsets(FrameDev F, uint32_t* hist)
+{
+    __shared__ uint32_t s_w[kOrderBuckets / 64];
+    const uint32_t n = F.counters[C_QLEN], nblk = (n + kOrderBlock - 1) / kOrderBlock;
+    const uint32_t b = kOrderBuckets - 1u - threadIdx.x;  // thread 0 holds the largest bucket
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < nblk; ++k) total += hist[k * kOrderBuckets + b];
+    // exclusive scan of the totals over the threads (waves of 64, then the wave sums)
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t incl = total;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
     }
+    if (lane == 63) s_w[w] = incl;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += kScanThreads) {
-        const uint32_t gb = in_queue[i];
-        const uint32_t pos = atomicAdd(&hist[cost_bucket(pred ? pred[gb] : (float)F.bin_count[gb])], 1u);
-        F.queue[pos] = gb;
+    uint32_t run = incl - total;
+    for (uint32_t k = 0; k < w; ++k) run += s_w[k];
+    for (uint32_t k = 0; k < nblk; ++k) {
+        const uint32_t c = hist[k * kOrderBuckets + b];
+        hist[k * kOrderBuckets + b] = run;
+        run += c;
     }
+}
+
+__global__ __launch_bounds__(kOrderBlock) void k_order_scatter(FrameDev F, const uint32_t* in_queue,
+                                                              const uint16_t* bkt, const uint32_t* hist)
+{
+    __shared__ uint32_t off[kOrderBuckets];
+    const uint32_t n = F.counters[C_QLEN], base = blockIdx.x * kOrderBlock;
+    if (base >= n) return;  // the whole workgroup
+    if (threadIdx.x < kOrderBuckets) off[threadIdx.x] = hist[blockIdx.x * kOrderBuckets + threadIdx.x];
+    __syncthreads();
+    const uint32_t i = base + threadIdx.x;
+    if (i < n) F.queue[atomicAdd(&off[bkt[i]], 1u)] = in_queue[i];
 }
 
 __global__ __launch_bounds__(256) void k_emit(FrameDev F)

@@ -195,6 +195,8 @@ struct srg_ctx {
     DevArray<float4> rec_shade;
     DevArray<uint32_t> rec_meta, clip_queue, clip_base, clip_order;
     DevArray<uint32_t> bin_count, bin_start, part_count, part_cursor, part_off, queue, queue2, entries, scan_blk;
+    DevArray<uint16_t> order_bkt;   // per queue entry: its cost bucket
+    DevArray<uint32_t> order_hist;  // per (queue block, bucket): count, then offset
     DevArray<uint64_t> bin_cost;
     DevArray<uint32_t> counters;
     DevArray<uint32_t> blit_staging;
@@ -611,6 +613,8 @@ int run_frame(srg_ctx* ctx)
         (st = ctx->part_count.ensure((size_t)kBinParts * TB)) || (st = ctx->part_cursor.ensure((size_t)kBinParts * TB)) ||
         (st = ctx->part_off.ensure((size_t)kBinParts * TB)) ||
         (st = ctx->queue.ensure(TB)) || (st = ctx->queue2.ensure(TB)) ||
+        (st = ctx->order_bkt.ensure(TB)) ||
+        (st = ctx->order_hist.ensure((size_t)((TB + kOrderBlock - 1) / kOrderBlock) * kOrderBuckets)) ||
         (st = ctx->scan_blk.ensure(2 * ((TB + kScanThreads - 1) / kScanThreads))) || (st = ctx->bin_cost.ensure(TB)) || (st = ctx->counters.ensure(C_NCOUNTERS)) ||
         (st = ctx->clip_queue.ensure(std::max<uint32_t>(NT, 1))) || (st = ctx->clip_base.ensure(std::max<uint32_t>(NT, 1))))
         return st;
@@ -704,15 +708,21 @@ int run_frame(srg_ctx* ctx)
                                (const uint32_t*)ctx->scan_blk.ptr);
             hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
             if (learned) {
-                hipLaunchKernelGGL(k_bin_predict, dim3(std::min<uint32_t>((TB + 3) / 4, 4u * (uint32_t)ctx->num_cus)),
+                hipLaunchKernelGGL(k_bin_predict, dim3(std::min<uint32_t>((TB + 3) / 4, 16384u)),
                                    dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr,
                                    (const MlpParams*)(ctx->sched_trained ? ctx->sched_params : nullptr),
                                    ctx->sched_pred.ptr);
             }
-            if (ordered)
-                hipLaunchKernelGGL(k_order, dim3(1), dim3(kScanThreads), 0, ctx->stream, F,
-                                   (const uint32_t*)ctx->queue2.ptr,
-                                   (const float*)(learned && use_model ? ctx->sched_pred.ptr : nullptr));
+            if (ordered) {
+                const float* pred = (const float*)(learned && use_model ? ctx->sched_pred.ptr : nullptr);
+                const uint32_t oblk = (TB + kOrderBlock - 1) / kOrderBlock;
+                hipLaunchKernelGGL(k_order_hist, dim3(oblk), dim3(kOrderBlock), 0, ctx->stream, F,
+                                   (const uint32_t*)ctx->queue2.ptr, pred, ctx->order_bkt.ptr, ctx->order_hist.ptr);
+                hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(kOrderBuckets), 0, ctx->stream, F, ctx->order_hist.ptr);
+                hipLaunchKernelGGL(k_order_scatter, dim3(oblk), dim3(kOrderBlock), 0, ctx->stream, F,
+                                   (const uint32_t*)ctx->queue2.ptr, (const uint16_t*)ctx->order_bkt.ptr,
+                                   (const uint32_t*)ctx->order_hist.ptr);
+            }
             if (split_events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
             if (const int lst = launch_backend(ctx, L, F)) return lst;
             if (learned)
@@ -892,6 +902,8 @@ void srg_destroy(srg_ctx* ctx)
     ctx->part_off.release();
     ctx->queue.release();
     ctx->queue2.release();
+    ctx->order_bkt.release();
+    ctx->order_hist.release();
     ctx->scan_blk.release();
     ctx->entries.release();
     ctx->bin_cost.release();
