@@ -8,7 +8,7 @@ reverse-Z. Vertex, index and texture data are resident in HBM before timing
 starts. Two paths are timed:
 
   plain    one view per EndFrame: clear + 8 DrawIndexed + synchronous EndFrame
-  batched  --views V (default 32) views of the same scene (cameras turned by
+  batched  --views V (default 64) views of the same scene (cameras turned by
            up to +-2 degrees, one framebuffer each) in one EndFrame, launched as a replayed
            hipGraph, work-bin layout chosen by the library's autotuner, tile
            queue ordered by --tile-order (default: the learned cost model,
@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--views", type=int, default=32, help="views per EndFrame in the batched path")
+    ap.add_argument("--views", type=int, default=64, help="views per EndFrame in the batched path")
     ap.add_argument("--tile-order", type=int, default=2, help="batched path: 0 natural, 1 tri-count, 2 learned")
     ap.add_argument("--no-autotune", action="store_true", help="batched path: library default layout")
     ap.add_argument("--bin-size", type=int, default=0, help="work-bin edge 16/32/64 (0 = library default)")

@@ -227,7 +227,7 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
 // block no pixel with any negative.
 template <uint32_t S, bool STATS>
 __device__ __forceinline__ void raster_wave(unsigned long long* keys, const EntryRaster& e, bool has, uint32_t sx0,
-                                            uint32_t sy0, unsigned long long& frags)
+                                            uint32_t sy0, unsigned long long& frags, uint32_t* mark)
 {
     const uint32_t lane = lane_id();
     const uint32_t cw = e.cx1 > e.cx0 ? e.cx1 - e.cx0 : 0u;
@@ -245,20 +245,12 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     const uint32_t pk_div = e.by0 | (magic << 7);
     const uint32_t first = incl - n;
     const uint32_t low = (uint32_t)e.low;
+    uint32_t carry = 0;  // 1 + owner of the previous pass's last item
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t it = base + lane;
         const bool valid = it < total;
-        // owner: the first lane whose inclusive count exceeds it (it < total =
-        // incl of lane 63, so the answer is in [0, 63]; 6 halvings of 64 candidates)
-        uint32_t lo = 0, hi = 63;
-#pragma unroll
-        for (int step = 0; step < 6; ++step) {
-            const uint32_t mid = (lo + hi) >> 1;
-            const uint32_t v = __shfl(incl, (int)mid, 64);
-            if (v > it) hi = mid;
-            else lo = mid + 1;
-        }
-        const int src = (int)(valid ? lo : 0u);
+        const uint32_t m = wave_owner(mark, n, first, base, carry);
+        const int src = (int)(valid && m ? min(m - 1u, 63u) : 0u);
         uint32_t ct[3], dxu[3], dyu[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -327,6 +319,7 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
     static_assert(NPIX % T == 0 && PPT % BATCH == 0, "strip must hold a whole number of pixels per thread");
     __shared__ unsigned long long keys[NPIX];
     __shared__ uint32_t s_q;
+    __shared__ uint32_t s_mark[W * 64];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
     unsigned long long frags = 0;
@@ -371,7 +364,7 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
                 order = r < F.ntris ? ((r << 3) | kFanUnclipped) : F.clip_order[r - F.ntris];
             }
             const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S, H);
-            raster_wave<S, STATS>(keys, e, has, sx0, sy0, frags);
+            raster_wave<S, STATS>(keys, e, has, sx0, sy0, frags, s_mark + wave * 64u);
         }
         __syncthreads();
 

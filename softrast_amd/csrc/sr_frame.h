@@ -162,6 +162,8 @@ __device__ __forceinline__ void wave_count(uint32_t* counters, uint32_t key, boo
 // Every slot is empty (key kHashEmpty, count 0) between uses. A wave has at
 // most 64 distinct keys, so linear probing in 128 slots always ends.
 constexpr uint32_t kHashSlots = 128, kHashWords = 3 * kHashSlots, kHashEmpty = 0xFFFFFFFFu;
+// A wave's LDS scratch in the front end: the hash, then 64 owner marks.
+constexpr uint32_t kWaveScratch = kHashWords + 64;
 
 __device__ __forceinline__ void wave_hash_init(uint32_t* t)
 {
@@ -226,6 +228,39 @@ __device__ __forceinline__ void wave_count_lds(uint32_t* t, uint32_t* counters, 
         t[kHashSlots + h] = 0;
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// Inclusive max over the wave's lanes (DPP: row shifts, then row broadcasts).
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v)
+{
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));  // row_shr:1
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));  // row_shr:2
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));  // row_shr:4
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));  // row_shr:8
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));  // row_bcast:15
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));  // row_bcast:31
+    return v;
+}
+
+// Owner of work item `base + lane` when every lane owns `n` consecutive
+// items starting at `first` (an exclusive prefix of the n): 1 + the owning
+// lane, or 0 past the last item. Lanes whose items start in this pass of 64
+// mark their start in the wave's 64-word LDS array; an inclusive max over
+// the marks and `carry` (the value of the previous pass's last item, 0 on
+// the first pass) names each item's owner. Passes must run in order with
+// base = 0, 64, 128, ...; uniform control flow required.
+__device__ __forceinline__ uint32_t wave_owner(uint32_t* mark, uint32_t n, uint32_t first, uint32_t base,
+                                               uint32_t& carry)
+{
+    const uint32_t lane = lane_id();
+    mark[lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (n && first >= base && first - base < 64u) mark[first - base] = lane + 1u;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t m = max(wave_incl_max(mark[lane]), carry);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    carry = readlane_u32(m, 63);
+    return m;
 }
 
 // Sum over the wave (result in every lane).

@@ -241,17 +241,12 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
     const uint32_t pk_nx = max(w.nx, 1u) | ((w.check ? 1u : 0u) << 31);  // nx < 2^31
     const bool any_check = __ballot(w.check && w.ncand != 0) != 0ull;
     const uint32_t K = F.bin_k;
+    uint32_t carry = 0;
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t it = base + lane;
         const bool valid = it < total;
-        uint32_t lo = 0, hi = 63;
-#pragma unroll
-        for (int step = 0; step < 6; ++step) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (__shfl(incl, (int)mid, 64) > it) hi = mid;
-            else lo = mid + 1;
-        }
-        const int src = (int)(valid ? lo : 0u);
+        const uint32_t m = wave_owner(ht + kHashWords, w.ncand, first, base, carry);
+        const int src = (int)(valid && m ? min(m - 1u, 63u) : 0u);
         const uint32_t org = __shfl(pk_org, src, 64), pn = __shfl(pk_nx, src, 64);
         const uint32_t i = it - __shfl(first, src, 64);
         const uint32_t bb = __shfl(w.bin_base, src, 64), vbx = __shfl(w.view_bins_x, src, 64);
@@ -359,8 +354,8 @@ __device__ __forceinline__ void block_stats_flush(const FrameDev& F, BlockStats*
 __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
 {
     __shared__ BlockStats bs;
-    __shared__ uint32_t s_hash[kSetupThreads / 64 * kHashWords];
-    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kHashWords;
+    __shared__ uint32_t s_hash[kSetupThreads / 64 * kWaveScratch];
+    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kWaveScratch;
     wave_hash_init(ht);
     const bool stats = (F.flags & FF_STATS) != 0;
     if (stats && threadIdx.x == 0) bs = BlockStats{ 0, 0 };
@@ -489,8 +484,8 @@ __device__ __forceinline__ uint32_t draw_of_triangle(const FrameDev& F, uint32_t
 __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
 {
     __shared__ float s_poly[2 * 9 * kClipComps * kClipThreads];
-    __shared__ uint32_t s_hash[kClipThreads / 64 * kHashWords];
-    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kHashWords;
+    __shared__ uint32_t s_hash[kClipThreads / 64 * kWaveScratch];
+    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kWaveScratch;
     wave_hash_init(ht);
     const uint32_t n = F.counters[C_CLIPQ];
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -785,9 +780,9 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(FrameDev F, const
 
 __global__ __launch_bounds__(256) void k_emit(FrameDev F)
 {
-    __shared__ uint32_t s_hash[256 / 64 * kHashWords];
+    __shared__ uint32_t s_hash[256 / 64 * kWaveScratch];
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
-    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kHashWords;
+    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kWaveScratch;
     wave_hash_init(ht);
     const uint32_t nrec = F.ntris + F.counters[C_CLIPREC];
     const uint32_t stride = gridDim.x * blockDim.x;
