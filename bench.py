@@ -218,6 +218,13 @@ def sched_study(sr, args):
         print(f"[bench] {name}: {out['orders'][name]}", file=sys.stderr, flush=True)
         fb.destroy()
         ctx.Shutdown()
+    if not args.no_cpu_baseline:
+        view = scenes.Scene(scene.width, scene.height, scene.mvp, scene.draws, [])
+        out["cpu_baseline"] = cpu_baseline_views([view], host_textures(sr, scene.textures), args, budget_s=15.0,
+                                                 what=": the same 4K frame")
+        best = max(v["frames_per_s"] for v in out["orders"].values())
+        out["speedup_vs_cpu"] = round(best / out["cpu_baseline"]["value"], 1)
+        print(f"[bench] cpu baseline: {out['cpu_baseline']}", file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 
@@ -241,6 +248,50 @@ def viewset_frame(sr, ctx, fbs, draws):
         ctx.ClearFrameBuffer(fb, 0)
     ctx.DrawIndexedList(draws)
     ctx.EndFrame()
+
+
+def host_textures(sr, images):
+    """TextureData (tiled, with mips) of each image, built on the GPU and read
+    back: the same bytes as the host build, for the CPU baseline."""
+    ctx = sr.RenderContext()
+    out = [ctx.CreateTexture(t, t.shape[1], t.shape[0], True).read() for t in images]
+    ctx.Shutdown()
+    return out
+
+
+def cpu_baseline_views(views, texdata, args, budget_s=20.0, min_views=2, what=""):
+    """The CPU oracle (-O3 AVX2/FMA build, the reference's task model) on the
+    host cores, one view per frame (clear + render), views taken evenly
+    spaced from `views` (cycling for fewer views than frames) until about
+    budget_s of rendering: a bounded sample of the workload."""
+    from oracle.oracle import Oracle, Plane, make_texture
+    nthreads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    orc = Oracle(nthreads=nthreads, fast=True)
+    stex = [make_texture(t) for t in texdata]
+    n = len(views)
+    stride = max(1, n // 8)
+    order = [i for off in range(stride) for i in range(off, n, stride)]  # every stride-th view first
+    times, done, k = [], [], 0
+    while k < min_views or (sum(times) < budget_s and k < max(n, min_views) * 4):
+        vi = order[k % n]
+        v = views[vi]
+        plane = Plane(v.width, v.height)
+        draws = []
+        for d in v.draws:
+            vf = d.vertices.view(np.float32).reshape(-1, 8)
+            draws.append(dict(shader=d.shader, tex=stex[d.texture][0] if d.texture >= 0 else None, idx=d.indices,
+                              idx_stride=d.indices.dtype.itemsize, pos=vf, pos_stride=32, pos_num=len(vf), attr=vf,
+                              attr_stride=32, attr_num=len(vf), uv_offset=d.uv_offset, plane=0, mvp=v.mvp))
+        t = time.perf_counter()
+        orc.clear(plane, 0)
+        orc.render(draws, [plane])
+        times.append(time.perf_counter() - t)
+        done.append(vi)
+        k += 1
+    orc.close()
+    return {"value": round(len(times) / sum(times), 3), "unit": "views/s", "cores": nthreads, "kind": "port",
+            "sample": f"{len(times)} view renders ({len(set(done))} distinct views{what}), clear + render each, "
+                      f"{sum(times):.1f} s, oracle -O3 AVX2/FMA build, reference task model"}
 
 
 def batch_workload(sr, args):
@@ -338,6 +389,11 @@ def batch_workload(sr, args):
         fb.destroy()
     sctx.Shutdown()
     value = out["runs"][best]["views_per_s"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_views(vs.views, host_textures(sr, vs.textures), args,
+                                                 what=f" of {len(vs.views)}, evenly spaced")
+        out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        print(f"[bench] cpu baseline: {out['cpu_baseline']}", file=sys.stderr, flush=True)
     batches = value / len(vs.views)
     algo = sum(d.indices.dtype.itemsize * len(d.indices) + 32 * len(d.vertices) for v in vs.views for d in v.draws)
     algo += sum(8 * v.width * v.height for v in vs.views) + 16 * covered
