@@ -96,8 +96,17 @@ __device__ __forceinline__ uint32_t shade_pixel(const FrameDev& F, uint32_t rec,
     return sample_unlit(T, var[0], var[1], der[0], der[1], der[2], der[3]);
 }
 
+// The view a bin belongs to: the last view whose first bin is <= gb. gb must
+// be the same in every lane of the wave: with at most 64 views each lane
+// reads one view's first bin and a ballot counts them (one load instead of
+// a chain of dependent loads); more views are binary-searched.
 __device__ __forceinline__ uint32_t view_of_bin(const FrameDev& F, uint32_t gb)
 {
+    if (F.nviews <= 64u) {
+        const uint32_t lane = lane_id();
+        const bool le = lane < F.nviews && F.views[lane].bin_base <= gb;
+        return (uint32_t)__popcll(__ballot(le)) - 1u;
+    }
     uint32_t lo = 0, hi = F.nviews;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;

@@ -746,6 +746,7 @@ __global__ __launch_bounds__(kOrderBuckets) void k_order_offsets(FrameDev F, uin
     const uint32_t n = F.counters[C_QLEN], nblk = (n + kOrderBlock - 1) / kOrderBlock;
     const uint32_t b = kOrderBuckets - 1u - threadIdx.x;  // thread 0 holds the largest bucket
     uint32_t total = 0;
+#pragma unroll 8
     for (uint32_t k = 0; k < nblk; ++k) total += hist[k * kOrderBuckets + b];
     // exclusive scan of the totals over the threads (waves of 64, then the wave sums)
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -759,6 +760,7 @@ __global__ __launch_bounds__(kOrderBuckets) void k_order_offsets(FrameDev F, uin
     __syncthreads();
     uint32_t run = incl - total;
     for (uint32_t k = 0; k < w; ++k) run += s_w[k];
+#pragma unroll 8
     for (uint32_t k = 0; k < nblk; ++k) {
         const uint32_t c = hist[k * kOrderBuckets + b];
         hist[k * kOrderBuckets + b] = run;
