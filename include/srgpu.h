@@ -43,8 +43,23 @@ enum srg_status {
 enum srg_shader {
     SRG_SHADER_UNLIT_DIFFUSE = 0,    /* Shaders.h:71-104: texture sample, white without texture */
     SRG_SHADER_VISUALIZE_NORMALS = 1,/* Shaders.h:106-121 */
-    SRG_SHADER_VISUALIZE_UVS = 2     /* Shaders.h:123-130 */
+    SRG_SHADER_VISUALIZE_UVS = 2,    /* Shaders.h:123-130 */
+    SRG_SHADER_SPONZA = 3            /* SponzaScene.cpp:13-104: sun + 16 point lights + ambient,
+                                        times the texture sample; white without texture. Reads all
+                                        eight varyings: attribute_stride must be 32 (Obj::Vertex) */
 };
+
+/* SponzaScene::Constants (Viewer/SponzaScene.h:15-47), the global the
+ * SponzaShader reads (SponzaScene.cpp:11): each m_sunDir / m_ambCol lane
+ * broadcast as one float, 16 PointLights (falloff is carried, not read). */
+#define SRG_SPONZA_LIGHTS 16
+typedef struct srg_sponza_light {
+    float pos[3], colour[3], intensity, falloff;
+} srg_sponza_light;
+typedef struct srg_sponza_constants {
+    float sun_dir[3], ambient[3];
+    srg_sponza_light lights[SRG_SPONZA_LIGHTS];
+} srg_sponza_constants;
 
 /* Back-end tile-queue ordering. Every policy yields the same image. */
 enum srg_tile_order {
@@ -218,6 +233,10 @@ int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count);
  * No counterpart in the reference, whose bin size is a compile-time constant
  * (Config.h:18-21) and whose tiles each take one task (Renderer.cpp:264-306). */
 int srg_get_tuning(srg_ctx* ctx, srg_tuning* out);
+/* Sets the SponzaShader constants for frames ended after the call (the
+ * reference writes the g_constants global in SponzaScene::Init / Update,
+ * SponzaScene.cpp:127-186, before issuing its draws). Zero until set. */
+int srg_set_sponza_constants(srg_ctx* ctx, const srg_sponza_constants* k);
 
 /* ---- scene ingest: Obj::Model::Load (Viewer/Obj.cpp:374-567) ----
  * Host code (no device needed). Reads a Wavefront OBJ as the reference's

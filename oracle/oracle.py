@@ -71,6 +71,9 @@ def load(fast=False):
     lib.sro_blit.restype = None
     lib.sro_rcpps.argtypes = [f]
     lib.sro_rcpps.restype = f
+    lib.sro_rsqrtps.argtypes = [f]
+    lib.sro_rsqrtps.restype = f
+    lib.sro_set_sponza_constants.argtypes = [vp, vp]
     lib.sro_pack_rgba.argtypes = [f, f, f, f]
     lib.sro_pack_rgba.restype = u32
     lib.sro_morton5.argtypes = [u32, u32]
@@ -168,6 +171,12 @@ class Oracle:
     def __init__(self, nthreads=1, fast=False):
         self.lib = load(fast)
         self.ctx = self.lib.sro_create(nthreads)
+
+    def set_sponza_constants(self, consts):
+        """consts: bytes-like sro_sponza_constants (6 + 16 * 8 float32)."""
+        buf = np.ascontiguousarray(np.frombuffer(bytes(consts), np.float32))
+        assert buf.size == 6 + 16 * 8
+        self.lib.sro_set_sponza_constants(self.ctx, buf.ctypes.data)
 
     def clear(self, plane, colour=0, clear_colour=True, clear_depth=True):
         p = plane.struct()

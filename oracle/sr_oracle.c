@@ -20,6 +20,8 @@
  *     coverage, depth)                                  closed-form predicate; depth vs float64 plane
  *   interpolants           Rasterizer.cpp:306-419        closed form on an affine (w = 1) triangle
  *   rcpps                  Rasterizer.cpp:375-376        recorded host table (tests/golden)
+ *   rsqrtps                SponzaScene.cpp:67            recorded host table (tests/golden)
+ *   SponzaShader           SponzaScene.cpp:13-104        closed forms (one light, no light, NaN)
  *   calc_mip               Texture.cpp:212-233           known-gradient cases (exact powers of two)
  *   sample_wrap / gather   Texture.cpp:236-452           constant texture, 0/1 weights, wrap symmetry,
  *                                                       Morton = bit-interleave bijection
@@ -51,6 +53,9 @@
 static const uint32_t g_rcpps_table[2048] = {
 #include "rcpps_table.inc"
 };
+static const uint32_t g_rsqrtps_table[4096] = {
+#include "rsqrtps_table.inc"
+};
 
 /* ------------------------------------------------------------------ */
 /* scalar helpers                                                      */
@@ -81,6 +86,24 @@ static inline float maxps(float a, float b) { return a > b ? a : b; }
 
 /* _mm256_rcp_ps, Rasterizer.cpp:375-376: table of the top 11 mantissa bits,
  * exact exponent scaling, subnormal results flushed to zero. */
+/* _mm256_rsqrt_ps on the recorded host (tests/golden/gen_rsqrtps_table.c):
+ * T[E & 1][top 11 mantissa bits] scaled by 2^-(E - (E & 1)) / 2 for a
+ * positive normal x = 2^E * f; +-0 and denormals -> +-inf, +inf -> +0,
+ * other negatives -> the default NaN, a NaN -> that NaN quieted. */
+float sro_rsqrtps(float x)
+{
+    uint32_t u = fbits(x);
+    uint32_t e = (u >> 23) & 255u;
+    if (e == 255u && (u & 0x7FFFFFu)) return ffrom(u | 0x00400000u);
+    if (e == 0u) return ffrom((u & 0x80000000u) | 0x7F800000u);
+    if (u & 0x80000000u) return ffrom(0xFFC00000u);
+    if (e == 255u) return 0.0f;
+    int ub = (int)e - 127, p = ub & 1;
+    uint32_t t = g_rsqrtps_table[(uint32_t)p * 2048u + ((u >> 12) & 2047u)];
+    int re = (int)((t >> 23) & 255u) - (ub - p) / 2;
+    return ffrom(((uint32_t)re << 23) | (t & 0x7FFFFFu));
+}
+
 float sro_rcpps(float x)
 {
     uint32_t u = fbits(x);
@@ -499,6 +522,7 @@ static int chunk_alloc(thread_state* ts, uint32_t* idx)
 }
 
 struct sro_ctx {
+    sro_sponza_constants sponza;
     uint32_t nthreads;
     pthread_t* threads;
     pthread_mutex_t mu;
@@ -738,10 +762,40 @@ static void raster_chunk(thread_state* ts, const chunk_t* ch, uint32_t chunk_pos
     }
 }
 
+/* SponzaShader (Viewer/SponzaScene.cpp:13-104) on one fragment: varyings 0-2
+ * position, 3-5 normal, 6-7 uv. Dot3SoA (SIMDUtil.h:123-126) is
+ * fma(x0, x1, fma(y0, y1, z0 * z1)); _mm256_max_ps(a, b) is a > b ? a : b. */
+static uint32_t shade_sponza(const sro_sponza_constants* k, const sro_draw* d, const float* var,
+                             const float der[4])
+{
+    if (!d->tex || !d->tex->texels || d->tex->size == 0) return 0xFFFFFFFFu;
+    const float px = var[0], py = var[1], pz = var[2], nx = var[3], ny = var[4], nz = var[5];
+    float ndl = maxps(0.1f, fmaf(nx, k->sun_dir[0], fmaf(ny, k->sun_dir[1], nz * k->sun_dir[2])));
+    float rad[3] = { ndl, ndl, ndl };
+    for (int i = 0; i < SRO_SPONZA_LIGHTS; ++i) {
+        const sro_sponza_light* L = &k->lights[i];
+        const float tx = L->pos[0] - px, ty = L->pos[1] - py, tz = L->pos[2] - pz;
+        const float dsq = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
+        const float rd = sro_rsqrtps(dsq);
+        const float dist = sro_rcpps(rd);
+        const float lx = tx * rd, ly = ty * rd, lz = tz * rd;
+        const float nl = maxps(0.0f, fmaf(lx, nx, fmaf(ly, ny, lz * nz)));
+        const float atten = sro_rcpps(1.0f + fmaf(0.1f, dist, dsq * 0.01f));
+        const float lr = nl * (L->intensity * atten);
+        for (int c = 0; c < 3; ++c) rad[c] = rad[c] + lr * L->colour[c];
+    }
+    for (int c = 0; c < 3; ++c) rad[c] = rad[c] + k->ambient[c];
+    float rgba[4];
+    sro_sample_wrap(d->tex, var[6], var[7], der[0], der[1], der[2], der[3], rgba);
+    return sro_pack_rgba(rad[0] * rgba[0], rad[1] * rgba[1], rad[2] * rgba[2], rgba[3]);
+}
+
 /* Pixel shaders, Viewer/Shaders.h:71-130, on one fragment's interpolants. */
-static uint32_t shade(const sro_draw* d, const float* var, const float der[4])
+static uint32_t shade(const sro_sponza_constants* sponza, const sro_draw* d, const float* var, const float der[4])
 {
     switch (d->shader) {
+    case SRO_SHADER_SPONZA:
+        return shade_sponza(sponza, d, var, der);
     case SRO_SHADER_VISUALIZE_NORMALS:
         return sro_pack_rgba(fmaf(var[3], 0.5f, 0.5f), fmaf(var[4], 0.5f, 0.5f), fmaf(var[5], 0.5f, 0.5f), 1.0f);
     case SRO_SHADER_VISUALIZE_UVS:
@@ -828,7 +882,7 @@ static void raster_tile(sro_ctx* ctx, thread_state* ts, uint32_t bin)
             der[2 * uv + 1] = t01 - var[k];
         }
         uint32_t px = (uint32_t)fr->y * SRO_BIN_SIZE + fr->x;
-        colour[px] = shade(d, var, der);
+        colour[px] = shade(&ctx->sponza, d, var, der);
         if (pl->primid) pl->primid[(size_t)tile * 4096u + px] = ch->primid[t];
     }
 }
@@ -971,6 +1025,11 @@ void sro_clear(sro_plane* pl, uint32_t colour, int clear_colour, int clear_depth
     if (clear_colour) memset(pl->colour, (int)(uint8_t)colour, n * 4u);
 }
 
+void sro_set_sponza_constants(sro_ctx* ctx, const sro_sponza_constants* k)
+{
+    if (ctx && k) ctx->sponza = *k;
+}
+
 void sro_blit(const sro_plane* pl, uint32_t* linear)
 {
     for (uint32_t ty = 0; ty < pl->tiles_y; ++ty)
@@ -994,6 +1053,7 @@ static int validate_draw(const sro_draw* d, uint32_t nplanes)
     case SRO_SHADER_UNLIT_DIFFUSE: if (apt < 8 || d->uv_offset + 1 >= apt) return -1; break;
     case SRO_SHADER_VISUALIZE_NORMALS: if (apt < 6 || d->uv_offset + 1 >= apt) return -1; break;
     case SRO_SHADER_VISUALIZE_UVS: if (apt < 8 || d->uv_offset + 1 >= apt) return -1; break;
+    case SRO_SHADER_SPONZA: if (apt != 8 || d->uv_offset + 1 >= apt) return -1; break;
     default: return -1;
     }
     return 0;

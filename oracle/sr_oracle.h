@@ -33,7 +33,20 @@ extern "C" {
 #define SRO_MAX_VARYINGS 8
 #define SRO_MAX_MIPS 14
 
-enum { SRO_SHADER_UNLIT_DIFFUSE = 0, SRO_SHADER_VISUALIZE_NORMALS = 1, SRO_SHADER_VISUALIZE_UVS = 2 };
+enum { SRO_SHADER_UNLIT_DIFFUSE = 0, SRO_SHADER_VISUALIZE_NORMALS = 1, SRO_SHADER_VISUALIZE_UVS = 2,
+       SRO_SHADER_SPONZA = 3 };
+
+/* SponzaScene::Constants (Viewer/SponzaScene.h:36-47), the global the
+ * SponzaShader reads: each m_sunDir / m_ambCol component is one float
+ * broadcast over the lanes; 16 PointLights (m_falloff is not read). */
+#define SRO_SPONZA_LIGHTS 16
+typedef struct sro_sponza_light {
+    float pos[3], colour[3], intensity, falloff;
+} sro_sponza_light;
+typedef struct sro_sponza_constants {
+    float sun_dir[3], ambient[3];
+    sro_sponza_light lights[SRO_SPONZA_LIGHTS];
+} sro_sponza_constants;
 
 /* Tex::TextureData (SoftRast/Texture.h:21-41): 32x32-tiled, Morton-swizzled
  * RGBA8 mip chain. texels == NULL or size == 0 means "empty texture". */
@@ -99,11 +112,16 @@ void sro_clear(sro_plane* plane, uint32_t colour, int clear_colour, int clear_de
 int sro_render(sro_ctx* ctx, const sro_draw* draws, uint32_t ndraws, sro_plane* planes,
                uint32_t nplanes, sro_stats* stats, uint32_t* tile_entries);
 
+/* The SponzaShader's global constants (g_constants, Viewer/SponzaScene.cpp:11)
+ * for later renders of this context. */
+void sro_set_sponza_constants(sro_ctx* ctx, const sro_sponza_constants* k);
+
 /* BlitJobFn (SoftRast/Renderer.cpp:319-347): tiled colour -> linear rows. */
 void sro_blit(const sro_plane* plane, uint32_t* linear);
 
 /* ---- single-function entry points, for known-answer tests ---- */
 float sro_rcpps(float x);
+float sro_rsqrtps(float x);
 uint32_t sro_pack_rgba(float r, float g, float b, float a);
 uint32_t sro_morton5(uint32_t x, uint32_t y);
 int32_t sro_calc_mip(const sro_texture* tex, float dudx, float dudy, float dvdx, float dvdy);

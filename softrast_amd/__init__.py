@@ -31,6 +31,8 @@ SRG_OK = 0
 SHADER_UNLIT_DIFFUSE = 0
 SHADER_VISUALIZE_NORMALS = 1
 SHADER_VISUALIZE_UVS = 2
+SHADER_SPONZA = 3
+SPONZA_LIGHTS = 16
 ORDER_NATURAL, ORDER_TRICOUNT, ORDER_LEARNED = 0, 1, 2
 CFG_KEEP_PRIMID, CFG_STATS, CFG_TIMING, CFG_TILE_COSTS, CFG_GRAPH, CFG_AUTOTUNE = 1, 2, 4, 8, 16, 32
 # back-end layouts (work-bin edge, workgroups per bin) the library accepts and the autotuner sweeps
@@ -46,7 +48,7 @@ SRG_SYMBOLS = (
     "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
     "srg_get_tuning", "srg_draw_indexed_n", "srg_obj_load", "srg_obj_parse", "srg_obj_mesh_count",
     "srg_obj_get_mesh", "srg_obj_material_count", "srg_obj_material_name", "srg_obj_material_diffuse",
-    "srg_obj_free",
+    "srg_obj_free", "srg_set_sponza_constants",
 )
 OBJ_FLIP_WINDING, OBJ_FLIP_UVS = 1, 4  # Obj::LoadFlags (Viewer/Obj.h:54-60)
 PLANE_WRITE = 0xFFFFFFFF
@@ -158,6 +160,7 @@ def load_library(path=None):
         "srg_obj_material_name": (ctypes.c_char_p, [vp, u32]),
         "srg_obj_material_diffuse": (ctypes.c_char_p, [vp, u32]),
         "srg_obj_free": (None, [vp]),
+        "srg_set_sponza_constants": (i32, [vp, vp]),
         "srg_texture_build_rgba8": (i32, [vp, u32, u32, i32, vp, pu64, ctypes.POINTER(u32),
                                           ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     }
@@ -272,6 +275,27 @@ class _Shader:
 UnlitDiffuseShader = _Shader("UnlitDiffuseShader", SHADER_UNLIT_DIFFUSE)
 VisualizeNormalsShader = _Shader("VisualizeNormalsShader", SHADER_VISUALIZE_NORMALS)
 VisualizeUVsShader = _Shader("VisualizeUVsShader", SHADER_VISUALIZE_UVS)
+SponzaShader = _Shader("SponzaShader", SHADER_SPONZA)
+_SAMPLING_SHADERS = (SHADER_UNLIT_DIFFUSE, SHADER_SPONZA)
+
+
+class SponzaConstants:
+    """SponzaScene::Constants (Viewer/SponzaScene.h:15-47) as the SponzaShader
+    reads it: sun_dir and ambient (3 floats each), 16 point lights of
+    (pos[3], colour[3], intensity, falloff). to_bytes() is the
+    srg_sponza_constants / sro_sponza_constants layout."""
+
+    def __init__(self, sun_dir=(0.0, 0.0, 0.0), ambient=(0.0, 0.0, 0.0), lights=()):
+        self.sun_dir = np.asarray(sun_dir, np.float32).reshape(3)
+        self.ambient = np.asarray(ambient, np.float32).reshape(3)
+        self.lights = np.zeros((SPONZA_LIGHTS, 8), np.float32)
+        for i, light in enumerate(lights):
+            pos, colour, intensity = light[0], light[1], light[2]
+            falloff = light[3] if len(light) > 3 else 0.0
+            self.lights[i] = [*pos, *colour, intensity, falloff]
+
+    def to_bytes(self):
+        return np.concatenate([self.sun_dir, self.ambient, self.lights.reshape(-1)]).astype(np.float32).tobytes()
 
 
 # ---------------------------------------------------------------------------
@@ -558,8 +582,8 @@ class RenderContext:
         d = SrgDraw()
         shader = call.m_pixelShader if call.m_pixelShader is not None else VisualizeNormalsShader
         d.shader = shader.id
-        d.texture = self.texture_handle(call.m_pixelUniforms) if shader.id == SHADER_UNLIT_DIFFUSE else 0
-        if isinstance(call.m_pixelUniforms, TextureData) and shader.id == SHADER_UNLIT_DIFFUSE and \
+        d.texture = self.texture_handle(call.m_pixelUniforms) if shader.id in _SAMPLING_SHADERS else 0
+        if isinstance(call.m_pixelUniforms, TextureData) and shader.id in _SAMPLING_SHADERS and \
                 call.m_pixelUniforms.m_texels.size == 0:
             d.texture = 0
         for name, buf in (("index", call.m_indexBuffer), ("position", call.m_positionBuffer),
@@ -632,6 +656,15 @@ class RenderContext:
     def set_sched_model(self, params):
         p = np.ascontiguousarray(params, np.float32)
         _check(self._lib.srg_sched_set_model(self._ctx, _ptr(p), p.size), "set_sched_model")
+
+    def set_sponza_constants(self, consts):
+        """SponzaShader constants for frames ended after this call
+        (SponzaConstants, or bytes in its layout)."""
+        raw = consts.to_bytes() if isinstance(consts, SponzaConstants) else bytes(consts)
+        buf = np.frombuffer(raw, np.float32).copy()
+        if buf.size != 6 + SPONZA_LIGHTS * 8:
+            raise ValueError("SponzaScene::Constants is 134 floats")
+        _check(self._lib.srg_set_sponza_constants(self._ctx, _ptr(buf)), "set_sponza_constants")
 
     def tuning(self):
         """Autotuner state for the last frame's signature: {"state": "off" |

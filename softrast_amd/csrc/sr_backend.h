@@ -40,13 +40,95 @@ __device__ __forceinline__ uint32_t record_of_order(const FrameDev& F, uint32_t 
     return fan == kFanUnclipped ? g : F.ntris + F.clip_base[g] + fan;
 }
 
+// SponzaShader (Viewer/SponzaScene.cpp:13-104) for the visible fragment of
+// one pixel: all eight varyings (0-2 position, 3-5 normal, 6-7 uv), the uv
+// derivatives from the slots at uv_offset, sun, 16 point lights by
+// RSQRTPS / RCPPS, ambient, times the texture sample.
+__device__ __forceinline__ uint32_t shade_sponza(const FrameDev& F, const DrawDev& D, uint32_t rec, int32_t X0,
+                                                 int32_t Y0, uint32_t px, uint32_t py)
+{
+    if (D.tex == kNoTex) return 0xFFFFFFFFu;
+    const TexDev& T = F.texs[D.tex];
+    if (T.empty) return 0xFFFFFFFFu;
+    const float4* Sp = F.rec_shade + (size_t)rec * 4;
+    const float4* Ep = F.rec_ext + (size_t)rec * 3;
+    const float4 s0 = Sp[0], s1 = Sp[1], s2 = Sp[2], s3 = Sp[3];
+    const float4 e0 = Ep[0], e1 = Ep[1], e2 = Ep[2];
+    const uint4 q2 = F.rec_raster[(size_t)rec * 4 + 2];
+    const uint4 q3 = F.rec_raster[(size_t)rec * 4 + 3];
+    const float sx = (float)X0 - ffrom(q2.w), sy = (float)Y0 - ffrom(q3.x);
+    const float wdx = s0.x, wdy = s0.y;
+    const float wc = wdx * sx + wdy * sy + s0.z;
+    const float pdx[kWideSlots] = { s0.w, s1.z, s2.y, s3.x, e0.x, e0.w, e1.z, e2.y };
+    const float pdy[kWideSlots] = { s1.x, s1.w, s2.z, s3.y, e0.y, e1.x, e1.w, e2.z };
+    const float pf0[kWideSlots] = { s1.y, s2.x, s2.w, s3.z, e0.z, e1.y, e2.x, e2.w };
+    const float X = (float)px, Y = (float)py;
+    const float iq = 1.0f / __builtin_fmaf(X, wdx, __builtin_fmaf(Y, wdy, wc));
+    float pc[kWideSlots], var[kWideSlots];
+#pragma unroll
+    for (uint32_t k = 0; k < kWideSlots; ++k) {
+        pc[k] = pdx[k] * sx + pdy[k] * sy + pf0[k];
+        var[k] = iq * __builtin_fmaf(pdy[k], Y, __builtin_fmaf(pdx[k], X, pc[k]));
+    }
+    // (u, v) planes at uv_offset: selected without indexing the arrays by a
+    // run-time value (which would move them to scratch memory)
+    float tdx[2] = { 0.0f, 0.0f }, tdy[2] = { 0.0f, 0.0f }, tc[2] = { 0.0f, 0.0f }, tv[2] = { 0.0f, 0.0f };
+    const uint32_t ds = D.deriv_slot;
+#pragma unroll
+    for (uint32_t k = 0; k + 1 < kWideSlots; ++k)
+        if (k == ds) {
+            tdx[0] = pdx[k]; tdy[0] = pdy[k]; tc[0] = pc[k]; tv[0] = var[k];
+            tdx[1] = pdx[k + 1]; tdy[1] = pdy[k + 1]; tc[1] = pc[k + 1]; tv[1] = var[k + 1];
+        }
+    const float X1 = 1.0f + X, Y1 = 1.0f + Y;
+    const float i10 = rcpps(__builtin_fmaf(wdx, X1, __builtin_fmaf(wdy, Y, wc)));
+    const float i01 = rcpps(__builtin_fmaf(wdx, X, __builtin_fmaf(wdy, Y1, wc)));
+    float der[4];
+#pragma unroll
+    for (uint32_t uv = 0; uv < 2; ++uv) {
+        der[2 * uv] = i10 * __builtin_fmaf(tdx[uv], X1, __builtin_fmaf(tdy[uv], Y, tc[uv])) - tv[uv];
+        der[2 * uv + 1] = i01 * __builtin_fmaf(tdx[uv], X, __builtin_fmaf(tdy[uv], Y1, tc[uv])) - tv[uv];
+    }
+
+    const SponzaDev& K = *F.sponza;
+    const float nx = var[3], ny = var[4], nz = var[5];
+    // Dot3SoA (SIMDUtil.h:123-126): fma(x0, x1, fma(y0, y1, z0 * z1))
+    const float ndl = maxps(0.1f, __builtin_fmaf(nx, K.sun_dir[0], __builtin_fmaf(ny, K.sun_dir[1], nz * K.sun_dir[2])));
+    float r0 = ndl, r1 = ndl, r2 = ndl;
+#pragma unroll 4
+    for (uint32_t i = 0; i < kSponzaLights; ++i) {
+        const SponzaLightDev& L = K.lights[i];
+        const float tx = L.pos[0] - var[0], ty = L.pos[1] - var[1], tz = L.pos[2] - var[2];
+        const float dsq = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz));
+        const float rd = rsqrtps(dsq);
+        const float dist = rcpps(rd);
+        const float lx = tx * rd, ly = ty * rd, lz = tz * rd;
+        const float nl = maxps(0.0f, __builtin_fmaf(lx, nx, __builtin_fmaf(ly, ny, lz * nz)));
+        const float atten = rcpps(1.0f + __builtin_fmaf(0.1f, dist, dsq * 0.01f));
+        const float lr = nl * (L.intensity * atten);
+        r0 = r0 + lr * L.colour[0];
+        r1 = r1 + lr * L.colour[1];
+        r2 = r2 + lr * L.colour[2];
+    }
+    r0 = r0 + K.ambient[0];
+    r1 = r1 + K.ambient[1];
+    r2 = r2 + K.ambient[2];
+    float o[4];
+    sample_wrap(T, var[6], var[7], der[0], der[1], der[2], der[3], o);
+    return pack_rgba(r0 * o[0], r1 * o[1], r2 * o[2], o[3]);
+}
+
 // ComputeInterpolants + pixel shader for the visible fragment of one pixel
 // (x, y tile-relative) (Rasterizer.cpp:356-400, Viewer/Shaders.h:71-130).
+// WIDE: the frame keeps eight attribute planes per record and may hold
+// SponzaShader draws.
+template <bool WIDE>
 __device__ __forceinline__ uint32_t shade_pixel(const FrameDev& F, uint32_t rec, int32_t X0, int32_t Y0, uint32_t px,
                                                 uint32_t py)
 {
     const uint32_t d = F.rec_meta[rec];
     const DrawDev& D = F.draws[d];
+    if (WIDE && D.shader == SRG_SHADER_SPONZA) return shade_sponza(F, D, rec, X0, Y0, px, py);
     const float4* Sp = F.rec_shade + (size_t)rec * 4;
     const float4 s0 = Sp[0], s1 = Sp[1], s2 = Sp[2], s3 = Sp[3];
     const uint4 q2 = F.rec_raster[(size_t)rec * 4 + 2];
@@ -314,7 +396,7 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     }
 }
 
-template <uint32_t S, uint32_t H, bool STATS>
+template <uint32_t S, uint32_t H, bool STATS, bool WIDE>
 __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
 {
     constexpr uint32_t T = backend_threads(S, H);
@@ -392,7 +474,7 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
 #pragma unroll
             for (uint32_t i = 0; i < BATCH; ++i) {
                 const uint32_t p = tid + (p0 + i) * T;
-                col[i] = (win[i] && (V.flags & VF_HAS_COLOUR)) ? shade_pixel(F, rec[i], X0, Y0, sx0 + p % S, sy0 + p / S)
+                col[i] = (win[i] && (V.flags & VF_HAS_COLOUR)) ? shade_pixel<WIDE>(F, rec[i], X0, Y0, sx0 + p % S, sy0 + p / S)
                                                                : V.clear_colour;
             }
 #pragma unroll

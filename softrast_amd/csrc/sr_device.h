@@ -13,10 +13,14 @@ namespace srg {
 constexpr uint32_t kBin = 64;             // Config.h:18-21 (the semantic tile)
 constexpr uint32_t kTilePixels = kBin * kBin;
 constexpr uint32_t kMaxSlots = 4;         // attribute planes kept per primitive
+constexpr uint32_t kWideSlots = 8;        // ... by frames with a draw that reads eight varyings
 constexpr uint32_t kNoTex = 0xFFFFFFFFu;
 
 __device__ __constant__ uint32_t c_rcpps_table[2048] = {
 #include "rcpps_table.inc"
+};
+__device__ __constant__ uint32_t c_rsqrtps_table[4096] = {
+#include "rsqrtps_table.inc"
 };
 
 __device__ __forceinline__ uint32_t fbits(float f) { return __float_as_uint(f); }
@@ -53,6 +57,24 @@ __device__ __forceinline__ float rcpps(float x)
     const uint32_t t = c_rcpps_table[(u >> 12) & 2047u];
     const int ne = (int)((t >> 23) & 255u) - ((int)e - 127);
     return ffrom(ne <= 0 ? sign : (sign | ((uint32_t)ne << 23) | (t & 0x7FFFFFu)));
+}
+
+// _mm256_rsqrt_ps as recorded on the reference's Intel host (SponzaScene.cpp:67):
+// for x = 2^E * f, table T[E & 1][top 11 mantissa bits] scaled by
+// 2^-(E - (E & 1)) / 2 (always a normal result); +-0 and denormals -> +-inf,
+// +inf -> +0, other negatives -> the default NaN, NaN quieted.
+__device__ __forceinline__ float rsqrtps(float x)
+{
+    const uint32_t u = fbits(x);
+    const uint32_t e = (u >> 23) & 255u;
+    if (e == 255u && (u & 0x7FFFFFu)) return ffrom(u | 0x00400000u);
+    if (e == 0u) return ffrom((u & 0x80000000u) | 0x7F800000u);
+    if (u & 0x80000000u) return ffrom(0xFFC00000u);
+    if (e == 255u) return 0.0f;
+    const int ub = (int)e - 127, p = ub & 1;
+    const uint32_t t = c_rsqrtps_table[(uint32_t)p * 2048u + ((u >> 12) & 2047u)];
+    const int re = (int)((t >> 23) & 255u) - (ub - p) / 2;
+    return ffrom(((uint32_t)re << 23) | (t & 0x7FFFFFu));
 }
 
 // One channel of RGBA32SoA_To_RGBA8AoS (SIMDUtil.h:87-121): RNE(fma(c,255,.5)),
@@ -94,9 +116,9 @@ __device__ __forceinline__ float lerp_fma(float a, float b, float t)
 __device__ __forceinline__ float texel_channel(uint32_t byte) { return (float)byte * (1.0f / 255.0f); }
 
 // CalcMipLevels (Texture.cpp:212-233), SampleWrap (:381-452) and
-// GatherQuadsAndInterpolate (:243-379) for one pixel; returns packed RGBA8.
-__device__ __forceinline__ uint32_t sample_unlit(const TexDev& T, float u, float v, float dudx, float dudy,
-                                                 float dvdx, float dvdy)
+// GatherQuadsAndInterpolate (:243-379) for one pixel: RGBA in [0, 1].
+__device__ __forceinline__ void sample_wrap(const TexDev& T, float u, float v, float dudx, float dudy, float dvdx,
+                                            float dvdy, float o[4])
 {
     const float height = (float)(1u << T.hlog2), width = (float)(1u << T.wlog2);
     const float p = dudx * width, q = dudy * height, r = dvdx * width, s = dvdy * height;
@@ -132,7 +154,6 @@ __device__ __forceinline__ uint32_t sample_unlit(const TexDev& T, float u, float
     const uint32_t t10 = base[1024u * (ty0 + (x1 >> 5)) + (mx1 | my0)];
     const uint32_t t11 = base[1024u * (ty1 + (x1 >> 5)) + (mx1 | my1)];
 
-    float o[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const uint32_t sh = 8u * (uint32_t)c;
@@ -140,6 +161,14 @@ __device__ __forceinline__ uint32_t sample_unlit(const TexDev& T, float u, float
         const float right = lerp_fma(texel_channel((t10 >> sh) & 255u), texel_channel((t11 >> sh) & 255u), vi);
         o[c] = lerp_fma(left, right, ui);
     }
+}
+
+// UnlitDiffuseShader's sample (Viewer/Shaders.h:71-104), packed to RGBA8.
+__device__ __forceinline__ uint32_t sample_unlit(const TexDev& T, float u, float v, float dudx, float dudy,
+                                                 float dvdx, float dvdy)
+{
+    float o[4];
+    sample_wrap(T, u, v, dudx, dudy, dvdx, dvdy, o);
     return pack_rgba(o[0], o[1], o[2], o[3]);
 }
 

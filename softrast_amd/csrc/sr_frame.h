@@ -14,10 +14,21 @@ struct DrawDev {
     uint32_t idx_stride, pos_stride, attr_stride, ntris;
     uint32_t pos_num, attr_num, tri_base, view;
     uint32_t shader, tex, nslots, deriv_slot;  // deriv_slot: first slot of (u, v) at uv_offset
-    uint32_t slot_attr[kMaxSlots];
+    uint32_t slot_attr[kWideSlots];
     uint32_t block_base;  // first k_setup workgroup of the draw (kSetupThreads triangles each)
     uint32_t pad[3];
     float mvp[16];
+};
+
+// SponzaScene::Constants as the SponzaShader reads it (Viewer/SponzaScene.h:15-47):
+// one float per m_sunDir / m_ambCol lane-broadcast component, 16 point lights.
+constexpr uint32_t kSponzaLights = 16;
+struct SponzaLightDev {
+    float pos[3], colour[3], intensity, falloff;
+};
+struct SponzaDev {
+    float sun_dir[3], ambient[3];
+    SponzaLightDev lights[kSponzaLights];
 };
 
 enum : uint32_t {
@@ -73,7 +84,9 @@ struct FrameDev {
     uint32_t ndraws, nviews, total_bins, ntris;
     uint32_t bin_k, bin_size, clip_cap, entry_cap;
     uint4* rec_raster;     // 4 x uint4 per record: edges, bbox, r0, z plane
-    float4* rec_shade;     // 4 x float4 per record: 1/w plane and attribute planes
+    float4* rec_shade;     // 4 x float4 per record: 1/w plane and attribute planes 0-3
+    float4* rec_ext;       // 3 x float4 per record: attribute planes 4-7 (wide frames only)
+    const SponzaDev* sponza;  // SponzaShader constants (wide frames only)
     uint32_t* rec_meta;    // draw index; 0xFFFFFFFF = no primitive
     uint32_t* clip_queue;  // global triangle ids routed to the clipper
     uint32_t* clip_base;   // per global triangle: first fan record (relative to ntris)

@@ -17,7 +17,7 @@ struct Prim {
     int32_t xmin, xmax, ymin, ymax;
     uint32_t tight;  // bit 0: tight bound holds; bits 1, 2: the bound starts one pixel left of xmin / above ymin
     float r0x, r0y, zdx, zdy, zf0, wdx, wdy, wf0;
-    float adx[kMaxSlots], ady[kMaxSlots], af0[kMaxSlots];
+    float adx[kWideSlots], ady[kWideSlots], af0[kWideSlots];  // the first NS used by a setup<NS>
 };
 
 // kt::Mat4 * kt::Vec4 (Binning.cpp:477-479), order ((c0*x + c1*y) + c2*z) + c3*w,
@@ -98,8 +98,9 @@ __device__ __forceinline__ uint32_t tight_bound(int64_t area2, int32_t mnx, int3
 }
 
 // Viewport, snap, cull, bbox, edges and planes (Binning.cpp:291-350) for the
-// kept attribute slots. Returns false if the triangle is culled.
-__device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const float a[3][kMaxSlots], uint32_t nslots,
+// kept attribute slots (NS of them at most). Returns false if the triangle is culled.
+template <uint32_t NS>
+__device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const float a[3][kWideSlots], uint32_t nslots,
                                            uint32_t width, uint32_t height)
 {
     const float hw = (float)width * 0.5f, hh = (float)height * 0.5f;
@@ -132,7 +133,7 @@ __device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const flo
     setup_plane(cterm, d10x, d10y, d20x, d20y, v[1].z * iw1 - v[0].z * iw0, v[2].z * iw2 - v[0].z * iw0, s.zdx, s.zdy);
     setup_plane(cterm, d10x, d10y, d20x, d20y, iw1 - iw0, iw2 - iw0, s.wdx, s.wdy);
 #pragma unroll
-    for (uint32_t k = 0; k < kMaxSlots; ++k)
+    for (uint32_t k = 0; k < NS; ++k)
         if (k < nslots)
             setup_plane(cterm, d10x, d10y, d20x, d20y, a[1][k] * iw1 - a[0][k] * iw0, a[2][k] * iw2 - a[0][k] * iw0,
                         s.adx[k], s.ady[k]);
@@ -141,7 +142,7 @@ __device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const flo
     s.zf0 = v[0].z * iw0;
     s.wf0 = iw0;
 #pragma unroll
-    for (uint32_t k = 0; k < kMaxSlots; ++k) s.af0[k] = k < nslots ? a[0][k] * iw0 : 0.0f;
+    for (uint32_t k = 0; k < NS; ++k) s.af0[k] = k < nslots ? a[0][k] * iw0 : 0.0f;
     return true;
 }
 
@@ -275,6 +276,7 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
     }
 }
 
+template <uint32_t NS>
 __device__ __forceinline__ void write_record(const FrameDev& F, uint32_t r, const Prim& s, uint32_t draw)
 {
     uint4* R = F.rec_raster + (size_t)r * 4;
@@ -293,13 +295,20 @@ __device__ __forceinline__ void write_record(const FrameDev& F, uint32_t r, cons
     S[1] = make_float4(s.ady[0], s.af0[0], s.adx[1], s.ady[1]);
     S[2] = make_float4(s.af0[1], s.adx[2], s.ady[2], s.af0[2]);
     S[3] = make_float4(s.adx[3], s.ady[3], s.af0[3], 0.0f);
+    if (NS > kMaxSlots) {
+        float4* E = F.rec_ext + (size_t)r * 3;
+        E[0] = make_float4(s.adx[4], s.ady[4], s.af0[4], s.adx[5]);
+        E[1] = make_float4(s.ady[5], s.af0[5], s.adx[6], s.ady[6]);
+        E[2] = make_float4(s.af0[6], s.adx[7], s.ady[7], s.af0[7]);
+    }
     F.rec_meta[r] = draw;
 }
 
 // FetchIndices / FetchPositions / attribute fetch + MVP (Binning.cpp:167-221, 474-479).
 // Returns false (and flags the frame) if an index is out of range.
+template <uint32_t NS>
 __device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev& D, uint32_t tri, float4 v[3],
-                                               float a[3][kMaxSlots])
+                                               float a[3][kWideSlots])
 {
     uint32_t idx[3];
 #pragma unroll
@@ -321,7 +330,7 @@ __device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev&
         v[k] = transform(D.mvp, p[0], p[1], p[2]);
         const float* at = (const float*)(D.attr + (size_t)idx[k] * D.attr_stride);
 #pragma unroll
-        for (uint32_t s = 0; s < kMaxSlots; ++s) a[k][s] = s < D.nslots ? at[D.slot_attr[s]] : 0.0f;
+        for (uint32_t s = 0; s < NS; ++s) a[k][s] = s < D.nslots ? at[D.slot_attr[s]] : 0.0f;
     }
     return true;
 }
@@ -351,6 +360,9 @@ __device__ __forceinline__ void block_stats_flush(const FrameDev& F, BlockStats*
     }
 }
 
+// NS: attribute planes kept per record (kMaxSlots, or kWideSlots in a frame
+// that has a draw reading eight varyings).
+template <uint32_t NS>
 __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
 {
     __shared__ BlockStats bs;
@@ -374,17 +386,17 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
     const uint32_t g = D.tri_base + tri;
 
     float4 v[3];
-    float a[3][kMaxSlots];
+    float a[3][kWideSlots];
     Prim s;
     bool survive = false, clipped = false;
-    if (valid && fetch_triangle(F, D, tri, v, a)) {
+    if (valid && fetch_triangle<NS>(F, D, tri, v, a)) {
         const uint32_t c0 = clip_codes(v[0]), c1 = clip_codes(v[1]), c2 = clip_codes(v[2]);
         const uint32_t mor = c0 | c1 | c2;
-        if (mor == 0) survive = setup_prim(s, v, a, D.nslots, V.width, V.height);
+        if (mor == 0) survive = setup_prim<NS>(s, v, a, D.nslots, V.width, V.height);
         else clipped = (c0 & c1 & c2) == 0;
     }
     if (valid) {
-        if (survive) write_record(F, g, s, d);
+        if (survive) write_record<NS>(F, g, s, d);
         else F.rec_meta[g] = 0xFFFFFFFFu;
     }
     const uint32_t q = wave_add_amount(&F.counters[C_CLIPQ], clipped ? 1u : 0u);
@@ -410,17 +422,18 @@ __device__ __forceinline__ float clip_dot(int plane, float4 v)
 __device__ __forceinline__ float lerp_kt(float a, float b, float t) { return a + (b - a) * t; }
 
 // The clipper's two polygon buffers (up to 9 vertices of 4 position floats
-// and kMaxSlots attribute floats) of every lane of a one-wave workgroup, in
-// LDS: component-major with the lane innermost, so a wave's accesses to the
-// same component are conflict-free.
+// and NS attribute floats) of every lane of a one-wave workgroup, in LDS:
+// component-major with the lane innermost, so a wave's accesses to the same
+// component are conflict-free.
 constexpr uint32_t kClipThreads = 64;
-constexpr uint32_t kClipComps = 4 + kMaxSlots;
+template <uint32_t NS>
 struct ClipPoly {
+    static constexpr uint32_t kComps = 4 + NS;
     float* lds;
     uint32_t lane;
     __device__ __forceinline__ float& at(uint32_t buf, uint32_t vtx, uint32_t comp) const
     {
-        return lds[((buf * 9u + vtx) * kClipComps + comp) * kClipThreads + lane];
+        return lds[((buf * 9u + vtx) * kComps + comp) * kClipThreads + lane];
     }
     __device__ __forceinline__ float4 pos(uint32_t buf, uint32_t vtx) const
     {
@@ -431,7 +444,8 @@ struct ClipPoly {
 // ClipPlane (Binning.cpp:85-165) over the set planes of `mask`, lowest first,
 // stopping when the polygon empties (:517-523). Returns the vertex count; the
 // result is in buffer `pin`.
-__device__ __forceinline__ uint32_t clip_polygon(const ClipPoly& poly, uint32_t mask, uint32_t& pin)
+template <uint32_t NS>
+__device__ __forceinline__ uint32_t clip_polygon(const ClipPoly<NS>& poly, uint32_t mask, uint32_t& pin)
 {
     uint32_t n = 3;
     pin = 0;
@@ -446,7 +460,7 @@ __device__ __forceinline__ uint32_t clip_polygon(const ClipPoly& poly, uint32_t 
             const float d1 = clip_dot(plane, poly.pos(pin, i1));
             const bool in0 = d0 >= 0.0f, in1 = d1 >= 0.0f;
             if (in0) {
-                for (uint32_t c = 0; c < kClipComps; ++c) poly.at(po, nout, c) = poly.at(pin, i0, c);
+                for (uint32_t c = 0; c < ClipPoly<NS>::kComps; ++c) poly.at(po, nout, c) = poly.at(pin, i0, c);
                 ++nout;
             }
             if (in0 != in1) {
@@ -454,7 +468,7 @@ __device__ __forceinline__ uint32_t clip_polygon(const ClipPoly& poly, uint32_t 
                 float tt;
                 if (in1) { tt = d1 / (d1 - d0); ia = i1; ib = i0; }
                 else     { tt = d0 / (d0 - d1); ia = i0; ib = i1; }
-                for (uint32_t c = 0; c < kClipComps; ++c)
+                for (uint32_t c = 0; c < ClipPoly<NS>::kComps; ++c)
                     poly.at(po, nout, c) = lerp_kt(poly.at(pin, ia, c), poly.at(pin, ib, c), tt);
                 ++nout;
             }
@@ -481,15 +495,16 @@ __device__ __forceinline__ uint32_t draw_of_triangle(const FrameDev& F, uint32_t
 // Slow path: the queued triangles that cross a clip plane, one per lane of
 // one-wave workgroups; the fan triangles (0, i-1, i) (Binning.cpp:525-533) get
 // records after the submitted triangles' and bin counts like any other.
+template <uint32_t NS>
 __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
 {
-    __shared__ float s_poly[2 * 9 * kClipComps * kClipThreads];
+    __shared__ float s_poly[2 * 9 * ClipPoly<NS>::kComps * kClipThreads];
     __shared__ uint32_t s_hash[kClipThreads / 64 * kWaveScratch];
     uint32_t* ht = s_hash + (threadIdx.x >> 6) * kWaveScratch;
     wave_hash_init(ht);
     const uint32_t n = F.counters[C_CLIPQ];
     const uint32_t stride = gridDim.x * blockDim.x;
-    const ClipPoly poly{ s_poly, threadIdx.x };
+    const ClipPoly<NS> poly{ s_poly, threadIdx.x };
     for (uint32_t base_i = blockIdx.x * blockDim.x; base_i < n; base_i += stride) {
         const uint32_t qi = base_i + threadIdx.x;
         const bool active = qi < n;
@@ -499,15 +514,15 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
             d = draw_of_triangle(F, g);
             const DrawDev& D = F.draws[d];
             float4 v[3];
-            float a[3][kMaxSlots];
-            fetch_triangle(F, D, g - D.tri_base, v, a);
+            float a[3][kWideSlots];
+            fetch_triangle<NS>(F, D, g - D.tri_base, v, a);
             const uint32_t mor = clip_codes(v[0]) | clip_codes(v[1]) | clip_codes(v[2]);
             for (uint32_t k = 0; k < 3; ++k) {
                 poly.at(0, k, 0) = v[k].x;
                 poly.at(0, k, 1) = v[k].y;
                 poly.at(0, k, 2) = v[k].z;
                 poly.at(0, k, 3) = v[k].w;
-                for (uint32_t t = 0; t < kMaxSlots; ++t) poly.at(0, k, 4 + t) = a[k][t];
+                for (uint32_t t = 0; t < NS; ++t) poly.at(0, k, 4 + t) = a[k][t];
             }
             const uint32_t np = clip_polygon(poly, mor, pin);
             nfan = np >= 3 ? np - 2 : 0;
@@ -546,18 +561,18 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
             bool ok = false;
             uint32_t r = 0;
             if (t < total) {
-                const ClipPoly owner{ poly.lds, (uint32_t)src };
+                const ClipPoly<NS> owner{ poly.lds, (uint32_t)src };
                 const float4 fv[3] = { owner.pos(opin, 0), owner.pos(opin, j + 1), owner.pos(opin, j + 2) };
-                float fa[3][kMaxSlots];
-                for (uint32_t a = 0; a < kMaxSlots; ++a) {
+                float fa[3][kWideSlots];
+                for (uint32_t a = 0; a < NS; ++a) {
                     fa[0][a] = owner.at(opin, 0, 4 + a);
                     fa[1][a] = owner.at(opin, j + 1, 4 + a);
                     fa[2][a] = owner.at(opin, j + 2, 4 + a);
                 }
                 r = F.ntris + obase + j;
-                ok = setup_prim(s, fv, fa, D.nslots, V.width, V.height);
+                ok = setup_prim<NS>(s, fv, fa, D.nslots, V.width, V.height);
                 if (ok) {
-                    write_record(F, r, s, od);
+                    write_record<NS>(F, r, s, od);
                     F.clip_order[obase + j] = (og << 3) | j;
                 } else {
                     F.rec_meta[r] = 0xFFFFFFFFu;

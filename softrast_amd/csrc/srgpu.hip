@@ -193,6 +193,8 @@ struct srg_ctx {
     // arenas
     DevArray<uint4> rec_raster;
     DevArray<float4> rec_shade;
+    DevArray<float4> rec_ext;         // attribute planes 4-7, frames with a SponzaShader draw
+    SponzaDev sponza{};               // SponzaShader constants (srg_set_sponza_constants)
     DevArray<uint32_t> rec_meta, clip_queue, clip_base, clip_order;
     DevArray<uint32_t> bin_count, bin_start, part_count, part_cursor, part_off, queue, queue2, entries, scan_blk;
     DevArray<uint16_t> order_bkt;   // per queue entry: its cost bucket
@@ -280,10 +282,10 @@ int flush_pending_clear(srg_ctx* ctx, srg_fb* fb, uint32_t plane)
 
 // Attribute slots the shader reads (Viewer/Shaders.h): the device keeps
 // planes only for these; the others never influence the output.
-int shader_slots(uint32_t shader, uint32_t uv_offset, uint32_t apt, uint32_t slot_attr[kMaxSlots], uint32_t& nslots,
+int shader_slots(uint32_t shader, uint32_t uv_offset, uint32_t apt, uint32_t slot_attr[kWideSlots], uint32_t& nslots,
                  uint32_t& deriv_slot)
 {
-    for (uint32_t k = 0; k < kMaxSlots; ++k) slot_attr[k] = 0;
+    for (uint32_t k = 0; k < kWideSlots; ++k) slot_attr[k] = 0;
     switch (shader) {
     case SRG_SHADER_UNLIT_DIFFUSE:
         if (apt < 8 || uv_offset + 1 >= apt) return SRG_ERR_INVALID;
@@ -314,10 +316,20 @@ int shader_slots(uint32_t shader, uint32_t uv_offset, uint32_t apt, uint32_t slo
         nslots = 2;
         deriv_slot = 0;
         return SRG_OK;
+    case SRG_SHADER_SPONZA:
+        // every varying of an Obj::Vertex (32 bytes): slot k = attribute k
+        if (apt != 8 || uv_offset + 1 >= apt) return SRG_ERR_INVALID;
+        for (uint32_t k = 0; k < kWideSlots; ++k) slot_attr[k] = k;
+        nslots = kWideSlots;
+        deriv_slot = uv_offset;
+        return SRG_OK;
     default:
         return SRG_ERR_UNKNOWN_SHADER;
     }
 }
+
+// Shaders that read their uniform as a texture (UnlitDiffuseShader, SponzaShader).
+bool shader_samples(uint32_t shader) { return shader == SRG_SHADER_UNLIT_DIFFUSE || shader == SRG_SHADER_SPONZA; }
 
 struct ViewKey {
     srg_fb* fb;
@@ -413,8 +425,13 @@ template <uint32_t S, uint32_t H>
 void launch_backend_t(srg_ctx* ctx, uint32_t wg, bool stats, const FrameDev& F)
 {
     const dim3 grid((uint32_t)ctx->num_cus * wg), block(backend_threads(S, H));
-    if (stats) hipLaunchKernelGGL((k_backend<S, H, true>), grid, block, 0, ctx->stream, F);
-    else hipLaunchKernelGGL((k_backend<S, H, false>), grid, block, 0, ctx->stream, F);
+    if (F.rec_ext) {
+        if (stats) hipLaunchKernelGGL((k_backend<S, H, true, true>), grid, block, 0, ctx->stream, F);
+        else hipLaunchKernelGGL((k_backend<S, H, false, true>), grid, block, 0, ctx->stream, F);
+    } else {
+        if (stats) hipLaunchKernelGGL((k_backend<S, H, true, false>), grid, block, 0, ctx->stream, F);
+        else hipLaunchKernelGGL((k_backend<S, H, false, false>), grid, block, 0, ctx->stream, F);
+    }
 }
 
 int launch_backend(srg_ctx* ctx, BackendLayout L, const FrameDev& F)
@@ -547,6 +564,7 @@ int run_frame(srg_ctx* ctx)
     std::unordered_map<uint64_t, uint32_t> tex_index;
     uint64_t total_tris = 0;
     uint32_t nblocks = 0;
+    bool wide = false;  // a draw keeps more than kMaxSlots attribute planes
     for (size_t i = 0; i < ctx->draws.size(); ++i) {
         const srg_draw& s = ctx->draws[i];
         DrawDev& d = ddev[i];
@@ -568,8 +586,9 @@ int run_frame(srg_ctx* ctx)
         d.shader = s.shader;
         int st = shader_slots(s.shader, s.uv_offset, s.attribute_stride / 4u, d.slot_attr, d.nslots, d.deriv_slot);
         if (st) return st;
+        wide = wide || d.nslots > kMaxSlots;
         d.tex = kNoTex;
-        if (s.shader == SRG_SHADER_UNLIT_DIFFUSE && s.texture) {
+        if (shader_samples(s.shader) && s.texture) {
             auto it = tex_index.find(s.texture);
             if (it == tex_index.end()) {
                 const Texture& t = ctx->textures[s.texture];
@@ -600,12 +619,14 @@ int run_frame(srg_ctx* ctx)
     const size_t o_draw = off; off = align_up(off + sizeof(DrawDev) * std::max<size_t>(ddev.size(), 1), 256);
     const size_t o_view = off; off = align_up(off + sizeof(ViewDev) * vdev.size(), 256);
     const size_t o_tex = off;  off = align_up(off + sizeof(TexDev) * std::max<size_t>(tdev.size(), 1), 256);
+    const size_t o_sponza = off; off = align_up(off + (wide ? sizeof(SponzaDev) : 0), 256);
     int st = ensure_host_tables(ctx, off);
     if (st) return st;
     if ((st = ctx->tables.ensure(off))) return st;
     if (!ddev.empty()) std::memcpy(ctx->h_tables + o_draw, ddev.data(), sizeof(DrawDev) * ddev.size());
     std::memcpy(ctx->h_tables + o_view, vdev.data(), sizeof(ViewDev) * vdev.size());
     if (!tdev.empty()) std::memcpy(ctx->h_tables + o_tex, tdev.data(), sizeof(TexDev) * tdev.size());
+    if (wide) std::memcpy(ctx->h_tables + o_sponza, &ctx->sponza, sizeof(SponzaDev));
 
     // ---- arenas ----
     const uint32_t TB = (uint32_t)std::max<uint64_t>(total_bins, 1);
@@ -631,7 +652,7 @@ int run_frame(srg_ctx* ctx)
         const uint64_t nrec = (uint64_t)NT + want_clip;
         if ((st = ctx->rec_raster.ensure(nrec * 4)) || (st = ctx->rec_shade.ensure(nrec * 4)) ||
             (st = ctx->rec_meta.ensure(nrec)) || (st = ctx->clip_order.ensure(want_clip)) ||
-            (st = ctx->entries.ensure(want_entries)))
+            (st = ctx->entries.ensure(want_entries)) || (wide && (st = ctx->rec_ext.ensure(nrec * 3))))
             return st;
         ctx->clip_cap = (uint32_t)want_clip;
         ctx->entry_cap = (uint32_t)want_entries;
@@ -651,6 +672,8 @@ int run_frame(srg_ctx* ctx)
         F.entry_cap = ctx->entry_cap;
         F.rec_raster = ctx->rec_raster.ptr;
         F.rec_shade = ctx->rec_shade.ptr;
+        F.rec_ext = wide ? ctx->rec_ext.ptr : nullptr;
+        F.sponza = wide ? (const SponzaDev*)(ctx->tables.ptr + o_sponza) : nullptr;
         F.rec_meta = ctx->rec_meta.ptr;
         F.clip_queue = ctx->clip_queue.ptr;
         F.clip_base = ctx->clip_base.ptr;
@@ -698,8 +721,13 @@ int run_frame(srg_ctx* ctx)
                     if (v.primid)
                         hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, ctx->stream, v.primid, 0xFFFFFFFFu,
                                            (size_t)v.tiles_x * v.tiles_y * kTilePixels);
-            if (nblocks) hipLaunchKernelGGL(k_setup, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
-            if (NT) hipLaunchKernelGGL(k_clip, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
+            if (wide) {
+                if (nblocks) hipLaunchKernelGGL(k_setup<kWideSlots>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
+                if (NT) hipLaunchKernelGGL(k_clip<kWideSlots>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
+            } else {
+                if (nblocks) hipLaunchKernelGGL(k_setup<kMaxSlots>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
+                if (NT) hipLaunchKernelGGL(k_clip<kMaxSlots>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
+            }
             FrameDev Fs = F;  // with ordering, the natural queue goes to the spare buffer first
             if (ordered) Fs.queue = ctx->queue2.ptr;
             hipLaunchKernelGGL(k_scan_blocks, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs, ctx->scan_blk.ptr);
@@ -891,6 +919,7 @@ void srg_destroy(srg_ctx* ctx)
     ctx->tables.release();
     ctx->rec_raster.release();
     ctx->rec_shade.release();
+    ctx->rec_ext.release();
     ctx->rec_meta.release();
     ctx->clip_queue.release();
     ctx->clip_base.release();
@@ -1157,10 +1186,10 @@ int srg_draw_indexed(srg_ctx* ctx, const srg_draw* d)
     if (d->attribute_stride == 0 || d->attribute_stride > 32 || (d->attribute_stride & 3) || (d->attribute_offset & 3))
         return SRG_ERR_INVALID;
     if (d->index_offset % d->index_stride) return SRG_ERR_INVALID;
-    uint32_t sa[kMaxSlots], ns, ds;
+    uint32_t sa[kWideSlots], ns, ds;
     int st = shader_slots(d->shader, d->uv_offset, d->attribute_stride / 4u, sa, ns, ds);
     if (st) return st;
-    if (d->shader == SRG_SHADER_UNLIT_DIFFUSE && d->texture && !ctx->textures.count(d->texture)) return SRG_ERR_HANDLE;
+    if (shader_samples(d->shader) && d->texture && !ctx->textures.count(d->texture)) return SRG_ERR_HANDLE;
     const Buffer& ib = ctx->buffers[d->index_buffer];
     const Buffer& pb = ctx->buffers[d->position_buffer];
     const Buffer& ab = ctx->buffers[d->attribute_buffer];
@@ -1259,6 +1288,14 @@ int srg_get_frame_stats(srg_ctx* ctx, srg_frame_stats* out)
 {
     if (!ctx || !out) return SRG_ERR_INVALID;
     *out = ctx->stats;
+    return SRG_OK;
+}
+
+int srg_set_sponza_constants(srg_ctx* ctx, const srg_sponza_constants* k)
+{
+    static_assert(sizeof(srg_sponza_constants) == sizeof(SponzaDev), "SponzaScene::Constants layout");
+    if (!ctx || !k) return SRG_ERR_INVALID;
+    std::memcpy(&ctx->sponza, k, sizeof(SponzaDev));
     return SRG_OK;
 }
 

@@ -303,3 +303,31 @@ def combined_views(seed=SEED_C5, nviews=64, width=2560, height=1440, ntris=2_000
         views = list(ex.map(make, range(nviews)))
         texs = list(ex.map(lambda t: value_noise_rgba8(seed ^ (0x7E470000 + t), tex_size, tex_size), range(ntex)))
     return ViewSet(views, texs, {"config": "combined_views", "nviews": nviews})
+
+
+def sponza_constants(seed=0, near=False):
+    """SponzaShader constants as SponzaScene::Init sets them up
+    (Viewer/SponzaScene.cpp:127-160): ambient 0.1, the sun direction
+    normalize(0.4, 0.7, 0.1) whose x component fills all three lanes (the
+    reference assigns m_sunDir[0..2] from sunDir.x), 16 lights with random
+    colours, intensity in [150, 350], falloff in [500, 2500] and positions
+    x in [-1000, 1000], y in [50, 250], z in [-150, 150] (the reference's
+    kt::XorShift32 stream is not reproduced: these are the same ranges from
+    this module's generator). near=True instead puts the lights inside the
+    view volume of the random test scenes with intensity in [0.5, 5], so
+    that many pixels saturate."""
+    from softrast_amd import SponzaConstants
+    rng = np.random.default_rng(0x5B0A2A00 + seed)
+    s = np.array([0.4, 0.7, 0.1], np.float32)
+    s = (s / np.float32(np.sqrt(np.float32(s @ s)))).astype(np.float32)
+    lights = []
+    for _ in range(16):
+        colour = rng.uniform(0.0, 1.0, 3)
+        if near:
+            pos = (rng.uniform(-12, 12), rng.uniform(-7, 7), rng.uniform(6, 40))
+            intensity = rng.uniform(0.5, 5.0)
+        else:
+            pos = (rng.uniform(-1000, 1000), rng.uniform(50, 250), rng.uniform(-150, 150))
+            intensity = rng.uniform(150.0, 350.0)
+        lights.append((pos, tuple(colour), intensity, rng.uniform(500.0, 2500.0)))
+    return SponzaConstants(sun_dir=(s[0], s[0], s[0]), ambient=(0.1, 0.1, 0.1), lights=lights)

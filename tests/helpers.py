@@ -69,10 +69,13 @@ def build_textures(scene):
     return out
 
 
-def oracle_render(scene, texdata, nthreads=1, clear=0, plane_init=None, fast=False, tile_entries=False):
-    """Render one frame of `scene` with the oracle; returns (plane, stats)."""
+def oracle_render(scene, texdata, nthreads=1, clear=0, plane_init=None, fast=False, tile_entries=False, sponza=None):
+    """Render one frame of `scene` with the oracle; returns (plane, stats).
+    sponza: SponzaConstants for SponzaShader draws."""
     from oracle.oracle import Oracle, Plane, make_texture
     o = Oracle(nthreads=nthreads, fast=fast)
+    if sponza is not None:
+        o.set_sponza_constants(sponza.to_bytes())
     p = Plane(scene.width, scene.height)
     if plane_init is not None:
         p.colour[:] = plane_init[0].reshape(-1)
@@ -92,9 +95,10 @@ def oracle_render(scene, texdata, nthreads=1, clear=0, plane_init=None, fast=Fal
     return p, stats
 
 
-def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, fb=None, **ctx_kw):
+def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, fb=None, sponza=None, **ctx_kw):
     """Render one frame on the GPU through the C ABI; returns (colour, depth, primid, stats).
-    With fb given, renders into it and returns the stats only."""
+    With fb given, renders into it and returns the stats only. sponza:
+    SponzaConstants set before the frame."""
     import softrast_amd as sr
     if flags is None:
         flags = sr.CFG_KEEP_PRIMID | sr.CFG_STATS
@@ -106,6 +110,8 @@ def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, f
         fb = sr.FrameBuffer(ctx, scene.width, scene.height)
     if plane_init is not None:
         fb.write_tiles(plane_init[0], plane_init[1])
+    if sponza is not None:
+        ctx.set_sponza_constants(sponza)
     ctx.BeginFrame()
     if clear is not None:
         ctx.ClearFrameBuffer(fb, clear)
@@ -115,7 +121,8 @@ def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, f
         call.SetIndexBuffer(d.indices, d.indices.dtype.itemsize, len(d.indices))
         call.SetPositionBuffer(d.vertices, ATTR_STRIDE, len(d.vertices))
         call.SetAttributeBuffer(d.vertices, ATTR_STRIDE, len(d.vertices), d.uv_offset)
-        shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader}[d.shader]
+        shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader,
+                  3: sr.SponzaShader}[d.shader]
         call.SetPixelShader(shader, texdata[d.texture] if d.texture >= 0 else None)
         ctx.DrawIndexed(call)
     ctx.EndFrame()

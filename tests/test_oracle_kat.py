@@ -89,6 +89,106 @@ def test_rcpps_fixture_matches_host_if_intel(tmp_path):
     assert [int(w, 16) for w in out] == _fixture_table()
 
 
+# ---------------------------------------------------------------- rsqrtps
+
+def _rsqrt_fixture():
+    return [int(w, 16) for w in (ROOT / "tests/golden/rsqrtps_table.txt").read_text().split()]
+
+
+def test_rsqrtps_table_files_agree():
+    tab = _rsqrt_fixture()
+    assert len(tab) == 4096
+    for inc in ("oracle/rsqrtps_table.inc", "softrast_amd/csrc/rsqrtps_table.inc"):
+        words = [int(t.strip().rstrip("u"), 16) for t in (ROOT / inc).read_text().split("*/")[-1].split(",")
+                 if t.strip()]
+        assert words == tab, inc
+
+
+def test_rsqrtps_within_intel_bound(lib):
+    # Intel documents |rel err| <= 1.5 * 2^-12 for RSQRTPS; checked over both
+    # table ranges [1, 2) and [2, 4), every table cell at its two ends.
+    for i in range(4096):
+        base = 1.0 + (i % 2048) / 2048.0 if i < 2048 else 2.0 + (i - 2048) / 1024.0
+        step = 1.0 / 2048.0 if i < 2048 else 1.0 / 1024.0
+        for x in (base, f32(base + step * (1.0 - 2.0 ** -12))):
+            r = lib.sro_rsqrtps(F(x))
+            assert abs(r * np.sqrt(x) - 1.0) <= 1.5 * 2.0 ** -12, (i, x, r)
+
+
+def test_rsqrtps_exponent_scaling_and_specials(lib):
+    """rsqrt(x * 4^k) = rsqrt(x) * 2^-k exactly, for every even scaling that
+    keeps input and result normal; the special inputs as the SDM defines them."""
+    rng = np.random.default_rng(2)
+    for _ in range(3000):
+        x = f32(rng.uniform(1.0, 4.0))
+        k = int(rng.integers(-62, 63))
+        assert lib.sro_rsqrtps(F(f32(x * 4.0 ** k))) == f32(lib.sro_rsqrtps(F(x)) * 2.0 ** -k)
+    assert bits(lib.sro_rsqrtps(F(0.0))) == 0x7F800000
+    assert bits(lib.sro_rsqrtps(F(-0.0))) == 0xFF800000
+    assert bits(lib.sro_rsqrtps(F(from_bits(0x00000100)))) == 0x7F800000   # denormal input
+    assert bits(lib.sro_rsqrtps(F(from_bits(0x80000100)))) == 0xFF800000
+    assert bits(lib.sro_rsqrtps(F(float("inf")))) == 0
+    assert bits(lib.sro_rsqrtps(F(-1.0))) == 0xFFC00000
+    assert bits(lib.sro_rsqrtps(F(float("-inf")))) == 0xFFC00000
+    assert lib.sro_rsqrtps(F(4.0)) == lib.sro_rsqrtps(F(1.0)) * 0.5
+    # largest and smallest normal inputs stay normal (no flush at either end)
+    assert 0.0 < lib.sro_rsqrtps(F(from_bits(0x7F7FFFFF))) < 1e-18
+    assert lib.sro_rsqrtps(F(from_bits(0x00800000))) > 1e18
+
+
+_HOST_RSQRT = r"""
+#include <immintrin.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+int main(void) {
+    uint32_t u; float f, r;
+    while (fread(&u, 4, 1, stdin) == 1) {
+        memcpy(&f, &u, 4);
+        r = _mm_cvtss_f32(_mm_rsqrt_ps(_mm_set1_ps(f)));
+        memcpy(&u, &r, 4);
+        fwrite(&u, 4, 1, stdout);
+    }
+    return 0;
+}
+"""
+
+
+def _intel_host():
+    info = pathlib.Path("/proc/cpuinfo").read_text() if pathlib.Path("/proc/cpuinfo").exists() else ""
+    return "GenuineIntel" in info
+
+
+def test_rsqrtps_matches_host_instruction(lib, tmp_path):
+    """The oracle against the host's RSQRTPS on 200k random bit patterns
+    (every exponent, both signs, NaNs, denormals) -- the instruction itself
+    is the expected answer, not the table model."""
+    if not _intel_host():
+        pytest.skip("RSQRTPS table was recorded on an Intel host; other vendors may differ")
+    src, exe = tmp_path / "h.c", tmp_path / "h"
+    src.write_text(_HOST_RSQRT)
+    subprocess.run(["gcc", "-O2", "-msse2", str(src), "-o", str(exe)], check=True)
+    u = np.random.default_rng(3).integers(0, 2 ** 32, 200_000, dtype=np.uint64).astype(np.uint32)
+    host = np.frombuffer(subprocess.run([str(exe)], input=u.tobytes(), capture_output=True, check=True).stdout,
+                         np.uint32)
+    f = u.view(np.float32)
+    ours = np.array([bits(lib.sro_rsqrtps(F(float(x)))) for x in f], np.uint32)
+    # ctypes passes a signalling NaN through a double and quiets it; compare those by quiet bit only
+    snan = ((u & 0x7F800000) == 0x7F800000) & ((u & 0x7FFFFF) != 0)
+    assert np.array_equal(ours[~snan], host[~snan])
+    assert np.array_equal(ours[snan] | 0x00400000, host[snan] | 0x00400000)
+
+
+def test_rsqrtps_fixture_matches_host_if_intel(tmp_path):
+    if not _intel_host():
+        pytest.skip("RSQRTPS table was recorded on an Intel host; other vendors may differ")
+    exe = tmp_path / "gen"
+    subprocess.run(["gcc", "-O2", "-msse2", str(ROOT / "tests/golden/gen_rsqrtps_table.c"), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert [int(w, 16) for w in out] == _rsqrt_fixture()
+
+
 # ---------------------------------------------------------------- pack / morton
 
 @pytest.mark.parametrize("c,byte", [

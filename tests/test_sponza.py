@@ -1,0 +1,224 @@
+"""SponzaShader (Viewer/SponzaScene.cpp:13-104): the oracle pinned by closed
+forms on the CPU, then the GPU against the oracle.
+
+The closed forms draw a viewport-filling quad whose attributes are the same
+at every vertex (position P, normal N, uv), so every pixel sees the same
+varyings and the expected colour follows from the shader's formula in
+float64: sun term max(0.1, N.sun), each light's N.L * intensity /
+(1 + 0.1 d + 0.01 d^2) * colour, ambient, times the texel. The inputs are
+chosen so that each term moves the result by many LSBs (a dropped or
+swapped term fails); the tolerance is 1 LSB, from RSQRTPS / RCPPS
+(|rel err| <= 1.5 * 2^-12) and the texel lerp's rounding."""
+import sys
+
+import numpy as np
+import pytest
+
+import softrast_amd as sr
+from softrast_amd import scenes
+from tests import helpers
+
+TEXEL = (200, 150, 100, 255)
+
+
+def _consts(lights=(), **kw):
+    """SponzaConstants whose unused lights sit far away with zero intensity
+    (a light at the surface point itself would make the result NaN)."""
+    far = [((1e3, 1e3 + i, 1e3), (0.0, 0.0, 0.0), 0.0, 0.0) for i in range(16 - len(lights))]
+    return sr.SponzaConstants(lights=list(lights) + far, **kw)
+
+
+def _const_texture(rgba=TEXEL, size=32):
+    img = np.empty((size, size, 4), np.uint8)
+    img[:] = rgba
+    return sr.TextureData().CreateFromRGBA8(img, size, size, False)
+
+
+def _quad_draw(tex, P, N, uv=(0.3, 0.6), uv_offset=6):
+    from oracle.oracle import make_texture
+    pos = np.array([[-1, -1, 0.5], [1, -1, 0.5], [1, 1, 0.5], [-1, 1, 0.5]], np.float32)
+    attr = np.zeros((4, 8), np.float32)
+    attr[:, 0:3] = P
+    attr[:, 3:6] = N
+    attr[:, uv_offset:uv_offset + 2] = uv
+    idx = np.array([0, 1, 2, 0, 2, 3, 0, 2, 1, 0, 3, 2], np.uint32)  # both windings: one of each pair is drawn
+    st = make_texture(tex)[0] if tex is not None else None
+    return dict(shader=3, tex=st, idx=idx, idx_stride=4, pos=pos, pos_stride=12, pos_num=4, attr=attr,
+                attr_stride=32, attr_num=4, uv_offset=uv_offset, plane=0, mvp=np.eye(4, dtype=np.float32)), tex
+
+
+def _render_const(consts, P, N, tex=TEXEL, uv_offset=6):
+    """Every pixel of a 64x64 frame with one set of varyings; returns the
+    distinct colours (must be one)."""
+    from oracle.oracle import Oracle, Plane
+    o = Oracle(nthreads=1)
+    o.set_sponza_constants(consts.to_bytes())
+    p = Plane(64, 64)
+    o.clear(p, 0)
+    t = _const_texture(tex) if tex is not None else None
+    draw, keep = _quad_draw(t, P, N, uv_offset=uv_offset)
+    o.render([draw], [p])
+    o.close()
+    vals = np.unique(p.colour)
+    assert len(vals) == 1, vals[:8]
+    return int(vals[0])
+
+
+def _expect(rad, tex=TEXEL):
+    """Packed RGBA8 of rad * texel / 255 (RNE(fma(c, 255, 0.5)) per channel)."""
+    out = []
+    for c in range(3):
+        out.append(int(np.rint(rad[c] * tex[c] / 255.0 * 255.0 + 0.5)))
+    out.append(int(np.rint(tex[3] / 255.0 * 255.0 + 0.5)) if tex[3] < 255 else 255)
+    return out
+
+
+def _bytes(u):
+    return [(u >> (8 * k)) & 255 for k in range(4)]
+
+
+def _close(got, want):
+    g = _bytes(got)
+    assert all(abs(a - b) <= 1 for a, b in zip(g, want)), (g, want)
+
+
+def test_sun_and_ambient_only():
+    """No light intensity: rad = max(0.1, N.sun) + ambient per channel."""
+    n = np.array([0.6, 0.0, 0.8])
+    k = _consts(sun_dir=(0.5, 0.5, 0.5), ambient=(0.05, 0.1, 0.2))
+    got = _render_const(k, (1, 2, 3), n)
+    ndl = max(0.1, float(n @ np.array([0.5, 0.5, 0.5])))   # 0.7
+    _close(got, _expect([ndl + 0.05, ndl + 0.1, ndl + 0.2]))
+    # a normal facing away: the 0.1 bias holds the sun term up
+    got = _render_const(k, (1, 2, 3), -n)
+    _close(got, _expect([0.1 + 0.05, 0.1 + 0.1, 0.1 + 0.2]))
+
+
+def test_one_light_attenuation():
+    """Light at distance 10 (d^2 = 100): atten = 1 / (1 + 1 + 1) = 1/3,
+    N.L = 0.8; every term of the attenuation moves the red byte by > 20."""
+    k = _consts(ambient=(0.05, 0.1, 0.2), lights=[((0.0, 6.0, 8.0), (1.0, 0.5, 0.25), 0.9, 1000.0)])
+    got = _render_const(k, (0, 0, 0), (0, 0, 1))
+    lr = 0.8 * 0.9 / 3.0
+    _close(got, _expect([0.1 + lr * 1.0 + 0.05, 0.1 + lr * 0.5 + 0.1, 0.1 + lr * 0.25 + 0.2]))
+
+
+def test_sixteen_lights_sum():
+    rng = np.random.default_rng(5)
+    P = np.array([1.5, -2.0, 4.0])
+    N = np.array([0.0, 0.6, 0.8])
+    lights = []
+    rad = np.full(3, max(0.1, float(N @ np.array([0.3, 0.3, 0.3]))))
+    for i in range(16):
+        d = rng.normal(0, 1, 3)
+        d /= np.linalg.norm(d)
+        dist = 2.0 + 3.0 * i
+        L = (P + d * dist).astype(np.float32)
+        col = rng.uniform(0, 1, 3).astype(np.float32)
+        inten = np.float32(rng.uniform(0.05, 0.4))
+        lights.append((tuple(L), tuple(col), float(inten), 0.0))
+        t = L.astype(np.float64) - P
+        dd = np.linalg.norm(t)
+        nl = max(0.0, float((t / dd) @ N))
+        rad += nl * inten / (1.0 + 0.1 * dd + 0.01 * dd * dd) * col
+    rad += np.array([0.02, 0.03, 0.04])
+    k = sr.SponzaConstants(sun_dir=(0.3, 0.3, 0.3), ambient=(0.02, 0.03, 0.04), lights=lights)
+    _close(_render_const(k, P, N), _expect(list(rad)))
+
+
+def test_light_at_the_surface_point_is_nan():
+    """d^2 = 0: RSQRTPS(0) = inf, L = 0 * inf = NaN; max(0, NaN) returns the
+    NaN (second operand), so the colour channels pack to 0; alpha is the
+    texel's."""
+    k = _consts(lights=[((1.0, 2.0, 3.0), (1.0, 1.0, 1.0), 1.0, 0.0)])
+    assert _render_const(k, (1, 2, 3), (0, 0, 1)) == 0xFF000000
+
+
+def test_no_texture_is_white():
+    k = _consts(ambient=(0.5, 0.5, 0.5))
+    assert _render_const(k, (0, 0, 0), (0, 0, 1), tex=None) == 0xFFFFFFFF
+
+
+def test_uv_offset_moves_the_sampled_pair():
+    """uv at attributes 2-3 instead of 6-7 (positions then overlap): only
+    the sample and its derivatives move; a constant texture hides that, so
+    this pins that the shader still reads the eight varyings as the
+    reference lays them out."""
+    k = _consts(ambient=(0.1, 0.1, 0.1))
+    got = _render_const(k, (0, 0, 0), (0, 0, 1), uv_offset=2)
+    _close(got, _expect([0.2, 0.2, 0.2]))
+
+
+def test_attribute_stride_must_be_32():
+    from oracle.oracle import Oracle, Plane
+    o = Oracle(nthreads=1)
+    p = Plane(64, 64)
+    draw, keep = _quad_draw(None, (0, 0, 0), (0, 0, 1))
+    draw["attr"] = draw["attr"][:, :6].copy()
+    draw["attr_stride"] = 24
+    with pytest.raises(RuntimeError):
+        o.render([draw], [p])
+    o.close()
+
+
+# ---------------------------------------------------------------- GPU parity
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _sponza_scene(seed, ntris, w, h, clip_heavy=False, uv_offset=6, mix=False):
+    shaders = (3, 0) if mix else (3,)
+    return helpers.random_scene(seed, ntris, w, h, clip_heavy=clip_heavy, shaders=shaders, uv_offset=uv_offset,
+                                ndraws=2 if mix else 1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", [
+    dict(seed=300, ntris=20000, w=640, h=360, near=False),
+    dict(seed=301, ntris=20000, w=640, h=360, near=True),
+    dict(seed=302, ntris=12000, w=320, h=240, near=True, clip_heavy=True),
+    dict(seed=303, ntris=12000, w=320, h=240, near=False, mix=True),
+    dict(seed=304, ntris=8000, w=320, h=240, near=True, uv_offset=2),
+])
+def test_gpu_sponza_matches_oracle(case):
+    """SponzaShader frames, bit-exact against the oracle: the reference's
+    own light set-up (far lights, SponzaScene::Init ranges), near lights
+    that saturate and hit the packus quirks, clipped triangles (the wide
+    clipper), a frame mixing SponzaShader and UnlitDiffuse draws, and uv at
+    attributes 2-3."""
+    sc = _sponza_scene(case["seed"], case["ntris"], case["w"], case["h"], case.get("clip_heavy", False),
+                       case.get("uv_offset", 6), case.get("mix", False))
+    k = scenes.sponza_constants(case["seed"], near=case["near"])
+    _say(f"[sponza] case={case}")
+    tex = helpers.build_textures(sc)
+    ref, rstats = helpers.oracle_render(sc, tex, nthreads=8, sponza=k)
+    col, dep, pid, st = helpers.gpu_render(sc, tex, sponza=k)
+    d = helpers.compare(ref, col, dep, pid)
+    _say(f"[sponza] diffs={d}")
+    assert not any(d.values()), d
+    assert st["frags_raster"] == rstats["frags_raster"]
+    lit = np.count_nonzero(ref.primid != 0xFFFFFFFF)
+    assert lit > 1000
+
+
+@pytest.mark.gpu
+def test_gpu_sponza_then_narrow_frame():
+    """A context that renders a SponzaShader frame (eight planes per record)
+    and then an UnlitDiffuse frame (four) gets both right: the record
+    layout follows each frame's draws."""
+    k = scenes.sponza_constants(7, near=False)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS)
+    for sc in (_sponza_scene(310, 6000, 320, 240), helpers.random_scene(311, 6000, 320, 240),
+               _sponza_scene(312, 6000, 320, 240, clip_heavy=True)):
+        tex = helpers.build_textures(sc)
+        ref, _ = helpers.oracle_render(sc, tex, nthreads=8, sponza=k)
+        fb = sr.FrameBuffer(ctx, sc.width, sc.height)
+        helpers.gpu_render(sc, tex, ctx=ctx, fb=fb, sponza=k)
+        col, dep, pid = fb.read_tiles(primid=True)
+        d = helpers.compare(ref, col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
+        _say(f"[sponza] sequence diffs={d}")
+        assert not any(d.values()), d
+        fb.destroy()
+    ctx.Shutdown()
