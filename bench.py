@@ -58,10 +58,11 @@ def parse():
                     help="tile-order study on the clustered 3840x2160 / 4M-triangle workload instead of the default")
     ap.add_argument("--no-plain", action="store_true",
                     help="profiling aid: time only the batched path (--steps launches) and print its rate")
-    ap.add_argument("--workload", choices=("spheres", "mixed", "combined"), default="spheres",
+    ap.add_argument("--workload", choices=("spheres", "mixed", "combined", "sponza"), default="spheres",
                     help="mixed: 256 mixed-resolution views per EndFrame under the autotuner; combined: 64 views "
-                         "of 2560x1440 x 2M triangles, learned vs tri-count order (the default is the 1080p/1M "
-                         "workload)")
+                         "of 2560x1440 x 2M triangles, learned vs tri-count order; sponza: the 1080p/1M frame "
+                         "drawn with the SponzaShader (16 point lights) against UnlitDiffuse (the default is the "
+                         "1080p/1M workload)")
     ap.add_argument("--nviews", type=int, default=0, help="mixed/combined: views per batch (0 = the workload's)")
     return ap.parse_args()
 
@@ -72,7 +73,7 @@ def view_mvp(scene, yaw_deg):
     return scenes.camera_mvp((0.0, 0.0, 0.0), (math.sin(t), 0.0, math.cos(t)), 60.0, scene.width / scene.height)
 
 
-def make_calls(sr, fb, scene, texs, mvp):
+def make_calls(sr, fb, scene, texs, mvp, shader=None):
     calls = []
     for d in scene.draws:
         c = sr.DrawCall()
@@ -80,23 +81,27 @@ def make_calls(sr, fb, scene, texs, mvp):
         c.SetIndexBuffer(d.indices, 4, len(d.indices))
         c.SetPositionBuffer(d.vertices, 32, len(d.vertices))
         c.SetAttributeBuffer(d.vertices, 32, len(d.vertices), 6)
-        c.SetPixelShader(sr.UnlitDiffuseShader, texs[d.texture])
+        c.SetPixelShader(shader or sr.UnlitDiffuseShader, texs[d.texture])
         calls.append(c)
     return calls
 
 
-def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup):
+def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup, shader=None, sponza=None):
     """Returns (frames/s, ms per EndFrame, per-frame stats, ctx, framebuffers).
     Untimed frames first: `warmup`, and with the autotuner or the learned
-    order, until the layout is chosen and the model trained on it."""
+    order, until the layout is chosen and the model trained on it. shader:
+    the pixel shader of every draw (UnlitDiffuse by default); sponza: the
+    SponzaShader constants."""
     ctx = sr.RenderContext(flags=flags, tile_order=tile_order, bin_size=bin_size, autotune_frames=3)
+    if sponza is not None:
+        ctx.set_sponza_constants(sponza)
     fbs, calls = [], []
     for v in range(views):
         fb = sr.FrameBuffer(ctx, scene.width, scene.height)
         # cameras turned by up to +-2 degrees: every view sees nearly the plain view's scene
         mvp = scene.mvp if views == 1 else view_mvp(scene, (v - (views - 1) / 2.0) * 4.0 / (views - 1))
         fbs.append(fb)
-        calls.append(make_calls(sr, fb, scene, texs, mvp))
+        calls.append(make_calls(sr, fb, scene, texs, mvp, shader))
 
     draws = ctx.PrepareDrawList([c for cs in calls for c in cs])  # uploads every buffer and texture
 
@@ -407,6 +412,94 @@ def batch_workload(sr, args):
     print(json.dumps(out))
 
 
+def sponza_workload(sr, args):
+    """--workload sponza: the 1080p / 1M-triangle frame with every draw on
+    the SponzaShader (sun, 16 point lights placed as SponzaScene::Init
+    places them, ambient; eight planes per primitive), timed as the plain
+    path and as 64-view batches, beside the same frame on UnlitDiffuse; the
+    plain frame is compared with the oracle and the oracle is timed on the
+    host as the CPU baseline."""
+    from softrast_amd import scenes
+    t0 = time.time()
+    scene = scenes.spheres_scene()
+    texs = [sr.TextureData().CreateFromRGBA8(t, t.shape[1], t.shape[0], True) for t in scene.textures]
+    k = scenes.sponza_constants(0)
+    print(f"[bench] sponza scene ready: {scene.num_tris} tris, {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    res = {}
+    for name, shader in (("unlit", sr.UnlitDiffuseShader), ("sponza", sr.SponzaShader)):
+        fps, ms, st, ctx, fbs = time_path(sr, scene, texs, 1, sr.CFG_TIMING, sr.ORDER_NATURAL, args.bin_size,
+                                          args.steps, args.warmup, shader, k)
+        res[name] = {"frames_per_s": round(fps, 2), "ms_per_frame": round(ms, 4),
+                     "device_ms": {"frontend": round(statistics.mean(s["ms_frontend"] for s in st), 4),
+                                   "backend": round(statistics.mean(s["ms_backend"] for s in st), 4),
+                                   "total": round(statistics.mean(s["ms_total"] for s in st), 4)}}
+        if name == "sponza":
+            colour, depth = fbs[0].read_tiles()
+        for fb in fbs:
+            fb.destroy()
+        ctx.Shutdown()
+        print(f"[bench] plain {name}: {fps:.1f} frames/s", file=sys.stderr, flush=True)
+    bflags = sr.CFG_TIMING | sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
+    fps_b, ms_b, st_b, bctx, bfbs = time_path(sr, scene, texs, args.views, bflags, args.tile_order, args.bin_size,
+                                              max(20, args.steps // args.views), args.warmup, sr.SponzaShader, k)
+    layout = {"bin_size": st_b[-1]["bin_size"], "wg_per_tile": st_b[-1]["wg_per_tile"]}
+    dev_b = statistics.mean(s["ms_total"] for s in st_b)
+    for fb in bfbs:
+        fb.destroy()
+    bctx.Shutdown()
+    print(f"[bench] batched x{args.views} sponza: {fps_b:.1f} frames/s, layout {layout}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "frames/s at 1080p/1M tris, SponzaShader (16 point lights), batched+learned",
+        "value": round(fps_b, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_b, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+int32", "data": "synthetic (seeded spheres 0x5EED0002, value-noise textures, lights as "
+                                       "SponzaScene::Init ranges)",
+        "config": {"workload": "sponza_spheres_1080p_1m", "width": scene.width, "height": scene.height,
+                   "triangles_per_view": scene.num_tris, "shader": "SponzaShader",
+                   "path": f"batched x{args.views} views, hipGraph, "
+                           f"{('natural', 'tri-count', 'learned')[args.tile_order]} tile order"},
+        "batched": {"views": args.views, "frames_per_s": round(fps_b, 2), "device_ms_per_launch": round(dev_b, 4),
+                    "layout": layout},
+        "plain": res,
+        "sponza_vs_unlit_plain": round(res["sponza"]["frames_per_s"] / res["unlit"]["frames_per_s"], 3),
+    }
+    if not args.no_parity or not args.no_cpu_baseline:
+        from oracle.oracle import Oracle, Plane, make_texture
+        stex = [make_texture(t) for t in texs]
+        odraws = []
+        for d in scene.draws:
+            vf = d.vertices.view(np.float32).reshape(-1, 8)
+            odraws.append(dict(shader=sr.SHADER_SPONZA, tex=stex[d.texture][0], idx=d.indices, idx_stride=4, pos=vf,
+                               pos_stride=32, pos_num=len(vf), attr=vf, attr_stride=32, attr_num=len(vf), uv_offset=6,
+                               plane=0, mvp=scene.mvp))
+        nthreads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        orc = Oracle(nthreads=nthreads, fast=True)
+        orc.set_sponza_constants(k.to_bytes())
+        plane = Plane(scene.width, scene.height)
+        if not args.no_parity:
+            orc.clear(plane, 0)
+            orc.render(odraws, [plane])
+            dd = int(np.count_nonzero(plane.depth.view(np.uint32) != depth.reshape(-1).view(np.uint32)))
+            dc = int(np.count_nonzero(plane.colour != colour.reshape(-1)))
+            out["parity"] = {"depth_px_diff": dd, "rgba8_px_diff": dc, "vs": "oracle", "frame": "plain path"}
+        if not args.no_cpu_baseline:
+            times = []
+            for _ in range(3 + args.cpu_frames):
+                t = time.perf_counter()
+                orc.clear(plane, 0)
+                orc.render(odraws, [plane])
+                times.append(time.perf_counter() - t)
+            med = statistics.median(times[3:])
+            out["cpu_baseline"] = {"value": round(1.0 / med, 3), "unit": "frames/s", "cores": nthreads,
+                                   "kind": "port",
+                                   "sample": f"same 1080p/1M-tri SponzaShader frame (clear+render), median of "
+                                             f"{args.cpu_frames} frames after 3 warm-up, {nthreads} threads, "
+                                             f"oracle -O3 AVX2/FMA build"}
+            out["speedup_vs_cpu"] = round(fps_b * med, 1)
+        orc.close()
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -418,6 +511,8 @@ def main():
     from softrast_amd import scenes
     if args.sched_study:
         return sched_study(sr, args)
+    if args.workload == "sponza":
+        return sponza_workload(sr, args)
     if args.workload != "spheres":
         return batch_workload(sr, args)
 

@@ -92,6 +92,10 @@ namespace shader
 inline void UnlitDiffuseShader(void const*, Interpolants const&, uint32_t o_texels[8], uint32_t) { memset(o_texels, 0, 32); }
 inline void VisualizeNormalsShader(void const*, Interpolants const&, uint32_t o_texels[8], uint32_t) { memset(o_texels, 0, 32); }
 inline void VisualizeUVsShader(void const*, Interpolants const&, uint32_t o_texels[8], uint32_t) { memset(o_texels, 0, 32); }
+// SponzaShader (Viewer/SponzaScene.cpp:13-104; a file-static function there):
+// its constants come from RenderContext::SetSponzaConstants instead of the
+// g_constants global.
+inline void SponzaShader(void const*, Interpolants const&, uint32_t o_texels[8], uint32_t) { memset(o_texels, 0, 32); }
 }  // namespace shader
 
 class RenderContext;
@@ -225,6 +229,10 @@ public:
         return srg_begin_frame(m_ctx);
     }
 
+    // SponzaScene::Constants for frames ended after the call (the reference's
+    // g_constants global, written by SponzaScene::Init / Update).
+    int SetSponzaConstants(srg_sponza_constants const& _k) { return srg_set_sponza_constants(m_ctx, &_k); }
+
     int ClearFrameBuffer(FrameBuffer& _buffer, uint32_t _color = 0x00000000, bool _clearColour = true,
                          bool _clearDepth = true)
     {
@@ -238,9 +246,10 @@ public:
         if (_call.m_pixelShader == &shader::UnlitDiffuseShader) d.shader = SRG_SHADER_UNLIT_DIFFUSE;
         else if (_call.m_pixelShader == &shader::VisualizeNormalsShader) d.shader = SRG_SHADER_VISUALIZE_NORMALS;
         else if (_call.m_pixelShader == &shader::VisualizeUVsShader) d.shader = SRG_SHADER_VISUALIZE_UVS;
+        else if (_call.m_pixelShader == &shader::SponzaShader) d.shader = SRG_SHADER_SPONZA;
         else return SRG_ERR_UNKNOWN_SHADER;
         int st = 0;
-        if (d.shader == SRG_SHADER_UNLIT_DIFFUSE && _call.m_pixelUniforms) {
+        if ((d.shader == SRG_SHADER_UNLIT_DIFFUSE || d.shader == SRG_SHADER_SPONZA) && _call.m_pixelUniforms) {
             Tex::TextureData const* tex = static_cast<Tex::TextureData const*>(_call.m_pixelUniforms);
             if (!tex->m_texels.empty() && (st = Texture(tex, &d.texture))) return st;
         }

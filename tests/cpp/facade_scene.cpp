@@ -1,7 +1,9 @@
 // A host program written against the reference's frame API the way
 // Viewer/Scene.cpp:32-65 and Viewer/Main.cpp:41-86 use it, compiled against
-// include/sr/Renderer.h. Draws a small quad grid textured with a checker and
-// prints the number of non-clear pixels after the blit.
+// include/sr/Renderer.h. Draws a small quad grid textured with a checker
+// (UnlitDiffuseShader, then SponzaShader with constants set the way
+// SponzaScene::Init / Update set them) and prints the number of non-clear
+// pixels after the last blit.
 // Exit codes: 0 = rendered (or no GPU: prints "no-device"), 1 = failure.
 #include <cstdio>
 #include <vector>
@@ -48,6 +50,24 @@ int main()
     float m[16] = { 0.5f, 0, 0, 0, 0, 0.8f, 0, 0, 0, 0, 0.25f, 1, 0, 0, 0, 0 };
     for (int i = 0; i < 16; ++i) mvp.m[i] = m[i];
 
+    srg_sponza_constants sponza{};
+    for (int c = 0; c < 3; ++c) {
+        sponza.sun_dir[c] = 0.49f;
+        sponza.ambient[c] = 0.1f;
+    }
+    for (int i = 0; i < SRG_SPONZA_LIGHTS; ++i) {
+        srg_sponza_light& l = sponza.lights[i];
+        l.pos[0] = -3.0f + 0.4f * (float)i;
+        l.pos[1] = 1.0f;
+        l.pos[2] = 2.0f;
+        l.colour[0] = 1.0f;
+        l.colour[1] = 0.5f + 0.03f * (float)i;
+        l.colour[2] = 0.25f;
+        l.intensity = 0.5f;
+        l.falloff = 1000.0f;
+    }
+    if (int st = renderCtx.SetSponzaConstants(sponza)) { std::printf("SetSponzaConstants: %s\n", srg_status_string(st)); return 1; }
+
     std::vector<uint32_t> pixels(W * H);
     for (int frame = 0; frame < 2; ++frame) {
         renderCtx.BeginFrame();
@@ -63,7 +83,7 @@ int main()
         call.m_indexBuffer.m_num = (uint32_t)indices.size();
         call.m_indexBuffer.m_stride = sizeof(uint16_t);
         call.m_pixelUniforms = &tex;
-        call.m_pixelShader = sr::shader::UnlitDiffuseShader;
+        call.m_pixelShader = frame == 0 ? sr::shader::UnlitDiffuseShader : sr::shader::SponzaShader;
         if (int st = renderCtx.DrawIndexed(call)) { std::printf("DrawIndexed: %s\n", srg_status_string(st)); return 1; }
         if (int st = renderCtx.EndFrame()) { std::printf("EndFrame: %s\n", srg_status_string(st)); return 1; }
         if (int st = renderCtx.Blit(framebuffer, (uint8_t*)pixels.data())) { std::printf("Blit: %s\n", srg_status_string(st)); return 1; }
