@@ -95,20 +95,43 @@ __device__ __forceinline__ uint32_t shade_sponza(const FrameDev& F, const DrawDe
     // Dot3SoA (SIMDUtil.h:123-126): fma(x0, x1, fma(y0, y1, z0 * z1))
     const float ndl = maxps(0.1f, __builtin_fmaf(nx, K.sun_dir[0], __builtin_fmaf(ny, K.sun_dir[1], nz * K.sun_dir[2])));
     float r0 = ndl, r1 = ndl, r2 = ndl;
+    // Fast pass: while d^2 is a positive normal number, RSQRTPS's result and
+    // RCPPS's two inputs (1/d <= 2^63, 1 + 0.1 d + 0.01 d^2 < 2^122) are
+    // positive normal too, so the special-case-free forms give the same bits.
+    // A lane with any other d^2 (0, denormal, inf, NaN) is redone exactly.
+    bool special = false;
 #pragma unroll 4
     for (uint32_t i = 0; i < kSponzaLights; ++i) {
         const SponzaLightDev& L = K.lights[i];
         const float tx = L.pos[0] - var[0], ty = L.pos[1] - var[1], tz = L.pos[2] - var[2];
         const float dsq = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz));
-        const float rd = rsqrtps(dsq);
-        const float dist = rcpps(rd);
+        special = special || (fbits(dsq) - 0x00800000u) >= 0x7F000000u;
+        const float rd = rsqrtps_pos(dsq);
+        const float dist = rcpps_pos(rd);
         const float lx = tx * rd, ly = ty * rd, lz = tz * rd;
         const float nl = maxps(0.0f, __builtin_fmaf(lx, nx, __builtin_fmaf(ly, ny, lz * nz)));
-        const float atten = rcpps(1.0f + __builtin_fmaf(0.1f, dist, dsq * 0.01f));
+        const float atten = rcpps_pos(1.0f + __builtin_fmaf(0.1f, dist, dsq * 0.01f));
         const float lr = nl * (L.intensity * atten);
         r0 = r0 + lr * L.colour[0];
         r1 = r1 + lr * L.colour[1];
         r2 = r2 + lr * L.colour[2];
+    }
+    if (special) {
+        r0 = r1 = r2 = ndl;
+        for (uint32_t i = 0; i < kSponzaLights; ++i) {
+            const SponzaLightDev& L = K.lights[i];
+            const float tx = L.pos[0] - var[0], ty = L.pos[1] - var[1], tz = L.pos[2] - var[2];
+            const float dsq = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz));
+            const float rd = rsqrtps(dsq);
+            const float dist = rcpps(rd);
+            const float lx = tx * rd, ly = ty * rd, lz = tz * rd;
+            const float nl = maxps(0.0f, __builtin_fmaf(lx, nx, __builtin_fmaf(ly, ny, lz * nz)));
+            const float atten = rcpps(1.0f + __builtin_fmaf(0.1f, dist, dsq * 0.01f));
+            const float lr = nl * (L.intensity * atten);
+            r0 = r0 + lr * L.colour[0];
+            r1 = r1 + lr * L.colour[1];
+            r2 = r2 + lr * L.colour[2];
+        }
     }
     r0 = r0 + K.ambient[0];
     r1 = r1 + K.ambient[1];

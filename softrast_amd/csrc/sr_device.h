@@ -77,6 +77,25 @@ __device__ __forceinline__ float rsqrtps(float x)
     return ffrom(((uint32_t)re << 23) | (t & 0x7FFFFFu));
 }
 
+// rsqrtps / rcpps for inputs known to be positive normal numbers (rcpps: biased
+// exponent <= 251), with no special cases: for those inputs they return the
+// same bits as the functions above (checked over every such input). The
+// table index of rsqrtps is the top 12 bits below the sign with the exponent's
+// low bit inverted (= T[E & 1]), the scale 2^-(E - (E & 1)) / 2 an integer
+// subtraction from the exponent field.
+__device__ __forceinline__ float rsqrtps_pos(float x)
+{
+    const uint32_t u = fbits(x);
+    const uint32_t t = c_rsqrtps_table[((u >> 12) & 0xFFFu) ^ 0x800u];
+    return ffrom(t - (((u + 0x00800000u) & 0xFF000000u) >> 1) + 0x20000000u);
+}
+__device__ __forceinline__ float rcpps_pos(float x)
+{
+    const uint32_t u = fbits(x);
+    const uint32_t t = c_rcpps_table[(u >> 12) & 2047u];
+    return ffrom(t - (u & 0x7F800000u) + 0x3F800000u);
+}
+
 // One channel of RGBA32SoA_To_RGBA8AoS (SIMDUtil.h:87-121): RNE(fma(c,255,.5)),
 // packus_epi32 then packus_epi16 (signed 16-bit input).
 __device__ __forceinline__ uint32_t pack_channel(float c)

@@ -222,3 +222,65 @@ def test_gpu_sponza_then_narrow_frame():
         assert not any(d.values()), d
         fb.destroy()
     ctx.Shutdown()
+
+
+def _strip_draw(x0, x1, P, N, W):
+    """Quad over pixel columns [x0, x1) of a W-wide frame (identity MVP, w = 1),
+    every vertex with position attribute P, normal N."""
+    a, b = 2.0 * x0 / W - 1.0, 2.0 * x1 / W - 1.0
+    pos = np.array([[a, -1, 0.5], [b, -1, 0.5], [b, 1, 0.5], [a, 1, 0.5]], np.float32)
+    attr = np.zeros((4, 8), np.float32)
+    attr[:, 0:3] = P
+    attr[:, 3:6] = N
+    attr[:, 6:8] = (0.3, 0.6)
+    idx = np.array([0, 1, 2, 0, 2, 3, 0, 2, 1, 0, 3, 2], np.uint32)
+    return pos, attr, idx
+
+
+@pytest.mark.gpu
+def test_gpu_sponza_special_distances():
+    """Waves that mix pixels whose squared light distance is a positive normal
+    number (the fast pass) with pixels where it is 0, denormal or NaN (redone
+    with the exact RSQRTPS / RCPPS): strips of 37, 29 and 51 columns so that
+    waves straddle their borders; bit-exact against the oracle."""
+    from oracle.oracle import Oracle, Plane, make_texture
+    W, H = 128, 64
+    k = _consts(ambient=(0.1, 0.1, 0.1), lights=[((0.0, 0.0, 0.0), (1.0, 0.5, 0.25), 0.9, 0.0),
+                                                 ((4.0, 6.0, 3.0), (0.2, 0.9, 0.4), 0.7, 0.0)])
+    # d^2 to the first light: 0, 1e-40 (denormal), NaN, then positive normal
+    strips = [(0, 37, np.zeros(3, np.float32)), (37, 66, np.array([1e-20, 0, 0], np.float32)),
+              (66, 117, np.array([np.nan, 0, 0], np.float32)), (117, 128, np.array([0.5, -1.0, 2.0], np.float32))]
+    tex = _const_texture((180, 90, 60, 200))
+    st = make_texture(tex)
+    o = Oracle(nthreads=1)
+    o.set_sponza_constants(k.to_bytes())
+    plane = Plane(W, H)
+    o.clear(plane, 0)
+    odraws, bufs = [], []
+    for x0, x1, P in strips:
+        pos, attr, idx = _strip_draw(x0, x1, P, (0.0, 0.0, 1.0), W)
+        bufs.append((pos, attr, idx))
+        odraws.append(dict(shader=3, tex=st[0], idx=idx, idx_stride=4, pos=pos, pos_stride=12, pos_num=4, attr=attr,
+                           attr_stride=32, attr_num=4, uv_offset=6, plane=0, mvp=np.eye(4, dtype=np.float32)))
+    o.render(odraws, [plane])
+    o.close()
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS)
+    ctx.set_sponza_constants(k)
+    fb = sr.FrameBuffer(ctx, W, H)
+    ctx.BeginFrame()
+    ctx.ClearFrameBuffer(fb, 0)
+    for pos, attr, idx in bufs:
+        c = sr.DrawCall().SetFrameBuffer(fb).SetMVP(np.eye(4, dtype=np.float32))
+        c.SetIndexBuffer(idx, 4, len(idx)).SetPositionBuffer(pos, 12, 4).SetAttributeBuffer(attr, 32, 4, 6)
+        c.SetPixelShader(sr.SponzaShader, tex)
+        ctx.DrawIndexed(c)
+    ctx.EndFrame()
+    col, dep, pid = fb.read_tiles(primid=True)
+    d = helpers.compare(plane, col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
+    _say(f"[sponza] special distances diffs={d}")
+    assert not any(d.values()), d
+    fb.destroy()
+    ctx.Shutdown()
+    vals = set(np.unique(plane.colour).tolist())
+    assert 0xC9000000 in vals                     # the NaN strips: colour 0, the texel's alpha
+    assert any(v & 0xFFFFFF for v in vals)        # the last strip is lit
