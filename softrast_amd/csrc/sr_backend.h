@@ -153,52 +153,65 @@ __device__ __forceinline__ uint32_t shade_pixel(const FrameDev& F, uint32_t rec,
     const DrawDev& D = F.draws[d];
     if (WIDE && D.shader == SRG_SHADER_SPONZA) return shade_sponza(F, D, rec, X0, Y0, px, py);
     const float4* Sp = F.rec_shade + (size_t)rec * 4;
-    const float4 s0 = Sp[0], s1 = Sp[1], s2 = Sp[2], s3 = Sp[3];
-    const uint4 q2 = F.rec_raster[(size_t)rec * 4 + 2];
-    const uint4 q3 = F.rec_raster[(size_t)rec * 4 + 3];
+    const float4 s0 = Sp[0], s1 = Sp[1];
+    const float pf01 = ((const float*)Sp)[8];
+    const uint32_t* Rw = (const uint32_t*)(F.rec_raster + (size_t)rec * 4);
     // plane constants re-based to the tile origin (Binning.cpp:435-452)
-    const float sx = (float)X0 - ffrom(q2.w), sy = (float)Y0 - ffrom(q3.x);
+    const float sx = (float)X0 - ffrom(Rw[11]), sy = (float)Y0 - ffrom(Rw[12]);
     const float wdx = s0.x, wdy = s0.y;
     const float wc = wdx * sx + wdy * sy + s0.z;
-    const float pdx[kMaxSlots] = { s0.w, s1.z, s2.y, s3.x };
-    const float pdy[kMaxSlots] = { s1.x, s1.w, s2.z, s3.y };
-    const float pf0[kMaxSlots] = { s1.y, s2.x, s2.w, s3.z };
     const float X = (float)px, Y = (float)py;
     const float iq = 1.0f / __builtin_fmaf(X, wdx, __builtin_fmaf(Y, wdy, wc));
-    float pc[kMaxSlots], var[kMaxSlots];
-    const uint32_t ns = D.nslots;
-#pragma unroll
-    for (uint32_t k = 0; k < kMaxSlots; ++k) {
-        pc[k] = pdx[k] * sx + pdy[k] * sy + pf0[k];
-        var[k] = k < ns ? iq * __builtin_fmaf(pdy[k], Y, __builtin_fmaf(pdx[k], X, pc[k])) : 0.0f;
-    }
+    // slots 0 and 1: the sampled (u, v), the normal's x, y, or the shown uv
+    float tdx[2] = { s0.w, s1.z }, tdy[2] = { s1.x, s1.w }, tc[2], tv[2];
+    tc[0] = tdx[0] * sx + tdy[0] * sy + s1.y;
+    tc[1] = tdx[1] * sx + tdy[1] * sy + pf01;
+    tv[0] = iq * __builtin_fmaf(tdy[0], Y, __builtin_fmaf(tdx[0], X, tc[0]));
+    tv[1] = iq * __builtin_fmaf(tdy[1], Y, __builtin_fmaf(tdx[1], X, tc[1]));
+    const float u = tv[0], v = tv[1];
     switch (D.shader) {
-    case SRG_SHADER_VISUALIZE_NORMALS:
-        return pack_rgba(__builtin_fmaf(var[0], 0.5f, 0.5f), __builtin_fmaf(var[1], 0.5f, 0.5f),
-                         __builtin_fmaf(var[2], 0.5f, 0.5f), 1.0f);
+    case SRG_SHADER_VISUALIZE_NORMALS: {
+        const float4 s2 = Sp[2];
+        const float c2 = s2.y * sx + s2.z * sy + s2.w;
+        const float n2 = iq * __builtin_fmaf(s2.z, Y, __builtin_fmaf(s2.y, X, c2));
+        return pack_rgba(__builtin_fmaf(u, 0.5f, 0.5f), __builtin_fmaf(v, 0.5f, 0.5f), __builtin_fmaf(n2, 0.5f, 0.5f),
+                         1.0f);
+    }
     case SRG_SHADER_VISUALIZE_UVS:
-        return pack_rgba(var[0], var[1], 0.0f, 0.0f);
+        return pack_rgba(u, v, 0.0f, 0.0f);
     default: break;
     }
     if (D.tex == kNoTex) return 0xFFFFFFFFu;
     const TexDev& T = F.texs[D.tex];
     if (T.empty) return 0xFFFFFFFFu;
-    // neighbour 1/w by RCPPS, forward differences (Rasterizer.cpp:373-400)
-    const float X1 = 1.0f + X, Y1 = 1.0f + Y;
-    const float i10 = rcpps(__builtin_fmaf(wdx, X1, __builtin_fmaf(wdy, Y, wc)));
-    const float i01 = rcpps(__builtin_fmaf(wdx, X, __builtin_fmaf(wdy, Y1, wc)));
-    float der[4];
-    const bool hi = D.deriv_slot != 0;
-#pragma unroll
-    for (uint32_t uv = 0; uv < 2; ++uv) {
-        const float tdx = hi ? pdx[2 + uv] : pdx[uv], tdy = hi ? pdy[2 + uv] : pdy[uv];
-        const float tc = hi ? pc[2 + uv] : pc[uv], tv = hi ? var[2 + uv] : var[uv];
-        const float t10 = i10 * __builtin_fmaf(tdx, X1, __builtin_fmaf(tdy, Y, tc));
-        const float t01 = i01 * __builtin_fmaf(tdx, X, __builtin_fmaf(tdy, Y1, tc));
-        der[2 * uv] = t10 - tv;
-        der[2 * uv + 1] = t01 - tv;
+    if (D.deriv_slot != 0) {  // uv_offset != 6: derivatives from slots 2 and 3
+        const float4 s2 = Sp[2], s3 = Sp[3];
+        tdx[0] = s2.y; tdy[0] = s2.z; tdx[1] = s3.x; tdy[1] = s3.y;
+        tc[0] = tdx[0] * sx + tdy[0] * sy + s2.w;
+        tc[1] = tdx[1] * sx + tdy[1] * sy + s3.z;
+        tv[0] = iq * __builtin_fmaf(tdy[0], Y, __builtin_fmaf(tdx[0], X, tc[0]));
+        tv[1] = iq * __builtin_fmaf(tdy[1], Y, __builtin_fmaf(tdx[1], X, tc[1]));
     }
-    return sample_unlit(T, var[0], var[1], der[0], der[1], der[2], der[3]);
+    // neighbour 1/w by RCPPS, forward differences (Rasterizer.cpp:373-400);
+    // the special-case-free RCPPS while both arguments are positive normal
+    // numbers below 2^125, the exact one otherwise
+    const float X1 = 1.0f + X, Y1 = 1.0f + Y;
+    const float a10 = __builtin_fmaf(wdx, X1, __builtin_fmaf(wdy, Y, wc));
+    const float a01 = __builtin_fmaf(wdx, X, __builtin_fmaf(wdy, Y1, wc));
+    float i10 = rcpps_pos(a10), i01 = rcpps_pos(a01);
+    if ((fbits(a10) - 0x00800000u) >= 0x7D800000u || (fbits(a01) - 0x00800000u) >= 0x7D800000u) {
+        i10 = rcpps(a10);
+        i01 = rcpps(a01);
+    }
+    float der[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 2; ++k) {
+        const float t10 = i10 * __builtin_fmaf(tdx[k], X1, __builtin_fmaf(tdy[k], Y, tc[k]));
+        const float t01 = i01 * __builtin_fmaf(tdx[k], X, __builtin_fmaf(tdy[k], Y1, tc[k]));
+        der[2 * k] = t10 - tv[k];
+        der[2 * k + 1] = t01 - tv[k];
+    }
+    return sample_unlit(T, u, v, der[0], der[1], der[2], der[3]);
 }
 
 // The view a bin belongs to: the last view whose first bin is <= gb. gb must

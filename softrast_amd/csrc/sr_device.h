@@ -143,8 +143,12 @@ __device__ __forceinline__ void sample_wrap(const TexDev& T, float u, float v, f
     const float p = dudx * width, q = dudy * height, r = dvdx * width, s = dvdy * height;
     const float du2 = __builtin_fmaf(p, p, q * q);
     const float dv2 = __builtin_fmaf(r, r, s * s);
-    const float m = __builtin_sqrtf(maxps(du2, dv2));
-    int32_t mip = (int32_t)((((int32_t)fbits(m)) >> 23) & 0xFF) - 127;
+    // ExtractExponent(sqrt(m)) = floor((E(m) - 127) / 2) for every normal m
+    // (checked over all of them); zero, denormals, inf and NaN take the sqrt
+    const float m2 = maxps(du2, dv2);
+    const uint32_t me = (fbits(m2) >> 23) & 0xFFu;
+    int32_t mip = ((int32_t)me - 127) >> 1;
+    if (me == 0u || me == 255u) mip = (int32_t)((((int32_t)fbits(__builtin_sqrtf(m2))) >> 23) & 0xFF) - 127;
     mip = mip > 0 ? mip : 0;
     mip = mip < (int32_t)T.nmips - 1 ? mip : (int32_t)T.nmips - 1;
 
