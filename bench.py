@@ -149,17 +149,21 @@ def batched_frags(sr, scene, texs, views, bin_size):
     return frags / views
 
 
-def load_traffic(name):
-    """HBM bytes per launch from a committed rocprofv3 summary: the line
-    'traffic_bytes_per_launch = N' (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+def load_profile_value(name, key):
+    """An integer 'key = N' line of a committed rocprofv3 summary in profiles/."""
     try:
         with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
-                if line.strip().startswith("traffic_bytes_per_launch"):
+                if line.strip().startswith(key):
                     return int(line.split("=", 1)[1].strip())
     except (OSError, ValueError):
         pass
     return None
+
+
+def load_traffic(name):
+    """HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    return load_profile_value(name, "traffic_bytes_per_launch")
 
 
 def profile_batched(sr, scene, texs, args):
@@ -600,6 +604,14 @@ def main():
                      "traffic": traffic},
         "frame_algorithmic_bytes": fe_bytes + be_bytes,
     }
+    # the back end is bound by instruction issue, not HBM: its VALU wave
+    # instructions per launch (SQ_INSTS_VALU, profiles/) at 4 cycles each on
+    # 1,024 SIMDs at 2.4 GHz, against the live back-end time
+    valu = load_profile_value("k_backend_spheres_1080p_1m_sq_counters.md", "valu_insts_per_launch")
+    if valu:
+        floor_us = valu * 4.0 / 1024 / 2.4e3
+        out["valu_issue"] = {"kernel": "k_backend", "valu_insts_per_launch": valu, "issue_floor_us": round(floor_us, 1),
+                             "measured_us": round(be_ms * 1e3, 1), "frac": round(floor_us / (be_ms * 1e3), 3)}
     if fps_batched:
         out["batched"] = {"views": args.views, "frames_per_s": round(fps_batched, 2),
                           "ms_per_launch": round(ms_batched, 4), "device_ms_per_launch": round(dev_batched, 4),
