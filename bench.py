@@ -67,6 +67,17 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def view_mvp(scene, yaw_deg):
     from softrast_amd import scenes
     t = math.radians(yaw_deg)
@@ -652,6 +663,17 @@ def main():
                                    "sample": f"same 1080p/1M-tri frame (clear+render), median of {args.cpu_frames} "
                                              f"frames after 5 warm-up, {nthreads} threads, oracle -O3 AVX2/FMA build"}
             out["speedup_vs_cpu"] = round(value * med, 1)
+            # the host the baseline ran on, and the same frame on one thread
+            one = Oracle(nthreads=1, fast=True)
+            t1 = []
+            for _ in range(3):
+                t = time.perf_counter()
+                one.clear(plane, 0)
+                one.render(odraws, [plane])
+                t1.append(time.perf_counter() - t)
+            one.close()
+            out["cpu_host"] = {"model": cpu_model(), "nproc": os.cpu_count(), "threads_used": nthreads,
+                               "one_thread_frames_per_s": round(1.0 / statistics.median(t1), 3)}
         orc.close()
 
     if "frags_raster" not in out:  # no oracle frame: count on the GPU in an untimed frame

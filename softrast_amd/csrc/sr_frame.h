@@ -113,61 +113,9 @@ __host__ __device__ constexpr uint32_t part_of(uint32_t rec) { return (rec >> 6)
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-__device__ __forceinline__ unsigned long long lanemask_lt()
-{
-    const uint32_t l = lane_id();
-    return l ? (~0ull >> (64u - l)) : 0ull;
-}
-
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
-}
-
-// Groups the active lanes of a wave by key: each lane learns the lowest lane
-// of its group (its leader) and its rank in the group, and each leader the
-// group size. Uniform control flow required.
-__device__ __forceinline__ void wave_group_by_key(uint32_t key, bool active, uint32_t& leader_of, uint32_t& rank,
-                                                  uint32_t& count)
-{
-    unsigned long long pending = __ballot(active);
-    const uint32_t lane = lane_id();
-    leader_of = lane;
-    rank = 0;
-    count = 0;
-    while (pending) {
-        const uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1u;
-        const uint32_t lkey = readlane_u32(key, leader);
-        const unsigned long long same = __ballot(active && key == lkey) & pending;
-        if ((same >> lane) & 1ull) {
-            leader_of = leader;
-            rank = (uint32_t)__popcll(same & lanemask_lt());
-        }
-        if (lane == leader) count = (uint32_t)__popcll(same);
-        pending &= ~same;
-    }
-}
-
-// Wave-aggregated counter increment: every active lane adds 1 to
-// counters[key] and receives its slot. Lanes with equal keys share one
-// atomic, and the atomics of all keys go out as one instruction, so the wave
-// waits for one round trip. Uniform control flow required.
-__device__ __forceinline__ uint32_t wave_add_slot(uint32_t* counters, uint32_t key, bool active)
-{
-    uint32_t leader_of, rank, count;
-    wave_group_by_key(key, active, leader_of, rank, count);
-    uint32_t base = 0;
-    if (count) base = atomicAdd(&counters[key], count);
-    base = __shfl(base, (int)leader_of, 64);
-    return base + rank;
-}
-
-// Wave-aggregated increment without slots. Uniform control flow required.
-__device__ __forceinline__ void wave_count(uint32_t* counters, uint32_t key, bool active)
-{
-    uint32_t leader_of, rank, count;
-    wave_group_by_key(key, active, leader_of, rank, count);
-    if (count) atomicAdd(&counters[key], count);
 }
 
 // Per-wave LDS hash for aggregating atomics by key, without a serial pass
