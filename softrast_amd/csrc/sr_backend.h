@@ -337,13 +337,14 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
 }
 
 // Rasterisation of a wave's 64 entries (one per lane, `e`). Every entry's
-// candidate pixels (cx0..cx1 x cy0..cy1) split into columns of 8x8 block rows:
-// cw columns in each of its block rows. These (entry, block row, column)
-// items form a queue the wave takes 64 at a time, one lane per item; the lane
-// walks its column down the block row, carrying the reference's per-column
-// depth (ComputeBlockMask8x8, Rasterizer.cpp:134-192): z = fma(yb, dy,
-// fma(l, dx, c0)) + xb*dx at the block's top row, then + dy per row, and the
-// edge values + dx_k per row.
+// candidate pixels (cx0..cx1 x cy0..cy1) split into column pairs of 8x8 block
+// rows: the even-aligned pairs (2m, 2m + 1) that overlap cx0..cx1, in each of
+// its block rows (a pair never straddles a block). These (entry, block row,
+// pair) items form a queue the wave takes 64 at a time, one lane per item;
+// the lane walks both columns of its pair down the block row, carrying the
+// reference's per-column depth (ComputeBlockMask8x8, Rasterizer.cpp:134-192):
+// z = fma(yb, dy, fma(l, dx, c0)) + xb*dx at the block's top row, then + dy
+// per row, and the edge values + dx_k per row.
 //
 // A block the reference rejects by its 64-wide corner test is skipped and a
 // fully accepted one is covered without per-pixel tests (Rasterizer.cpp:
@@ -358,7 +359,8 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
 {
     const uint32_t lane = lane_id();
     const uint32_t cw = e.cx1 > e.cx0 ? e.cx1 - e.cx0 : 0u;
-    const uint32_t n = has ? e.nby * cw : 0u;
+    const uint32_t np = cw ? ((e.cx1 - 1u) >> 1) - (e.cx0 >> 1) + 1u : 0u;  // column pairs per block row
+    const uint32_t n = has ? e.nby * np : 0u;
     uint32_t incl = n;
 #pragma unroll
     for (uint32_t o = 1; o < 64; o <<= 1) {
@@ -367,7 +369,7 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     }
     const uint32_t total = bcast(incl, 63);
     // per-entry values the column workers need, packed
-    const uint32_t magic = (65536u + max(cw, 1u) - 1u) / max(cw, 1u);  // k / cw == (k * magic) >> 16 for k < 512
+    const uint32_t magic = (65536u + max(np, 1u) - 1u) / max(np, 1u);  // k / np == (k * magic) >> 16 for k < 512
     const uint32_t pk_cand = e.cx0 | (cw << 7) | (e.cy0 << 14) | (e.cy1 << 21) | ((e.tight ? 1u : 0u) << 28);
     const uint32_t pk_div = e.by0 | (magic << 7);
     const uint32_t first = incl - n;
@@ -391,11 +393,13 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         const uint32_t k = it - __shfl(first, src, 64);
         const uint32_t cx0 = pc & 127u, ccw = (pc >> 7) & 127u, cy0 = (pc >> 14) & 127u, cy1 = (pc >> 21) & 127u;
         const bool tight = ((pc >> 28) & 1u) != 0;
+        const uint32_t snp = ccw ? ((cx0 + ccw - 1u) >> 1) - (cx0 >> 1) + 1u : 1u;
         const uint32_t brow = (k * (pd >> 7)) >> 16;
-        const uint32_t x = cx0 + (k - brow * ccw), yb = (pd & 127u) + 8u * brow;
+        const uint32_t x = 2u * ((cx0 >> 1) + (k - brow * snp)), yb = (pd & 127u) + 8u * brow;
         const uint32_t xb = x & ~7u, l = x & 7u;
-        // the item lies in its owner's candidate rectangle (checked, not assumed)
-        const bool inside = valid && yb < cy1 && x - cx0 < ccw;
+        // the pair's columns inside the owner's candidate rectangle (checked, not assumed)
+        const bool in0 = x - cx0 < ccw, in1 = x + 1u - cx0 < ccw;
+        const bool inside = valid && yb < cy1 && (in0 || in1);
         const uint32_t r_lo = inside && cy0 > yb ? cy0 - yb : 0u;
         const uint32_t r_hi = inside ? min(cy1 - yb, 8u) : 0u;
         // per-pixel rule: covered = full || (test && all edge values >= 0)
@@ -405,27 +409,42 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
             full = cls == 2u;
             test = cls == 1u;
         }
-        float z = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
-        for (uint32_t r = 0; r < r_lo; ++r) z = z + zdy;
+        float z0 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
+        float z1 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)(l + 1u), zdx, zc0)) + (float)xb * zdx;
+        for (uint32_t r = 0; r < r_lo; ++r) {
+            z0 = z0 + zdy;
+            z1 = z1 + zdy;
+        }
         uint32_t E[3];
 #pragma unroll
         for (int kk = 0; kk < 3; ++kk) E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * (yb + r_lo);
         // key_slot of (x - sx0, yb + r - sy0): rows of S keys, the in-group
-        // rotation alternating with the row's parity (yb is a multiple of 8)
+        // rotation alternating with the row's parity (yb is a multiple of 8);
+        // x is even, so x + 1 is in the same group of G keys
         constexpr uint32_t G = S < 16u ? S : 16u;
         const uint32_t sxl = x - sx0, ybl = yb - sy0;
         const uint32_t t = sxl + 2u * (ybl >> 3) + (sxl >> 4);
-        const uint32_t a_even = ybl * S + (sxl & ~(G - 1u)) + (t & (G - 1u));
-        const uint32_t a_odd = ybl * S + (sxl & ~(G - 1u)) + ((t + 8u) & (G - 1u));
+        const uint32_t grp = ybl * S + (sxl & ~(G - 1u));
         for (uint32_t r = r_lo; r < r_hi; ++r) {
-            const bool pos = (int32_t)(E[0] | E[1] | E[2]) >= 0;
-            bool cov = full | (test & pos);
-            if (tight & pos & (min(min(E[0], E[1]), E[2]) == 0u))
-                cov = block_class(ct, dxu, dyu, xb, yb) != 0u;  // an edge through the pixel: the block test decides
-            if (STATS) frags += cov ? 1u : 0u;
-            if (cov & (z > 0.0f))
-                atomicMax(&keys[((r & 1u) ? a_odd : a_even) + r * S], ((unsigned long long)fbits(z) << 32) | lw);
-            z = z + zdy;
+            const uint32_t tr = t + ((r & 1u) << 3), ro = grp + r * S;
+            const uint32_t F0 = E[0] + dyu[0], F1 = E[1] + dyu[1], F2 = E[2] + dyu[2];
+            const bool pos0 = (int32_t)(E[0] | E[1] | E[2]) >= 0;
+            const bool pos1 = (int32_t)(F0 | F1 | F2) >= 0;
+            bool cov0 = in0 & (full | (test & pos0));
+            bool cov1 = in1 & (full | (test & pos1));
+            if (tight & ((in0 & pos0 & (min(min(E[0], E[1]), E[2]) == 0u)) | (in1 & pos1 & (min(min(F0, F1), F2) == 0u)))) {
+                // an edge through a pixel: the block test decides that pixel
+                const bool bc = block_class(ct, dxu, dyu, xb, yb) != 0u;
+                if (in0 & pos0 & (min(min(E[0], E[1]), E[2]) == 0u)) cov0 = bc;
+                if (in1 & pos1 & (min(min(F0, F1), F2) == 0u)) cov1 = bc;
+            }
+            if (STATS) frags += (cov0 ? 1u : 0u) + (cov1 ? 1u : 0u);
+            if (cov0 & (z0 > 0.0f))
+                atomicMax(&keys[ro + (tr & (G - 1u))], ((unsigned long long)fbits(z0) << 32) | lw);
+            if (cov1 & (z1 > 0.0f))
+                atomicMax(&keys[ro + ((tr + 1u) & (G - 1u))], ((unsigned long long)fbits(z1) << 32) | lw);
+            z0 = z0 + zdy;
+            z1 = z1 + zdy;
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk) E[kk] += dxu[kk];
         }
