@@ -151,11 +151,13 @@ def compare(oracle_plane, colour, depth, primid=None):
     return out
 
 
-def oracle_render_views(viewset, texdata, nthreads=8, clear=0):
+def oracle_render_views(viewset, texdata, nthreads=8, clear=0, sponza=None):
     """Every view of a scenes.ViewSet in one oracle frame, one plane per view
     (the order key runs over all views' draws, as in one EndFrame)."""
     from oracle.oracle import Oracle, Plane, make_texture
     o = Oracle(nthreads=nthreads)
+    if sponza is not None:
+        o.set_sponza_constants(sponza.to_bytes())
     planes = [Plane(v.width, v.height) for v in viewset.views]
     for p in planes:
         o.clear(p, clear)
@@ -173,11 +175,14 @@ def oracle_render_views(viewset, texdata, nthreads=8, clear=0):
     return planes, stats
 
 
-def gpu_render_views(viewset, texdata, ctx, fbs, clear=0):
+def gpu_render_views(viewset, texdata, ctx, fbs, clear=0, sponza=None):
     """Every view of a scenes.ViewSet in one EndFrame, view k into fbs[k];
     `texdata` holds TextureData or device textures by ViewSet texture index."""
     import softrast_amd as sr
-    shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader}
+    shader = {0: sr.UnlitDiffuseShader, 1: sr.VisualizeNormalsShader, 2: sr.VisualizeUVsShader,
+              3: sr.SponzaShader}
+    if sponza is not None:
+        ctx.set_sponza_constants(sponza)
     ctx.BeginFrame()
     for v, fb in zip(viewset.views, fbs):
         ctx.ClearFrameBuffer(fb, clear)

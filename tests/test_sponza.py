@@ -284,3 +284,39 @@ def test_gpu_sponza_special_distances():
     vals = set(np.unique(plane.colour).tolist())
     assert 0xC9000000 in vals                     # the NaN strips: colour 0, the texel's alpha
     assert any(v & 0xFFFFFF for v in vals)        # the last strip is lit
+
+
+@pytest.mark.gpu
+def test_gpu_sponza_views_batch_graph_autotuned():
+    """Five views mixing SponzaShader and UnlitDiffuse draws in one
+    hipGraph-captured EndFrame while the autotuner sweeps every layout, in
+    the learned order; the constants change half-way (the captured frame
+    must be re-captured with them). Every view of every frame is the
+    oracle's image."""
+    sizes = [(320, 240), (256, 256), (640, 360), (200, 130), (512, 288)]
+    views = []
+    for k, (w, h) in enumerate(sizes):
+        sc = helpers.random_scene(320 + k, 5000, w, h, clip_heavy=(k % 2 == 1), shaders=(3, 0), ndraws=2)
+        views.append(sc)
+    vs = scenes.ViewSet([scenes.Scene(v.width, v.height, v.mvp, v.draws, []) for v in views], views[0].textures)
+    tex = [sr.TextureData().CreateFromRGBA8(t, t.shape[1], t.shape[0], True) for t in vs.textures]
+    consts = [scenes.sponza_constants(11, near=False), scenes.sponza_constants(12, near=True)]
+    refs = [helpers.oracle_render_views(vs, tex, sponza=k) for k in consts]
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS | sr.CFG_GRAPH | sr.CFG_AUTOTUNE,
+                           tile_order=sr.ORDER_LEARNED, autotune_frames=1, sched_warmup_frames=2)
+    fbs = [sr.FrameBuffer(ctx, v.width, v.height) for v in vs.views]
+    nframes = len(sr.LAYOUTS) + 4
+    for frame in range(nframes):
+        which = 0 if frame < nframes // 2 else 1
+        _say(f"[sponza] case=views_batch frame={frame} constants={which}")
+        st = helpers.gpu_render_views(vs, tex, ctx, fbs, sponza=consts[which])
+        planes, rstats = refs[which]
+        for vi, (fb, p) in enumerate(zip(fbs, planes)):
+            col, dep, pid = fb.read_tiles(primid=True)
+            d = helpers.compare(p, col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
+            assert not any(d.values()), (frame, vi, d)
+        assert st["frags_raster"] == rstats["frags_raster"]
+    assert ctx.tuning()["state"] == "chosen"
+    for fb in fbs:
+        fb.destroy()
+    ctx.Shutdown()
