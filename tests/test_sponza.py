@@ -320,3 +320,30 @@ def test_gpu_sponza_views_batch_graph_autotuned():
     for fb in fbs:
         fb.destroy()
     ctx.Shutdown()
+
+
+@pytest.mark.gpu
+def test_gpu_sponza_draw_validation():
+    """A SponzaShader draw reads eight varyings: an attribute stride other
+    than 32 bytes, or a uv offset past the seventh float, is rejected at
+    DrawIndexed with SRG_ERR_INVALID; the context then renders normally."""
+    ctx = sr.RenderContext(flags=sr.CFG_STATS)
+    fb = sr.FrameBuffer(ctx, 64, 64)
+    pos, attr, idx = _strip_draw(0, 64, np.zeros(3, np.float32), (0.0, 0.0, 1.0), 64)
+    ctx.BeginFrame()
+    for stride, attrs, uv in ((24, attr[:, :6].copy(), 4), (32, attr, 7)):
+        c = sr.DrawCall().SetFrameBuffer(fb).SetMVP(np.eye(4, dtype=np.float32))
+        c.SetIndexBuffer(idx, 4, len(idx)).SetPositionBuffer(pos, 12, 4).SetAttributeBuffer(attrs, stride, 4, uv)
+        c.SetPixelShader(sr.SponzaShader, None)
+        with pytest.raises(sr.SrgError) as e:
+            ctx.DrawIndexed(c)
+        assert e.value.status == -1
+    c = sr.DrawCall().SetFrameBuffer(fb).SetMVP(np.eye(4, dtype=np.float32))
+    c.SetIndexBuffer(idx, 4, len(idx)).SetPositionBuffer(pos, 12, 4).SetAttributeBuffer(attr, 32, 4, 6)
+    c.SetPixelShader(sr.SponzaShader, None)
+    ctx.DrawIndexed(c)
+    ctx.EndFrame()
+    col, _ = fb.read_tiles()
+    assert np.mean(col.reshape(-1)[:64 * 64] == 0xFFFFFFFF) > 0.9  # no texture: white
+    fb.destroy()
+    ctx.Shutdown()
