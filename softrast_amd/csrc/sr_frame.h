@@ -16,7 +16,8 @@ struct DrawDev {
     uint32_t shader, tex, nslots, deriv_slot;  // deriv_slot: first slot of (u, v) at uv_offset
     uint32_t slot_attr[kWideSlots];
     uint32_t block_base;  // first k_setup workgroup of the draw (kSetupThreads triangles each)
-    uint32_t pad[3];
+    uint32_t tex_info;    // the texture's max(wlog2, hlog2) | nmips << 8 (0 without one): for scheduling features
+    uint32_t pad[2];
     float mvp[16];
 };
 
@@ -229,6 +230,25 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v)
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_min_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float wave_max_f(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
     return v;
 }
 

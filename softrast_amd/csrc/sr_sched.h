@@ -5,10 +5,16 @@
 // only the order bins are processed in, never the arithmetic of a pixel.
 //
 //   k_bin_predict   one wave per work bin: the features, from its entry list:
-//                   x0 = log2(1 + entries),
-//                   x1 = log2(1 + walked 8x8 blocks),
-//                   x2 = log2(1 + walked blocks of primitives without a tight bound),
-//                   x3 = log2(1 + summed bbox-in-bin area / bin area);
+//                   x0 = log2(1 + entries) (the bin count),
+//                   x1 = log2(1 + summed clipped triangle area / bin area),
+//                        a triangle's clipped area taken as the smaller of
+//                        its area and its bounding box inside the bin,
+//                   x2 = log2(1 + depth-range overlap): the entries' depth
+//                        extents over the bin summed, over the span of all of
+//                        them (about 1 when the ranges tile the span, up to
+//                        the entry count when they all coincide),
+//                   x3 = mean mip level of the textured entries (from the
+//                        uv planes at the first vertex, 0 without texture);
 //                   the sums are estimated from at most 64 entries spread
 //                   evenly over the list, scaled by entries / sampled. Then,
 //                   once trained, the 4 -> 32 -> 32 -> 1 ReLU MLP (fp32) ->
@@ -61,29 +67,53 @@ __device__ __forceinline__ void bin_predict_one(const FrameDev& F, const MlpLds&
     const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S;
     const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
     const uint32_t ns = cnt < 64u ? cnt : 64u;  // sampled entries: lane j takes entry j * cnt / ns
-    uint32_t blocks = 0, open_blocks = 0, area = 0;
+    float area = 0.0f, zlo = 3.0e38f, zhi = -3.0e38f, zext = 0.0f, mip = 0.0f, textured = 0.0f;
     if (lane < ns) {
         const uint32_t e = (uint32_t)(((uint64_t)lane * cnt) / ns);
-        const uint4 q2 = F.rec_raster[(size_t)F.entries[beg + e] * 4 + 2];
+        const uint32_t rec = F.entries[beg + e];
+        const uint4 q0 = F.rec_raster[(size_t)rec * 4], q2 = F.rec_raster[(size_t)rec * 4 + 2];
+        const uint4 q3 = F.rec_raster[(size_t)rec * 4 + 3];
         const int32_t xmin = q2.y & 0x3FFF, xmax = (q2.y >> 16) & 0x3FFF, ymin = q2.z & 0x3FFF, ymax = (q2.z >> 16) & 0x3FFF;
-        const uint32_t xs = max((uint32_t)clampi(xmin - X0, 0, 64) & ~7u, sx0);
-        const uint32_t xe = min((uint32_t)clampi(xmax - X0, 0, 64), sx0 + S);
-        const uint32_t ys = max((uint32_t)clampi(ymin - Y0, 0, 64) & ~7u, sy0);
-        const uint32_t ye = min((uint32_t)clampi(ymax - Y0, 0, 64), sy0 + S);
-        const uint32_t nb = (xe > xs ? (xe - xs + 7u) >> 3 : 0u) * (ye > ys ? (ye - ys + 7u) >> 3 : 0u);
-        blocks += nb;
-        if (!((q2.y >> 14) & 1u)) open_blocks += nb;  // no tight bound: every row of every block is visited
-        const uint32_t ax = (uint32_t)max(0, min(xmax - X0 + 1, (int32_t)(sx0 + S)) - max(xmin - X0, (int32_t)sx0));
-        const uint32_t ay = (uint32_t)max(0, min(ymax - Y0 + 1, (int32_t)(sy0 + S)) - max(ymin - Y0, (int32_t)sy0));
-        area += ax * ay;
+        // bounding box inside the bin (pixels, ends exclusive)
+        const int32_t bx0 = max(xmin - X0, (int32_t)sx0), bx1 = min(xmax - X0 + 1, (int32_t)(sx0 + S));
+        const int32_t by0 = max(ymin - Y0, (int32_t)sy0), by1 = min(ymax - Y0 + 1, (int32_t)(sy0 + S));
+        const float box = (float)max(0, bx1 - bx0) * (float)max(0, by1 - by0);
+        // the three edge constants sum to the doubled area in subpixel x pixel units
+        const float tri = (float)max(0, (int32_t)(q0.x + q0.y + q0.z)) * (1.0f / 512.0f);
+        area = fminf(tri, box);
+        // depth over the box corners: z = zdx (x - r0x) + zdy (y - r0y) + z0
+        const float zdx = ffrom(q3.y), zdy = ffrom(q3.z);
+        const float ux = (float)(X0 + bx0) - ffrom(q2.w), uy = (float)(Y0 + by0) - ffrom(q3.x);
+        const float za = zdx * ux + zdy * uy + ffrom(q3.w);
+        const float wx = zdx * (float)max(0, bx1 - bx0), wy = zdy * (float)max(0, by1 - by0);
+        const float z0 = fminf(za, za + wx) + fminf(0.0f, wy), z1 = fmaxf(za, za + wx) + fmaxf(0.0f, wy);
+        if (z0 == z0 && z1 == z1 && z1 - z0 < 1.0e30f) {
+            zlo = z0;
+            zhi = z1;
+            zext = z1 - z0;
+        }
+        // mip level from the uv planes at the first vertex (slots 0 and 1)
+        const float4 s0 = F.rec_shade[(size_t)rec * 4], s1 = F.rec_shade[(size_t)rec * 4 + 1];
+        const DrawDev& D = F.draws[F.rec_meta[rec]];
+        const uint32_t ti = D.tex_info;
+        if (ti && D.deriv_slot == 0) {
+            const float g = fmaxf(fmaxf(fabsf(s0.w), fabsf(s1.x)), fmaxf(fabsf(s1.z), fabsf(s1.w))) /
+                            fmaxf(fabsf(s0.z), 1.0e-30f) * (float)(1u << (ti & 31u));
+            const float m = g > 1.0f ? __log2f(g) : 0.0f;
+            mip = m == m ? fminf(m, (float)((ti >> 8) ? (ti >> 8) - 1u : 0u)) : 0.0f;
+            textured = 1.0f;
+        }
     }
-    blocks = wave_sum(blocks);
-    open_blocks = wave_sum(open_blocks);
-    area = wave_sum(area);
+    area = wave_sum_f(area);
+    zext = wave_sum_f(zext);
+    zlo = wave_min_f(zlo);
+    zhi = wave_max_f(zhi);
+    mip = wave_sum_f(mip);
+    textured = wave_sum_f(textured);
     const float scale = ns ? (float)cnt / (float)ns : 0.0f;
-    const float x[kFeat] = { __log2f(1.0f + (float)cnt), __log2f(1.0f + scale * (float)blocks),
-                             __log2f(1.0f + scale * (float)open_blocks),
-                             __log2f(1.0f + scale * (float)area / (float)(S * S)) };
+    const float overlap = zhi > zlo ? zext / (zhi - zlo) : (float)ns;  // one shared depth: all coincide
+    const float x[kFeat] = { __log2f(1.0f + (float)cnt), __log2f(1.0f + scale * area / (float)(S * S)),
+                             __log2f(1.0f + scale * overlap), textured > 0.0f ? mip / textured : 0.0f };
     if (lane == 0) feat[gb] = make_float4(x[0], x[1], x[2], x[3]);
     if (!trained) return;
     // hidden unit j = lane & 31 (the upper half repeats it); layer 2 reads

@@ -606,6 +606,8 @@ int run_frame(srg_ctx* ctx)
             } else {
                 d.tex = it->second;
             }
+            const TexDev& td = tdev[d.tex];
+            d.tex_info = td.empty ? 0u : (std::max(td.wlog2, td.hlog2) | (td.nmips << 8));
         }
         std::memcpy(d.mvp, s.mvp, sizeof(d.mvp));
         d.block_base = nblocks;
