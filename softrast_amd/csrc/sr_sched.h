@@ -15,7 +15,7 @@
 //                        the entry count when they all coincide),
 //                   x3 = mean mip level of the textured entries (from the
 //                        uv planes at the first vertex, 0 without texture);
-//                   the sums are estimated from at most 64 entries spread
+//                   the sums are estimated from at most 16 entries spread
 //                   evenly over the list, scaled by entries / sampled. Then,
 //                   once trained, the 4 -> 32 -> 32 -> 1 ReLU MLP (fp32) ->
 //                   predicted log2 cost, one hidden unit per lane
@@ -66,7 +66,7 @@ __device__ __forceinline__ void bin_predict_one(const FrameDev& F, const MlpLds&
     const int32_t X0 = (int32_t)((sbx / K) * kBin), Y0 = (int32_t)((sby / K) * kBin);
     const uint32_t sx0 = (sbx % K) * S, sy0 = (sby % K) * S;
     const uint32_t beg = F.bin_start[gb], cnt = F.bin_count[gb];
-    const uint32_t ns = cnt < 64u ? cnt : 64u;  // sampled entries: lane j takes entry j * cnt / ns
+    const uint32_t ns = cnt < 16u ? cnt : 16u;  // sampled entries: lane j takes entry j * cnt / ns
     float area = 0.0f, zlo = 3.0e38f, zhi = -3.0e38f, zext = 0.0f, mip = 0.0f, textured = 0.0f;
     if (lane < ns) {
         const uint32_t e = (uint32_t)(((uint64_t)lane * cnt) / ns);
