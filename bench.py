@@ -64,6 +64,9 @@ def parse():
                          "drawn with the SponzaShader (16 point lights) against UnlitDiffuse (the default is the "
                          "1080p/1M workload)")
     ap.add_argument("--nviews", type=int, default=0, help="mixed/combined: views per batch (0 = the workload's)")
+    ap.add_argument("--present", action="store_true",
+                    help="the plain 1080p/1M frame with its pixels brought to the host each frame: a synchronous "
+                         "Blit, and an asynchronous one overlapped with the next frame (PCIe-inclusive rates)")
     return ap.parse_args()
 
 
@@ -515,6 +518,47 @@ def sponza_workload(sr, args):
     print(json.dumps(out))
 
 
+def present_rates(sr, args):
+    """--present: frames/s of the plain path when every frame's pixels go to
+    host memory (linear RGBA8), by the synchronous Blit and by BlitAsync
+    overlapped with rendering the next frame; beside it the plain rate
+    without a blit, in the same process."""
+    from softrast_amd import scenes
+    scene = scenes.spheres_scene()
+    texs = [sr.TextureData().CreateFromRGBA8(t, t.shape[1], t.shape[0], True) for t in scene.textures]
+    out = {"metric": "frames/s at 1080p/1M tris with the frame's pixels in host memory", "unit": "frames/s",
+           "config": {"workload": "spheres_1080p_1m", "bytes_per_frame": scene.width * scene.height * 4}}
+    for mode in ("none", "sync", "async"):
+        ctx = sr.RenderContext(flags=0)
+        fb = sr.FrameBuffer(ctx, scene.width, scene.height)
+        draws = ctx.PrepareDrawList(make_calls(sr, fb, scene, texs, scene.mvp))
+        host = [np.empty((scene.height, scene.width), np.uint32) for _ in range(2)]  # pageable, as a caller's
+
+        def frame(i):
+            ctx.BeginFrame()
+            ctx.ClearFrameBuffer(fb, 0)
+            ctx.DrawIndexedList(draws)
+            ctx.EndFrame()
+            if mode == "sync":
+                ctx.Blit(fb, host[i & 1])
+            elif mode == "async":
+                ctx.BlitWait(fb)  # the previous frame's copy, overlapped with this frame's rendering
+                ctx.BlitAsync(fb, host[i & 1])
+
+        for i in range(args.warmup):
+            frame(i)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            frame(i)
+        if mode == "async":
+            ctx.BlitWait(fb)
+        out[mode] = round(args.steps / (time.perf_counter() - t0), 2)
+        print(f"[bench] present {mode}: {out[mode]} frames/s", file=sys.stderr, flush=True)
+        fb.destroy()
+        ctx.Shutdown()
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -526,6 +570,8 @@ def main():
     from softrast_amd import scenes
     if args.sched_study:
         return sched_study(sr, args)
+    if args.present:
+        return present_rates(sr, args)
     if args.workload == "sponza":
         return sponza_workload(sr, args)
     if args.workload != "spheres":
