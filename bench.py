@@ -118,11 +118,11 @@ def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup
         calls.append(make_calls(sr, fb, scene, texs, mvp, shader))
 
     draws = ctx.PrepareDrawList([c for cs in calls for c in cs])  # uploads every buffer and texture
+    clears = ctx.PrepareClearList(fbs)
 
     def frame():
         ctx.BeginFrame()
-        for fb in fbs:
-            ctx.ClearFrameBuffer(fb, 0)
+        ctx.ClearFrameBuffers(clears, 0)
         ctx.DrawIndexedList(draws)
         ctx.EndFrame()
 
@@ -445,8 +445,14 @@ def sponza_workload(sr, args):
     print(f"[bench] sponza scene ready: {scene.num_tris} tris, {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     res = {}
     for name, shader in (("unlit", sr.UnlitDiffuseShader), ("sponza", sr.SponzaShader)):
-        fps, ms, st, ctx, fbs = time_path(sr, scene, texs, 1, sr.CFG_TIMING, sr.ORDER_NATURAL, args.bin_size,
-                                          args.steps, args.warmup, shader, k)
+        # device times from an instrumented pass, frames/s from an uninstrumented one
+        _, _, st, ctx, fbs = time_path(sr, scene, texs, 1, sr.CFG_TIMING, sr.ORDER_NATURAL, args.bin_size,
+                                       min(args.steps, 100), args.warmup, shader, k)
+        for fb in fbs:
+            fb.destroy()
+        ctx.Shutdown()
+        fps, ms, _, ctx, fbs = time_path(sr, scene, texs, 1, 0, sr.ORDER_NATURAL, args.bin_size, args.steps,
+                                         args.warmup, shader, k)
         res[name] = {"frames_per_s": round(fps, 2), "ms_per_frame": round(ms, 4),
                      "device_ms": {"frontend": round(statistics.mean(s["ms_frontend"] for s in st), 4),
                                    "backend": round(statistics.mean(s["ms_backend"] for s in st), 4),
@@ -457,11 +463,16 @@ def sponza_workload(sr, args):
             fb.destroy()
         ctx.Shutdown()
         print(f"[bench] plain {name}: {fps:.1f} frames/s", file=sys.stderr, flush=True)
-    bflags = sr.CFG_TIMING | sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
+    bflags = sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
+    _, _, st_t, tctx, tfbs = time_path(sr, scene, texs, args.views, bflags | sr.CFG_TIMING, args.tile_order,
+                                       args.bin_size, 10, args.warmup, sr.SponzaShader, k)
+    dev_b = statistics.mean(s["ms_total"] for s in st_t)
+    for fb in tfbs:
+        fb.destroy()
+    tctx.Shutdown()
     fps_b, ms_b, st_b, bctx, bfbs = time_path(sr, scene, texs, args.views, bflags, args.tile_order, args.bin_size,
                                               max(20, args.steps // args.views), args.warmup, sr.SponzaShader, k)
     layout = {"bin_size": st_b[-1]["bin_size"], "wg_per_tile": st_b[-1]["wg_per_tile"]}
-    dev_b = statistics.mean(s["ms_total"] for s in st_b)
     for fb in bfbs:
         fb.destroy()
     bctx.Shutdown()
@@ -586,11 +597,18 @@ def main():
     if args.no_plain:
         return profile_batched(sr, scene, texs, args)
     # ---- plain path: one view per EndFrame, no graph, natural tile order ----
-    fps_plain, ms_plain, st_plain, ctx, fbs = time_path(sr, scene, texs, 1, sr.CFG_TIMING, sr.ORDER_NATURAL,
-                                                        args.bin_size, args.steps, args.warmup)
+    # Frames/s of every path come from uninstrumented contexts; the device
+    # times (HIP events, SRG_CFG_TIMING) from a second, instrumented pass.
+    fps_plain, ms_plain, _, ctx, fbs = time_path(sr, scene, texs, 1, 0, sr.ORDER_NATURAL, args.bin_size, args.steps,
+                                                 args.warmup)
     print(f"[bench] plain: {fps_plain:.1f} frames/s", file=sys.stderr, flush=True)
     colour, depth = fbs[0].read_tiles()
     covered = int(np.count_nonzero(depth.reshape(-1).view(np.uint32) != 0))
+    for fb in fbs:
+        fb.destroy()
+    ctx.Shutdown()
+    _, _, st_plain, ctx, fbs = time_path(sr, scene, texs, 1, sr.CFG_TIMING, sr.ORDER_NATURAL, args.bin_size,
+                                         min(args.steps, 100), args.warmup)
     be_ms = statistics.mean(s["ms_backend"] for s in st_plain)
     fe_ms = statistics.mean(s["ms_frontend"] for s in st_plain)
     dev_ms = statistics.mean(s["ms_total"] for s in st_plain)
@@ -609,14 +627,20 @@ def main():
     fps_batched = ms_batched = dev_batched = None
     batched_runs = {}
     if not args.no_batched:
-        bflags = sr.CFG_TIMING | sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
+        bflags = sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
         orders = [args.tile_order] + ([sr.ORDER_TRICOUNT] if args.tile_order != sr.ORDER_TRICOUNT else [])
         for order in orders:
+            # device time per launch from an instrumented context first
+            _, _, st_t, tctx, tfbs = time_path(sr, scene, texs, args.views, bflags | sr.CFG_TIMING, order,
+                                               args.bin_size, 10, args.warmup)
+            dev_ms_launch = statistics.mean(s["ms_total"] for s in st_t)
+            for fb in tfbs:
+                fb.destroy()
+            tctx.Shutdown()
             fps, ms, st_b, bctx, bfbs = time_path(sr, scene, texs, args.views, bflags, order, args.bin_size,
                                                   max(20, args.steps // args.views), args.warmup)
             run = {"tile_order": ("natural", "tricount", "learned")[order], "frames_per_s": round(fps, 2),
-                   "ms_per_launch": round(ms, 4),
-                   "device_ms_per_launch": round(statistics.mean(s["ms_total"] for s in st_b), 4),
+                   "ms_per_launch": round(ms, 4), "device_ms_per_launch": round(dev_ms_launch, 4),
                    "layout": {"bin_size": st_b[-1]["bin_size"], "wg_per_tile": st_b[-1]["wg_per_tile"]}}
             if bflags & sr.CFG_AUTOTUNE:
                 t = bctx.tuning()

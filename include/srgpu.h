@@ -189,6 +189,10 @@ int srg_fb_write_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, const uint32_t*
 /* ---- frame: BeginFrame / ClearFrameBuffer / DrawIndexed / EndFrame / Blit ---- */
 int srg_begin_frame(srg_ctx* ctx);
 int srg_clear(srg_ctx* ctx, srg_fb* fb, uint32_t colour, int clear_colour, int clear_depth);
+/* srg_clear for n framebuffers in one call (a batch of views; stops at the
+ * first failing one). No counterpart in the reference: n ClearFrameBuffer
+ * calls (Renderer.cpp:168-194). */
+int srg_clear_n(srg_ctx* ctx, srg_fb* const* fbs, uint32_t n, uint32_t colour, int clear_colour, int clear_depth);
 int srg_draw_indexed(srg_ctx* ctx, const srg_draw* draw);
 /* n draws in one call, each as srg_draw_indexed; returns the first failing
  * status (the draws before it are recorded). No counterpart in the reference,

@@ -48,7 +48,7 @@ SRG_SYMBOLS = (
     "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
     "srg_get_tuning", "srg_draw_indexed_n", "srg_obj_load", "srg_obj_parse", "srg_obj_mesh_count",
     "srg_obj_get_mesh", "srg_obj_material_count", "srg_obj_material_name", "srg_obj_material_diffuse",
-    "srg_obj_free", "srg_set_sponza_constants",
+    "srg_obj_free", "srg_set_sponza_constants", "srg_clear_n",
 )
 OBJ_FLIP_WINDING, OBJ_FLIP_UVS = 1, 4  # Obj::LoadFlags (Viewer/Obj.h:54-60)
 PLANE_WRITE = 0xFFFFFFFF
@@ -161,6 +161,7 @@ def load_library(path=None):
         "srg_obj_material_diffuse": (ctypes.c_char_p, [vp, u32]),
         "srg_obj_free": (None, [vp]),
         "srg_set_sponza_constants": (i32, [vp, vp]),
+        "srg_clear_n": (i32, [vp, vp, u32, u32, i32, i32]),
         "srg_texture_build_rgba8": (i32, [vp, u32, u32, i32, vp, pu64, ctypes.POINTER(u32),
                                           ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)]),
     }
@@ -547,6 +548,19 @@ class RenderContext:
     def ClearFrameBuffer(self, fb, color=0, clearColour=True, clearDepth=True):
         _check(self._lib.srg_clear(self._ctx, fb._h, color & 0xFFFFFFFF, int(clearColour), int(clearDepth)),
                "ClearFrameBuffer")
+
+    def PrepareClearList(self, fbs):
+        """The framebuffers of a batch of views, for ClearFrameBuffers."""
+        fbs = list(fbs)
+        arr = (ctypes.c_void_p * max(1, len(fbs)))(*[fb._h for fb in fbs])
+        return (arr, len(fbs), fbs)
+
+    def ClearFrameBuffers(self, clear_list, color=0, clearColour=True, clearDepth=True):
+        """ClearFrameBuffer for every framebuffer of a PrepareClearList, in one
+        library call (srg_clear_n)."""
+        arr, n, _ = clear_list
+        _check(self._lib.srg_clear_n(self._ctx, arr, n, color & 0xFFFFFFFF, int(clearColour), int(clearDepth)),
+               "ClearFrameBuffers")
 
     def DrawIndexed(self, call):
         ib, pb, ab = call.m_indexBuffer, call.m_positionBuffer, call.m_attributeBuffer

@@ -471,6 +471,8 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
     unsigned long long frags = 0;
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
     const uint32_t qlen = F.counters[C_QLEN] * PARTS;  // strips of the queued bins, a bin's strips adjacent
+    // per-bin costs: requested, and (for the learned order) a recording frame
+    const bool timed = (F.flags & FF_TILE_COSTS) && (!F.sched_gate || F.sched_gate[1]);
 
     for (;;) {
         if (tid == 0) s_q = atomicAdd(&F.counters[C_QHEAD], 1u);
@@ -478,7 +480,7 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
         const uint32_t qi = s_q;
         __syncthreads();
         if (qi >= qlen) break;
-        const uint64_t t_start = (F.flags & FF_TILE_COSTS) ? clock64() : 0;
+        const uint64_t t_start = timed ? clock64() : 0;
         const uint32_t gb = F.queue[qi / PARTS];
         const ViewDev V = F.views[view_of_bin(F, gb)];
         const uint32_t lb = gb - V.bin_base;
@@ -547,7 +549,7 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
             }
         }
         // a bin's cost is the sum over its strips (zeroed at frame start)
-        if ((F.flags & FF_TILE_COSTS) && tid == 0)
+        if (timed && tid == 0)
             atomicAdd((unsigned long long*)&F.bin_cost[gb], (unsigned long long)(clock64() - t_start));
         __syncthreads();
     }

@@ -88,6 +88,7 @@ struct FrameDev {
     float4* rec_shade;     // 4 x float4 per record: 1/w plane and attribute planes 0-3
     float4* rec_ext;       // 3 x float4 per record: attribute planes 4-7 (wide frames only)
     const SponzaDev* sponza;  // SponzaShader constants (wide frames only)
+    const uint32_t* sched_gate;  // learned order: [1] != 0 on frames that predict and record (else null)
     uint32_t* rec_meta;    // draw index; 0xFFFFFFFF = no primitive
     uint32_t* clip_queue;  // global triangle ids routed to the clipper
     uint32_t* clip_base;   // per global triangle: first fan record (relative to ntris)

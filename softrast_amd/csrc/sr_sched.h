@@ -163,6 +163,7 @@ __global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, c
     // An overflowed frame has bin starts past the entry arena and no entry
     // lists (k_emit skipped it); it is rendered again with larger arenas.
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
+    if (F.sched_gate && !F.sched_gate[1]) return;  // a frame between recordings keeps the last prediction
     if (params) {
         for (uint32_t i = threadIdx.x; i < kHid * kFeat; i += 256) (&P.w1[0][0])[i] = (&params->w1[0][0])[i];
         for (uint32_t i = threadIdx.x; i < kHid * kHid; i += 256) P.w2t[i % kHid][i / kHid] = (&params->w2[0][0])[i];
@@ -184,6 +185,7 @@ __global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, c
 __global__ __launch_bounds__(256) void k_mlp_record(FrameDev F, const float4* feat, const float* pred,
                                                     SchedSample* ring, uint32_t ring_cap, uint32_t* ring_head)
 {
+    if (F.sched_gate && !F.sched_gate[1]) return;  // not a recording frame
     const uint32_t gb = blockIdx.x * 256u + threadIdx.x;
     // a frame that overflowed an arena is rendered again; its costs are not recorded
     const bool clean = (F.counters[C_FLAGS] & (FLAG_CLIP_OVERFLOW | FLAG_ENTRY_OVERFLOW)) == 0;

@@ -34,11 +34,17 @@
 namespace srg {
 
 // Frame-start fill: counters, per-(part, bin) counts and cursors, per-bin
-// costs (graph-safe).
+// costs (graph-safe). With the learned order, gate[1] says whether this frame
+// predicts and records (every `every`-th frame, counted in gate[0] on the
+// device so that replayed graphs count too).
 __global__ void k_frame_init(uint32_t* counters, uint32_t* part_count, uint32_t* part_cursor, uint64_t* bin_cost,
-                             uint32_t nbins)
+                             uint32_t nbins, uint32_t* gate, uint32_t every)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gate && i == 0) {
+        gate[1] = (every <= 1u || gate[0] % every == 0u) ? 1u : 0u;
+        gate[0] = gate[0] + 1u;
+    }
     if (i < C_NCOUNTERS) counters[i] = 0;
     for (uint32_t t = i; t < nbins * kBinParts; t += gridDim.x * blockDim.x) {
         part_count[t] = 0;
@@ -211,6 +217,9 @@ struct srg_ctx {
     void* sched_adam = nullptr;
     SchedSample* sched_ring = nullptr;
     uint32_t* sched_head = nullptr;
+    uint32_t* sched_gate = nullptr;   // [0] frame count, [1] this frame predicts and records
+    float sched_last_err = 0.0f;      // error of the last frame that recorded
+    bool sched_force_record = true;   // the next frame predicts and records (the model just changed)
     DevArray<float4> sched_feat;
     DevArray<float> sched_pred;
     bool sched_trained = false;
@@ -344,8 +353,10 @@ int ensure_sched(srg_ctx* ctx)
     if (hipMalloc(&ctx->sched_params, sizeof(MlpParams)) != hipSuccess ||
         hipMalloc(&ctx->sched_adam, sizeof(AdamState)) != hipSuccess ||
         hipMalloc((void**)&ctx->sched_ring, sizeof(SchedSample) * kSchedRing) != hipSuccess ||
-        hipMalloc((void**)&ctx->sched_head, sizeof(uint32_t)) != hipSuccess)
+        hipMalloc((void**)&ctx->sched_head, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&ctx->sched_gate, 2 * sizeof(uint32_t)) != hipSuccess)
         return SRG_ERR_NOMEM;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->sched_gate, 0u, (size_t)2);
     hipLaunchKernelGGL(k_mlp_init, dim3((kMlpParams + 255) / 256), dim3(256), 0, ctx->stream,
                        (MlpParams*)ctx->sched_params, (AdamState*)ctx->sched_adam, 0x5EED5C4Du);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->sched_head, 0u, (size_t)1);
@@ -356,6 +367,10 @@ int ensure_sched(srg_ctx* ctx)
 // Mean |log2 predicted - log2 measured| bin cost above which the learned
 // order gives way to the entry-count order (a factor of 2 off on average).
 constexpr float kSchedFallbackErr = 1.0f;
+// Once trained, the model's features, prediction and cost recording run on
+// every kSchedEvery-th frame; the frames between order their queue by the
+// last prediction (bins keep their numbers across frames of the same views).
+constexpr uint32_t kSchedEvery = 4;
 
 // A new bin size: the recorded samples and the model describe other bins, so
 // the model starts again from its initial weights and an empty ring.
@@ -367,6 +382,8 @@ int reset_sched(srg_ctx* ctx)
     HIP_TRY(hipGetLastError());
     ctx->sched_trained = false;
     ctx->sched_fallback = false;
+    ctx->sched_force_record = true;
+    ctx->sched_last_err = 0.0f;
     ctx->sched_frames = 0;
     ctx->sched_last_train = 0;
     return SRG_OK;
@@ -376,11 +393,17 @@ int reset_sched(srg_ctx* ctx)
 // once enough frames are recorded (and again every retrain interval).
 int sched_after_frame(srg_ctx* ctx)
 {
+    ctx->sched_force_record = false;
     const uint32_t n = ctx->h_counters[C_SCHED_N];
     float err;
     std::memcpy(&err, &ctx->h_counters[C_SCHED_ERR], sizeof(err));
-    ctx->stats.sched_err = (ctx->sched_trained && n) ? err / (float)n : 0.0f;
-    if (ctx->sched_trained && n) ctx->sched_fallback = ctx->stats.sched_err > kSchedFallbackErr;
+    // a trained model predicts and records every kSchedEvery-th frame; the
+    // other frames keep the last measured error and fallback decision
+    if (ctx->sched_trained && n) {
+        ctx->sched_last_err = err / (float)n;
+        ctx->sched_fallback = ctx->sched_last_err > kSchedFallbackErr;
+    }
+    ctx->stats.sched_err = ctx->sched_trained ? ctx->sched_last_err : 0.0f;
     ctx->sched_frames++;
     const uint32_t warm = ctx->cfg.sched_warmup_frames ? ctx->cfg.sched_warmup_frames : 8u;
     uint32_t steps = 0;
@@ -400,6 +423,7 @@ int sched_after_frame(srg_ctx* ctx)
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         ctx->sched_trained = true;
+        ctx->sched_force_record = true;
         ctx->sched_last_train = ctx->sched_frames;
         ctx->stats.sched_trainings++;
     }
@@ -713,11 +737,22 @@ int run_frame(srg_ctx* ctx)
         // the model orders the queue once trained, unless its last measured
         // error was too large: then the entry-count order is the fallback
         const bool use_model = ctx->sched_trained && !ctx->sched_fallback;
+        // predict and record every frame until trained, then every
+        // kSchedEvery-th frame (the count restarts, outside any captured
+        // graph, when the model changes, so that frame records); a caller
+        // that asked for tile costs gets them every frame
+        const uint32_t sched_every = (!ctx->sched_trained || (ctx->cfg.flags & SRG_CFG_TILE_COSTS)) ? 1u : kSchedEvery;
+        F.sched_gate = learned ? ctx->sched_gate : nullptr;
+        if (learned && ctx->sched_force_record) {
+            hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->sched_gate, 0u, (size_t)1);
+            HIP_TRY(hipGetLastError());
+        }
         // The whole frame on the stream: tables, six kernels, counters back.
         auto enqueue = [&](bool split_events) -> int {
             HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
             hipLaunchKernelGGL(k_frame_init, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->counters.ptr,
-                               ctx->part_count.ptr, ctx->part_cursor.ptr, ctx->bin_cost.ptr, TB);
+                               ctx->part_count.ptr, ctx->part_cursor.ptr, ctx->bin_cost.ptr, TB,
+                               learned ? ctx->sched_gate : nullptr, sched_every);
             if (ctx->cfg.flags & SRG_CFG_KEEP_PRIMID)
                 for (auto& v : vdev)
                     if (v.primid)
@@ -776,7 +811,8 @@ int run_frame(srg_ctx* ctx)
             const uint64_t extra[] = { (uint64_t)(uintptr_t)ctx->h_tables, (uint64_t)(uintptr_t)ctx->tables.ptr,
                                        (uint64_t)(uintptr_t)ctx->h_counters, nblocks, emit_blocks, clip_blocks,
                                        init_blocks, TB, K, L.parts, ctx->cfg.flags, ctx->cfg.wg_per_cu,
-                                       (uint64_t)ctx->sched_trained | ((uint64_t)ctx->sched_fallback << 1), ctx->cfg.tile_order,
+                                       (uint64_t)ctx->sched_trained | ((uint64_t)ctx->sched_fallback << 1) |
+                                           ((uint64_t)sched_every << 2), ctx->cfg.tile_order,
                                        (uint64_t)(uintptr_t)ctx->sched_feat.ptr, (uint64_t)(uintptr_t)ctx->sched_pred.ptr };
             sig.insert(sig.end(), (const uint8_t*)&F, (const uint8_t*)&F + sizeof(F));
             sig.insert(sig.end(), (const uint8_t*)extra, (const uint8_t*)extra + sizeof(extra));
@@ -912,6 +948,7 @@ void srg_destroy(srg_ctx* ctx)
     if (ctx->sched_adam) (void)hipFree(ctx->sched_adam);
     if (ctx->sched_ring) (void)hipFree(ctx->sched_ring);
     if (ctx->sched_head) (void)hipFree(ctx->sched_head);
+    if (ctx->sched_gate) (void)hipFree(ctx->sched_gate);
     ctx->sched_feat.release();
     ctx->sched_pred.release();
     for (auto& kv : ctx->buffers)
@@ -1174,6 +1211,14 @@ int srg_clear(srg_ctx* ctx, srg_fb* fb, uint32_t colour, int clear_colour, int c
     return SRG_OK;
 }
 
+int srg_clear_n(srg_ctx* ctx, srg_fb* const* fbs, uint32_t n, uint32_t colour, int clear_colour, int clear_depth)
+{
+    if (!ctx || (n && !fbs)) return SRG_ERR_INVALID;
+    for (uint32_t i = 0; i < n; ++i)
+        if (const int st = srg_clear(ctx, fbs[i], colour, clear_colour, clear_depth)) return st;
+    return SRG_OK;
+}
+
 int srg_draw_indexed(srg_ctx* ctx, const srg_draw* d)
 {
     if (!ctx || !d) return SRG_ERR_INVALID;
@@ -1343,6 +1388,7 @@ int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count)
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->sched_trained = true;
     ctx->sched_fallback = false;
+    ctx->sched_force_record = true;
     return SRG_OK;
 }
 

@@ -155,8 +155,9 @@ def test_layouts_same_image(bin_size, wg_per_tile):
 @pytest.mark.parametrize("bin_size,wg_per_tile", [(8, 1), (32, 4), (64, 2), (64, 4)])
 def test_layouts_clustered_learned(bin_size, wg_per_tile):
     """Strips and one-wave bins under the learned order: the image is the
-    oracle's, and every non-empty bin has a recorded cost (summed over its
-    strips)."""
+    oracle's, and on every frame that records (each frame until trained, the
+    frame after training, then every fourth) every non-empty bin has a
+    recorded cost (summed over its strips)."""
     import softrast_amd as sr
     from softrast_amd import scenes
     sc = scenes.clustered_scene(ntris=60000, width=960, height=544, ndraws=2, tex_size=128)
@@ -164,16 +165,21 @@ def test_layouts_clustered_learned(bin_size, wg_per_tile):
     ref, rstats = helpers.oracle_render(sc, tex, nthreads=8)
     ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=sr.ORDER_LEARNED,
                            bin_size=bin_size, wg_per_tile=wg_per_tile, sched_warmup_frames=1)
-    for frame in range(3):
+    recorded = []
+    for frame in range(6):
         _say(f"[parity] case=layout_clustered {bin_size}x{wg_per_tile} frame={frame}")
         col, dep, pid, st = helpers.gpu_render(sc, tex, ctx=ctx)
         d = helpers.compare(ref, col, dep, pid)
         assert not any(d.values()), (frame, d)
         assert (st["bin_size"], st["wg_per_tile"]) == (bin_size, wg_per_tile)
+        counts, costs = ctx.bin_stats()
+        assert len(counts) == ((sc.width + 63) // 64) * ((sc.height + 63) // 64) * (64 // bin_size) ** 2
+        if costs.any():
+            assert np.all(costs[counts > 0] > 0), frame
+            recorded.append(frame)
     assert st["frags_raster"] == rstats["frags_raster"] and st["sched_trained"]
-    counts, costs = ctx.bin_stats()
-    assert len(counts) == ((sc.width + 63) // 64) * ((sc.height + 63) // 64) * (64 // bin_size) ** 2
-    assert np.all(costs[counts > 0] > 0)
+    # frames 0 (untrained) and 1 (right after training) record, then one in four
+    assert recorded[:2] == [0, 1] and 2 <= len(recorded) <= 4, recorded
     ctx.Shutdown()
 
 
