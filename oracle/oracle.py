@@ -36,7 +36,8 @@ class SroPlane(ctypes.Structure):
 
 class SroStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
-                ("tris_in", "tris_clipped", "prims_binned", "bin_entries", "frags_raster", "frags_passed")]
+                ("tris_in", "tris_clipped", "prims_binned", "bin_entries", "frags_raster", "frags_passed",
+                 "blocks_tested")]
 
 
 _libs = {}

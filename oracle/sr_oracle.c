@@ -499,7 +499,7 @@ typedef struct {
     chunk_ref* refs;
     uint32_t ref_cap;
     /* statistics */
-    uint64_t tris_clipped, prims_binned, bin_entries, frags_raster, frags_passed;
+    uint64_t tris_clipped, prims_binned, bin_entries, frags_raster, frags_passed, blocks_tested;
     int error;
 } thread_state;
 
@@ -727,6 +727,7 @@ static void raster_chunk(thread_state* ts, const chunk_t* ch, uint32_t chunk_pos
                            (((int32_t)(c + dy * X1 + dx * Y0) > 0) << 2) | (((int32_t)(c + dy * X1 + dx * Y1) > 0) << 3);
                 }
                 if (!m[0] || !m[1] || !m[2]) continue;
+                ts->blocks_tested++;
                 int depth_only = m[0] == 0xF && m[1] == 0xF && m[2] == 0xF;
                 float z[8];
                 for (int l = 0; l < 8; ++l) z[l] = fmaf((float)yb, zdy, top[l]) + (float)xb * zdx;
@@ -1118,6 +1119,7 @@ int sro_render(sro_ctx* ctx, const sro_draw* draws, uint32_t ndraws, sro_plane* 
         for (uint32_t b = 0; b < ts->nbins; ++b) ts->bins[b].n = 0;
         ts->nchunks = 0;
         ts->tris_clipped = ts->prims_binned = ts->bin_entries = ts->frags_raster = ts->frags_passed = 0;
+        ts->blocks_tested = 0;
         ts->error = 0;
     }
 
@@ -1152,6 +1154,7 @@ int sro_render(sro_ctx* ctx, const sro_draw* draws, uint32_t ndraws, sro_plane* 
         st.bin_entries += ts->bin_entries;
         st.frags_raster += ts->frags_raster;
         st.frags_passed += ts->frags_passed;
+        st.blocks_tested += ts->blocks_tested;
     }
     if (stats) *stats = st;
     return 0;

@@ -91,6 +91,9 @@ typedef struct sro_stats {
     uint64_t bin_entries;    /* triangle-tile entries */
     uint64_t frags_raster;   /* pixel samples passing coverage (before depth) */
     uint64_t frags_passed;   /* fragments passing the depth test (shaded) */
+    uint64_t blocks_tested;  /* 8x8 blocks that pass the block reject test (Rasterizer.cpp:230-261) and reach
+                                ComputeBlockMask8x8 or its depth-only variant (:263-272); no pixel depends on
+                                which blocks these are, so this count is what observes the reject test */
 } sro_stats;
 
 typedef struct sro_ctx sro_ctx;

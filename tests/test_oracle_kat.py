@@ -629,3 +629,362 @@ def test_thread_count_and_build_independence():
         assert np.array_equal(a.depth.view(np.uint32), q.depth.view(np.uint32))
         assert np.array_equal(a.primid, q.primid)
     assert sa == sb == sc_
+
+
+# ---------------------------------------------------------------- bin and block walk
+#
+# The bin corner test (Binning.cpp:352-409) and the block reject test
+# (Rasterizer.cpp:230-261) never decide a pixel: a pixel is covered only where
+# every edge value is >= 0, and a box whose four corners all have E <= 0 can
+# hold an E == 0 pixel only on its top or left side, where E <= 0 on the whole
+# quadrant to the lower right, so every block of the walk rejects it as well.
+# What they decide is the reference's work: which tiles receive an entry and
+# which blocks reach ComputeBlockMask8x8. Those counts are checked here
+# against values derived by hand from the geometry, and against an exact
+# integer walk on lattice triangles whose edges pass through bin and block
+# corners (E == 0 at the test points).
+
+def _raster_tri(pts, W, H, z=0.5):
+    """Vertex rows (clip space, w = 1) that land exactly on raster points `pts`
+    (pixels, multiples of 1/256) on a W x H target, W and H powers of two;
+    reordered so that the triangle faces the camera (Binning.cpp:305-311)."""
+    (x0, y0), (x1, y1), (x2, y2) = pts
+    if (x2 - x0) * (y1 - y0) - (y2 - y0) * (x1 - x0) <= 0:
+        pts = [pts[0], pts[2], pts[1]]
+    hw, hh = W / 2, H / 2
+    v = np.zeros((3, 8), np.float32)
+    for k, (x, y) in enumerate(pts):
+        v[k, 0:3] = [(x - hw) / hw, (hh - y) / hh, z]
+    return v, [(int(round(x * 256)), int(round(y * 256))) for x, y in pts]
+
+
+def _render_tris(tris, W, H, shader=2, tex=None, mvp=None, uv_offset=6):
+    """Renders vertex rows (3 per triangle) in one draw; returns (plane, stats)."""
+    verts = np.ascontiguousarray(np.concatenate(tris), np.float32)
+    o = O.Oracle(nthreads=1)
+    p = O.Plane(W, H)
+    o.clear(p)
+    idx = np.arange(len(verts), dtype=np.uint32)
+    st = o.render([dict(shader=shader, tex=tex, idx=idx, idx_stride=4, pos=verts, pos_stride=32,
+                        pos_num=len(verts), attr=verts, attr_stride=32, attr_num=len(verts), uv_offset=uv_offset,
+                        plane=0, mvp=np.eye(4, dtype=np.float32) if mvp is None else mvp)], [p], tile_entries=True)
+    o.close()
+    return p, st
+
+
+def _tiles_with_entries(p, st):
+    return {(int(b) % p.tiles_x, int(b) // p.tiles_x) for b in np.nonzero(st["tile_entries"])[0]}
+
+
+def test_bin_corner_test_drops_tiles_beyond_the_hypotenuse(lib):
+    """Right triangle (0.5, 0.5), (400.5, 0.5), (0.5, 400.5) on 512 x 512: its
+    bbox covers bins 0..6 in x and y (7 rows, so the test runs). The
+    hypotenuse is x + y = 401. A bin's top-left corner (64 bx, 64 by) lies
+    inside iff bx + by <= 6; for bx + by >= 7 all four corners have
+    x + y >= 448, so that edge is <= 0 at each of them and the bin is dropped.
+    The legs x = 0.5 and y = 0.5 are > 0 at every bin's right/bottom corners.
+    Expected: the 28 bins with bx + by <= 6 (49 without the test)."""
+    W = H = 512
+    v, fp = _raster_tri([(0.5, 0.5), (400.5, 0.5), (0.5, 400.5)], W, H)
+    p, st = _render_tris([v], W, H)
+    want = {(bx, by) for bx in range(7) for by in range(7) if bx + by <= 6}
+    assert len(want) == 28
+    assert _tiles_with_entries(p, st) == want
+    assert st["bin_entries"] == 28
+    # the pixels are those of the exact edge functions
+    got = {(x, y) for y in range(H) for x in range(W) if p.primid.reshape(8, 8, 64, 64)[y >> 6, x >> 6, y & 63,
+                                                                                       x & 63] != 0xFFFFFFFF}
+    assert got == _exact_coverage(fp, W, H)
+
+
+def test_bin_test_skipped_on_two_bin_rows_numxbins_quirk(lib):
+    """Thin wedge (0.5, 0.5), (500.5, 0.5), (0.5, 100.5): bins 0..7 in x, 0..1
+    in y. Binning.cpp:359 computes numXbins from the rows, so with two bin
+    rows the corner test is skipped however many columns there are, and all
+    16 bins of the bbox get an entry. (Counting the columns would run the
+    test: row 1's top-left corners (64 bx, 64) lie inside the hypotenuse
+    x/500 + y/100 = 1 only for bx <= 2, leaving 8 + 3 = 11 bins.)"""
+    W = H = 512
+    v, _ = _raster_tri([(0.5, 0.5), (500.5, 0.5), (0.5, 100.5)], W, H)
+    p, st = _render_tris([v], W, H)
+    assert _tiles_with_entries(p, st) == {(bx, by) for bx in range(8) for by in range(2)}
+    assert st["bin_entries"] == 16
+    # three bin rows: the test runs and drops the far corner bins
+    v, _ = _raster_tri([(0.5, 0.5), (500.5, 0.5), (0.5, 130.5)], W, H)
+    p, st = _render_tris([v], W, H)
+    tiles = _tiles_with_entries(p, st)
+    assert (7, 2) not in tiles and (0, 2) in tiles and len(tiles) < 24
+
+
+def test_block_reject_uses_64_wide_corners(lib):
+    """Triangle (63, 40), (63, 63), (40, 63) in one tile (hypotenuse x + y = 103):
+    blocks xb, yb in {40, 48, 56} are walked. Rasterizer.cpp:230-234 tests the
+    corners {xb, xb + 64} x {yb, yb + 64}; (xb + 64, yb + 64) is inside the
+    hypotenuse for every block, so all 9 blocks reach the mask computation.
+    With corners 8 apart, block (40, 40) would be dropped (its corners have
+    x + y <= 96), leaving 8. The same triangle in tile (1, 1) checks the tile
+    re-basing of the edge constants (Binning.cpp:420-424)."""
+    for off in (0, 64):
+        W = H = 128
+        v, fp = _raster_tri([(63 + off, 40 + off), (63 + off, 63 + off), (40 + off, 63 + off)], W, H)
+        p, st = _render_tris([v], W, H)
+        assert st["bin_entries"] == 1
+        assert st["blocks_tested"] == 9, off
+        got = {(x, y) for y in range(H) for x in range(W)
+               if p.primid.reshape(2, 2, 64, 64)[y >> 6, x >> 6, y & 63, x & 63] != 0xFFFFFFFF}
+        assert got == _exact_coverage(fp, W, H)
+
+
+def _exact_walk(fp, W, H):
+    """The reference's walk of one triangle in exact integers: tiles with an
+    entry (bin corner test, skipped when the bbox spans <= 2 bin rows, as
+    numXbins is computed from the rows, Binning.cpp:352-409), blocks passing
+    the reject test with corners 64 apart (Rasterizer.cpp:199-261) and the
+    covered pixels. E(X, Y) = floor((E_full(256X, 256Y) + bias) / 256)."""
+    (x0, y0), (x1, y1), (x2, y2) = fp
+    edges = []
+    for (ax, ay), (bx, by) in (((x0, y0), (x1, y1)), ((x1, y1), (x2, y2)), ((x2, y2), (x0, y0))):
+        dy, dx = by - ay, ax - bx
+        edges.append((ax, ay, bx, by, 256 if (dy < 0 or (dy == 0 and dx > 0)) else 0))
+
+    def E(e, X, Y):
+        ax, ay, bx, by, b = e
+        return ((by - ay) * (256 * X - ax) - (bx - ax) * (256 * Y - ay) + b) // 256
+
+    def ceil_px(v):
+        return -((-v) // 256)
+
+    xmin = min(max(ceil_px(min(x0, x1, x2)), 0), W - 1)
+    xmax = min(max(ceil_px(max(x0, x1, x2)), 0), W - 1)
+    ymin = min(max(ceil_px(min(y0, y1, y2)), 0), H - 1)
+    ymax = min(max(ceil_px(max(y0, y1, y2)), 0), H - 1)
+    check = (ymax >> 6) - (ymin >> 6) + 1 > 2
+
+    def any_corner_inside(e, X, Y, size):
+        return any(E(e, X + cx, Y + cy) > 0 for cx in (0, size) for cy in (0, size))
+
+    tiles, blocks, cov = set(), 0, set()
+    ly, lx = np.mgrid[0:8, 0:8]
+    for ty in range(ymin >> 6, (ymax >> 6) + 1):
+        for tx in range(xmin >> 6, (xmax >> 6) + 1):
+            X0, Y0 = 64 * tx, 64 * ty
+            if check and not all(any_corner_inside(e, X0, Y0, 64) for e in edges):
+                continue
+            tiles.add((tx, ty))
+            bminx, bmaxx = min(max(xmin - X0, 0), 64), min(max(xmax - X0, 0), 64)
+            bminy, bmaxy = min(max(ymin - Y0, 0), 64), min(max(ymax - Y0, 0), 64)
+            for yb in range(bminy & ~7, bmaxy, 8):
+                for xb in range(bminx & ~7, bmaxx, 8):
+                    if not all(any_corner_inside(e, X0 + xb, Y0 + yb, 64) for e in edges):
+                        continue
+                    blocks += 1
+                    X, Y = X0 + xb + lx, Y0 + yb + ly
+                    ok = np.ones((8, 8), bool)
+                    for ax, ay, bx, by, b in edges:
+                        ok &= (by - ay) * (256 * X - ax) - (bx - ax) * (256 * Y - ay) + b >= 0
+                    cov.update(zip((X[ok]).tolist(), (Y[ok]).tolist()))
+    return tiles, blocks, cov
+
+
+def test_walk_matches_exact_model_on_lattice_triangles(lib):
+    """Large triangles with vertices on the 64- and 8-pixel lattices (edges
+    through bin and block corners, E == 0 at test points), thin wide and
+    thin tall ones among them: tiles with entries, blocks tested and pixels
+    equal the exact integer walk."""
+    rng = np.random.default_rng(41)
+    W = H = 512
+    n = 0
+    for it in range(48):
+        grid = (64, 8, 1 / 256)[it % 3]
+        kind = it % 4
+        pts = rng.uniform(0, 511, (3, 2))
+        if kind == 1:    # wide and short: <= 2 bin rows, many columns
+            pts[:, 1] = rng.uniform(0, 120, 3) + rng.integers(0, 6) * 64
+        elif kind == 2:  # tall and narrow
+            pts[:, 0] = rng.uniform(0, 120, 3) + rng.integers(0, 6) * 64
+        pts = np.clip(np.round(pts / grid) * grid, 0, 511)
+        v, fp = _raster_tri([tuple(map(float, q)) for q in pts], W, H)
+        ok, ints, _ = _setup(lib, np.column_stack([v[:, 0:3], np.ones(3)]), np.zeros((3, 8)), W, H)
+        if not ok:
+            continue
+        assert [(ints[13], ints[14]), (ints[15], ints[16]), (ints[17], ints[18])] == fp
+        p, st = _render_tris([v], W, H)
+        tiles, blocks, cov = _exact_walk(fp, W, H)
+        assert _tiles_with_entries(p, st) == tiles, it
+        assert st["bin_entries"] == len(tiles)
+        assert st["blocks_tested"] == blocks, it
+        pid = p.primid.reshape(8, 8, 64, 64)
+        got = {(int(tx) * 64 + int(xx), int(ty) * 64 + int(yy)) for ty, tx, yy, xx in
+               zip(*np.nonzero(pid != 0xFFFFFFFF))}
+        assert got == cov, it
+        n += 1
+    assert n >= 30
+
+
+# ---------------------------------------------------------------- uv derivatives through the pipeline
+#
+# Rasterizer.cpp:373-400: du/dx = rcp(q(x+1, y)) * U(x+1, y) - u(x, y) and
+# du/dy = rcp(q(x, y+1)) * U(x, y+1) - u(x, y), q the 1/w plane, U the u/w
+# plane, rcp the RCPPS approximation (pinned above by recorded outputs), and
+# the same for v; Texture.cpp:212-233 turns them into a mip level. A texture
+# whose every level is one flat colour makes the level the oracle picked
+# readable from each pixel; the expected level comes from float64 planes.
+
+def _level_texture(w, h):
+    """TextureData layout (Texture.cpp:145-170) with level k filled with red = 20k + 10."""
+    from types import SimpleNamespace
+    nm = int(np.log2(max(w, h))) + 1
+    offs, chunks = [], []
+    off = 0
+    for k in range(nm):
+        mw, mh = max(1, w >> k), max(1, h >> k)
+        pw, ph = -(-mw // 32) * 32, -(-mh // 32) * 32
+        offs.append(off)
+        px = np.zeros((pw * ph, 4), np.uint8)
+        px[:, 0] = 20 * k + 10
+        px[:, 3] = 255
+        chunks.append(px.reshape(-1))
+        off += pw * ph * 4
+    t = SimpleNamespace(m_texels=np.concatenate(chunks), m_mipOffsets=offs + [0] * (14 - nm),
+                        m_widthLog2=int(np.log2(w)), m_heightLog2=int(np.log2(h)), m_numMips=nm)
+    return O.make_texture(t)
+
+
+def _levels_seen(p):
+    """Level read back from each written pixel (red = 20k + 10), -1 elsewhere."""
+    red = (p.colour & 0xFF).astype(np.int64)
+    lev = np.where(p.primid != 0xFFFFFFFF, np.round((red - 10) / 20.0).astype(np.int64), -1)
+    assert np.all(np.abs(red[lev >= 0] - (20 * lev[lev >= 0] + 10)) <= 1)
+    return lev.reshape(p.tiles_y, p.tiles_x, 64, 64)
+
+
+def _plane64(rx, ry, vals):
+    return np.linalg.solve(np.column_stack([rx, ry, np.ones(3)]), np.asarray(vals, np.float64))
+
+
+def _expected_levels(lib, rx, ry, invw, us, vs, W, H, tw, th, nmips, pixels, rcp_fixed=None):
+    """Level per pixel from float64 planes; None where the floor of log2 is
+    not decided within the rounding of the fp32 path. The x+1 / y+1
+    neighbours' 1/q go through RCPPS: for w = 1 its input is exactly 1.0
+    (rcp_fixed); otherwise it is taken on the float64 q, and a pixel whose q
+    is within 2^-16 of an RCPPS table step is left undecided."""
+    cq = _plane64(rx, ry, invw)
+    cu = _plane64(rx, ry, np.asarray(us) * invw)
+    cv = _plane64(rx, ry, np.asarray(vs) * invw)
+
+    def ev(c, x, y):
+        return c[0] * x + c[1] * y + c[2]
+
+    def rcp(q):
+        if rcp_fixed is not None:
+            return rcp_fixed
+        r = [lib.sro_rcpps(F(f32(q * s))) for s in (1 - 2.0 ** -16, 1.0, 1 + 2.0 ** -16)]
+        return r[1] if r[0] == r[1] == r[2] else None
+
+    out = {}
+    for x, y in pixels:
+        q0 = ev(cq, x, y)
+        u0, v0 = ev(cu, x, y) / q0, ev(cv, x, y) / q0
+        r10, r01 = rcp(ev(cq, x + 1, y)), rcp(ev(cq, x, y + 1))
+        if r10 is None or r01 is None:
+            out[(x, y)] = None
+            continue
+        d = [ev(cu, x + 1, y) * r10 - u0, ev(cu, x, y + 1) * r01 - u0,
+             ev(cv, x + 1, y) * r10 - v0, ev(cv, x, y + 1) * r01 - v0]
+        scale = [tw, th, tw, th]  # Texture.cpp:217-221: d/dy scaled by the height
+        eps = (abs(u0) + abs(v0) + 1.0) * 2.0 ** -15
+        lo = [max(0.0, abs(a) * s - eps * s) for a, s in zip(d, scale)]
+        hi = [abs(a) * s + eps * s for a, s in zip(d, scale)]
+        m_lo = np.sqrt(max(lo[0] ** 2 + lo[1] ** 2, lo[2] ** 2 + lo[3] ** 2))
+        m_hi = np.sqrt(max(hi[0] ** 2 + hi[1] ** 2, hi[2] ** 2 + hi[3] ** 2))
+
+        def level(m):
+            return 0 if m < 1.0 else min(nmips - 1, int(np.floor(np.log2(m))))
+
+        out[(x, y)] = level(m_lo) if level(m_lo) == level(m_hi) else None
+    return out
+
+
+def _check_levels(lib, p, want):
+    lev = _levels_seen(p)
+    decided, seen = 0, set()
+    for (x, y), w in want.items():
+        got = int(lev[y >> 6, x >> 6, y & 63, x & 63])
+        assert got >= 0, (x, y)
+        if w is None:
+            continue
+        assert got == w, ((x, y), got, w)
+        decided += 1
+        seen.add(w)
+    return decided, seen
+
+
+def test_mip_level_affine_quad_known_gradient(lib):
+    """w = 1: q is exactly 1, so iq = 1 and both neighbours take RCPPS(1.0) =
+    1 - 2^-12: du/dx = (1 - 2^-12) u(x+1, y) - u(x, y), du/dy = -2^-12 u.
+    256 x 64 texture, u = x / 12 + u0, v = y / 64. With u0 = 0 the texel
+    gradient is about (21.3 - u/16, u/64): level 4. With u0 = 290 the RCPPS
+    term takes 18.2-18.8 texels off du/dx and adds 4.5-4.7 to du/dy, giving
+    |grad u| of 5.3-5.6 texels: level 2 (exact reciprocals would keep 4)."""
+    st, keep = _level_texture(256, 64)
+    W = H = 128
+    r1 = lib.sro_rcpps(F(1.0))
+    assert r1 == 1.0 - 2.0 ** -12
+    for u0, lvl in ((0.0, 4), (290.0, 2)):
+        v, fp = _raster_tri([(8.0, 8.0), (120.0, 8.0), (8.0, 120.0)], W, H)
+        rx = np.array([q[0] for q in fp]) / 256.0
+        ry = np.array([q[1] for q in fp]) / 256.0
+        v[:, 6] = rx / 12 + u0
+        v[:, 7] = ry / 64
+        p, _ = _render_tris([v], W, H, shader=0, tex=st)
+        pid = p.primid.reshape(2, 2, 64, 64)
+        px = [(int(tx) * 64 + int(xx), int(ty) * 64 + int(yy)) for ty, tx, yy, xx in zip(*np.nonzero(pid != 0xFFFFFFFF))]
+        want = _expected_levels(lib, rx, ry, np.ones(3), v[:, 6], v[:, 7], W, H, 256, 64, 9, px, rcp_fixed=r1)
+        decided, seen = _check_levels(lib, p, want)
+        assert decided > 0.95 * len(px) and seen == {lvl}, (u0, decided, len(px), seen)
+
+
+def test_mip_level_perspective_floor_matches_float64(lib):
+    """A floor in perspective (w = depth, so 1/w changes down the screen and
+    not across it): the level of every pixel whose floor(log2) is decided
+    equals the float64 forward differences of Rasterizer.cpp:373-400. Taking
+    the y+1 neighbour's 1/w for the x derivative (or the reverse) is off by
+    u * (q(x, y)/q(x, y+1) - 1), several texels here, and fails."""
+    st, keep = _level_texture(256, 64)
+    W, H = 256, 256
+    # clip = (x, y, 0.1, z): column-major, z goes to w
+    mvp = np.zeros(16, np.float32)
+    mvp[0], mvp[5], mvp[11], mvp[14] = 1.0, 1.0, 1.0, 0.1
+    cases = [  # (u per world x, v per world z, u0, v0)
+        (2.0, 0.25, 0.0, 0.0), (0.25, 1.5, 1.0, 0.5), (3.0, 2.0, -2.0, 3.0)]
+    total, levels = 0, set()
+    for ku, kv, u0, v0 in cases:
+        # floor at y = -1.37: 1/w along a pixel row is then no short binary fraction
+        # (at y = -1 it would be (row - 128) / 128, always on an RCPPS table step)
+        quad = np.array([[-1.1, -1.37, 1.6], [1.1, -1.37, 1.6], [1.1, -1.37, 12.0], [-1.1, -1.37, 12.0]])
+        tris = []
+        rxs, rys, ws, us, vs = [], [], [], [], []
+        for tri in ((0, 1, 2), (0, 2, 3)):
+            v = np.zeros((3, 8), np.float32)
+            v[:, 0:3] = quad[list(tri)]
+            v[:, 6] = v[:, 0] * ku + u0
+            v[:, 7] = v[:, 2] * kv + v0
+            rx = (v[:, 0].astype(np.float64) / v[:, 2]) * (W / 2) + W / 2
+            ry = (v[:, 1].astype(np.float64) / v[:, 2]) * (-H / 2) + H / 2
+            if (rx[2] - rx[0]) * (ry[1] - ry[0]) - (ry[2] - ry[0]) * (rx[1] - rx[0]) <= 0:
+                v = v[[0, 2, 1]]
+                rx, ry = rx[[0, 2, 1]], ry[[0, 2, 1]]
+            tris.append(v)
+            rxs.append(rx), rys.append(ry), ws.append(1.0 / v[:, 2].astype(np.float64))
+            us.append(v[:, 6]), vs.append(v[:, 7])
+        for k in range(2):
+            p, _ = _render_tris([tris[k]], W, H, shader=0, tex=st, mvp=mvp)
+            pid = p.primid.reshape(p.tiles_y, p.tiles_x, 64, 64)
+            px = [(int(tx) * 64 + int(xx), int(ty) * 64 + int(yy)) for ty, tx, yy, xx in
+                  zip(*np.nonzero(pid != 0xFFFFFFFF))]
+            want = _expected_levels(lib, rxs[k], rys[k], ws[k], us[k], vs[k], W, H, 256, 64, 9, px)
+            decided, seen = _check_levels(lib, p, want)
+            assert decided > 0.8 * len(px), (ku, kv, k, decided, len(px))
+            total += decided
+            levels |= seen
+    assert total > 10000 and len(levels) >= 4, (total, levels)
