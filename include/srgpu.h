@@ -177,6 +177,10 @@ int srg_release_texture(srg_ctx* ctx, srg_handle h);
 /* ---- framebuffers: FrameBuffer (Renderer.h:75-108, Renderer.cpp:23-57) ---- */
 enum srg_fb_flags { SRG_FB_COLOUR = 1u, SRG_FB_DEPTH = 2u };
 int srg_fb_create(srg_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags, srg_fb** out);
+/* Destroying a framebuffer that draws of the open frame target makes
+ * srg_end_frame fail with SRG_ERR_HANDLE. Every entry point taking an srg_fb
+ * returns SRG_ERR_HANDLE for one that is not (or no longer) ctx's; a destroyed
+ * framebuffer is not dereferenced. */
 void srg_fb_destroy(srg_ctx* ctx, srg_fb* fb);
 uint32_t srg_fb_write_plane(const srg_fb* fb);
 void srg_fb_swap_planes(srg_fb* fb);
@@ -189,8 +193,9 @@ int srg_fb_write_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, const uint32_t*
 /* ---- frame: BeginFrame / ClearFrameBuffer / DrawIndexed / EndFrame / Blit ---- */
 int srg_begin_frame(srg_ctx* ctx);
 int srg_clear(srg_ctx* ctx, srg_fb* fb, uint32_t colour, int clear_colour, int clear_depth);
-/* srg_clear for n framebuffers in one call (a batch of views; stops at the
- * first failing one). No counterpart in the reference: n ClearFrameBuffer
+/* srg_clear for n framebuffers in one call (a batch of views). Every handle
+ * is checked first: with one unknown handle nothing is cleared and the call
+ * returns SRG_ERR_HANDLE. No counterpart in the reference: n ClearFrameBuffer
  * calls (Renderer.cpp:168-194). */
 int srg_clear_n(srg_ctx* ctx, srg_fb* const* fbs, uint32_t n, uint32_t colour, int clear_colour, int clear_depth);
 int srg_draw_indexed(srg_ctx* ctx, const srg_draw* draw);
@@ -201,7 +206,10 @@ int srg_draw_indexed(srg_ctx* ctx, const srg_draw* draw);
  * SRG_PLANE_WRITE keeps such a list valid across SwapPlanes. */
 #define SRG_PLANE_WRITE 0xFFFFFFFFu
 int srg_draw_indexed_n(srg_ctx* ctx, const srg_draw* draws, uint32_t n);
-/* Synchronous like the reference: on return every tile is rastered and shaded. */
+/* Synchronous like the reference: on return every tile is rastered and shaded.
+ * Every recorded draw is checked again first (buffers, texture and framebuffer
+ * still registered, sizes still in range): a handle released since its
+ * srg_draw_indexed fails the frame with SRG_ERR_HANDLE, drawing nothing. */
 int srg_end_frame(srg_ctx* ctx);
 /* Copies the write plane's colour to linear RGBA8 rows (width*height*4 bytes),
  * swaps the planes, then calls on_finish(user) (Renderer.cpp:350-372). */

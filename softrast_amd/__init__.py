@@ -243,6 +243,13 @@ class DeviceTexture:
                "texture_create_rgba8")
         self._ctx, self.handle = ctx, out.value
 
+    def release(self):
+        """Frees the device copy (srg_release_texture). A frame that drew with
+        it and has not ended fails at EndFrame with SRG_ERR_HANDLE."""
+        if self.handle:
+            _check(self._ctx._lib.srg_release_texture(self._ctx._ctx, self.handle), "release_texture")
+            self.handle = 0
+
     def read(self):
         """Host copy as a TextureData (tiled bytes and layout)."""
         lib, c = self._ctx._lib, self._ctx._ctx

@@ -883,6 +883,44 @@ int run_frame(srg_ctx* ctx)
 
 bool valid_handle(srg_ctx* ctx, srg_handle h) { return h && ctx->buffers.count(h); }
 
+bool fb_known(const srg_ctx* ctx, const srg_fb* fb)
+{
+    return fb && std::find(ctx->fbs.begin(), ctx->fbs.end(), fb) != ctx->fbs.end();
+}
+
+// The checks srg_draw_indexed makes. srg_end_frame makes them again for every
+// recorded draw: a buffer, texture or framebuffer may have been released
+// between the two calls, and the frame must then fail with a status instead of
+// reading a released handle.
+int validate_draw(srg_ctx* ctx, const srg_draw* d)
+{
+    if (!d->fb || (d->plane > 1 && d->plane != SRG_PLANE_WRITE)) return SRG_ERR_INVALID;
+    if (!fb_known(ctx, d->fb)) return SRG_ERR_HANDLE;
+    if (!valid_handle(ctx, d->index_buffer) || !valid_handle(ctx, d->position_buffer) ||
+        !valid_handle(ctx, d->attribute_buffer))
+        return SRG_ERR_HANDLE;
+    if (d->index_stride != 1 && d->index_stride != 2 && d->index_stride != 4) return SRG_ERR_INVALID;
+    if (d->position_stride < 12 || (d->position_stride & 3) || (d->position_offset & 3)) return SRG_ERR_INVALID;
+    if (d->attribute_stride == 0 || d->attribute_stride > 32 || (d->attribute_stride & 3) || (d->attribute_offset & 3))
+        return SRG_ERR_INVALID;
+    if (d->index_offset % d->index_stride) return SRG_ERR_INVALID;
+    uint32_t sa[kWideSlots], ns, ds;
+    int st = shader_slots(d->shader, d->uv_offset, d->attribute_stride / 4u, sa, ns, ds);
+    if (st) return st;
+    if (shader_samples(d->shader) && d->texture && !ctx->textures.count(d->texture)) return SRG_ERR_HANDLE;
+    const Buffer& ib = ctx->buffers.at(d->index_buffer);
+    const Buffer& pb = ctx->buffers.at(d->position_buffer);
+    const Buffer& ab = ctx->buffers.at(d->attribute_buffer);
+    const uint64_t ntri = d->index_num / 3u;
+    if (d->index_offset + ntri * 3u * d->index_stride > ib.bytes) return SRG_ERR_INVALID;
+    if (d->position_num && d->position_offset + (uint64_t)(d->position_num - 1) * d->position_stride + 12 > pb.bytes)
+        return SRG_ERR_INVALID;
+    if (d->attribute_num &&
+        d->attribute_offset + (uint64_t)(d->attribute_num - 1) * d->attribute_stride + d->attribute_stride > ab.bytes)
+        return SRG_ERR_INVALID;
+    return SRG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1119,7 +1157,9 @@ int srg_fb_create(srg_ctx* ctx, uint32_t width, uint32_t height, uint32_t flags,
 
 void srg_fb_destroy(srg_ctx* ctx, srg_fb* fb)
 {
-    if (!ctx || !fb) return;
+    if (!ctx || !fb_known(ctx, fb)) return;
+    for (srg_draw& d : ctx->draws)
+        if (d.fb == fb) d.fb = nullptr;  // srg_end_frame rejects the frame
     for (auto& p : fb->planes) (void)finish_blit(p);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
@@ -1144,6 +1184,7 @@ void srg_fb_swap_planes(srg_fb* fb)
 int srg_fb_read_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, uint32_t* colour, float* depth, uint32_t* primid)
 {
     if (!ctx || !fb || plane > 1) return SRG_ERR_INVALID;
+    if (!fb_known(ctx, fb)) return SRG_ERR_HANDLE;
     int st = flush_pending_clear(ctx, fb, plane);
     if (st) return st;
     const size_t n = (size_t)fb->tiles_x * fb->tiles_y * kTilePixels;
@@ -1164,6 +1205,7 @@ int srg_fb_read_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, uint32_t* colour
 int srg_fb_write_tiles(srg_ctx* ctx, srg_fb* fb, uint32_t plane, const uint32_t* colour, const float* depth)
 {
     if (!ctx || !fb || plane > 1) return SRG_ERR_INVALID;
+    if (!fb_known(ctx, fb)) return SRG_ERR_HANDLE;
     const size_t n = (size_t)fb->tiles_x * fb->tiles_y * kTilePixels;
     PlaneState& p = fb->planes[plane];
     stream_wait_blit_read(ctx, p);
@@ -1192,6 +1234,7 @@ int srg_begin_frame(srg_ctx* ctx)
 int srg_clear(srg_ctx* ctx, srg_fb* fb, uint32_t colour, int clear_colour, int clear_depth)
 {
     if (!ctx || !fb) return SRG_ERR_INVALID;
+    if (!fb_known(ctx, fb)) return SRG_ERR_HANDLE;
     PlaneState& p = fb->planes[fb->write_plane];
     // ClearFrameBuffer acts immediately in the reference, so within a frame
     // it always precedes the rasterisation done by EndFrame; the clear is
@@ -1214,6 +1257,11 @@ int srg_clear(srg_ctx* ctx, srg_fb* fb, uint32_t colour, int clear_colour, int c
 int srg_clear_n(srg_ctx* ctx, srg_fb* const* fbs, uint32_t n, uint32_t colour, int clear_colour, int clear_depth)
 {
     if (!ctx || (n && !fbs)) return SRG_ERR_INVALID;
+    // all or nothing: no plane is cleared when one handle is bad
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!fbs[i]) return SRG_ERR_INVALID;
+        if (!fb_known(ctx, fbs[i])) return SRG_ERR_HANDLE;
+    }
     for (uint32_t i = 0; i < n; ++i)
         if (const int st = srg_clear(ctx, fbs[i], colour, clear_colour, clear_depth)) return st;
     return SRG_OK;
@@ -1223,30 +1271,7 @@ int srg_draw_indexed(srg_ctx* ctx, const srg_draw* d)
 {
     if (!ctx || !d) return SRG_ERR_INVALID;
     if (!ctx->in_frame) return SRG_ERR_STATE;
-    if (!d->fb || (d->plane > 1 && d->plane != SRG_PLANE_WRITE)) return SRG_ERR_INVALID;
-    if (std::find(ctx->fbs.begin(), ctx->fbs.end(), d->fb) == ctx->fbs.end()) return SRG_ERR_HANDLE;
-    if (!valid_handle(ctx, d->index_buffer) || !valid_handle(ctx, d->position_buffer) ||
-        !valid_handle(ctx, d->attribute_buffer))
-        return SRG_ERR_HANDLE;
-    if (d->index_stride != 1 && d->index_stride != 2 && d->index_stride != 4) return SRG_ERR_INVALID;
-    if (d->position_stride < 12 || (d->position_stride & 3) || (d->position_offset & 3)) return SRG_ERR_INVALID;
-    if (d->attribute_stride == 0 || d->attribute_stride > 32 || (d->attribute_stride & 3) || (d->attribute_offset & 3))
-        return SRG_ERR_INVALID;
-    if (d->index_offset % d->index_stride) return SRG_ERR_INVALID;
-    uint32_t sa[kWideSlots], ns, ds;
-    int st = shader_slots(d->shader, d->uv_offset, d->attribute_stride / 4u, sa, ns, ds);
-    if (st) return st;
-    if (shader_samples(d->shader) && d->texture && !ctx->textures.count(d->texture)) return SRG_ERR_HANDLE;
-    const Buffer& ib = ctx->buffers[d->index_buffer];
-    const Buffer& pb = ctx->buffers[d->position_buffer];
-    const Buffer& ab = ctx->buffers[d->attribute_buffer];
-    const uint64_t ntri = d->index_num / 3u;
-    if (d->index_offset + ntri * 3u * d->index_stride > ib.bytes) return SRG_ERR_INVALID;
-    if (d->position_num && d->position_offset + (uint64_t)(d->position_num - 1) * d->position_stride + 12 > pb.bytes)
-        return SRG_ERR_INVALID;
-    if (d->attribute_num &&
-        d->attribute_offset + (uint64_t)(d->attribute_num - 1) * d->attribute_stride + d->attribute_stride > ab.bytes)
-        return SRG_ERR_INVALID;
+    if (const int st = validate_draw(ctx, d)) return st;
     ctx->draws.push_back(*d);
     if (d->plane == SRG_PLANE_WRITE) ctx->draws.back().plane = d->fb->write_plane;
     return SRG_OK;
@@ -1266,6 +1291,14 @@ int srg_end_frame(srg_ctx* ctx)
     if (!ctx->in_frame) return SRG_ERR_STATE;
     ctx->in_frame = false;
     ctx->stats = srg_frame_stats{};
+    for (const srg_draw& d : ctx->draws) {
+        // a released framebuffer leaves its draws with fb == nullptr
+        const int vst = d.fb ? validate_draw(ctx, &d) : SRG_ERR_HANDLE;
+        if (vst) {
+            ctx->draws.clear();
+            return vst;
+        }
+    }
     int st = run_frame(ctx);
     ctx->draws.clear();
     return st;
@@ -1274,6 +1307,7 @@ int srg_end_frame(srg_ctx* ctx)
 int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear, void (*on_finish)(void*), void* user)
 {
     if (!ctx || !fb || !linear) return SRG_ERR_INVALID;
+    if (!fb_known(ctx, fb)) return SRG_ERR_HANDLE;
     PlaneState& p = fb->planes[fb->write_plane];
     if (!p.colour) return SRG_ERR_INVALID;
     int st = flush_pending_clear(ctx, fb, fb->write_plane);
@@ -1293,6 +1327,7 @@ int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear, void (*on_finish)(void*), v
 int srg_blit_async(srg_ctx* ctx, srg_fb* fb, void* linear, uint32_t flags, void (*on_finish)(void*), void* user)
 {
     if (!ctx || !fb || !linear || (flags & ~SRG_BLIT_BGRA)) return SRG_ERR_INVALID;
+    if (!fb_known(ctx, fb)) return SRG_ERR_HANDLE;
     if (ctx->in_frame) return SRG_ERR_STATE;
     const uint32_t wp = fb->write_plane;
     PlaneState& p = fb->planes[wp];
@@ -1326,6 +1361,7 @@ int srg_blit_async(srg_ctx* ctx, srg_fb* fb, void* linear, uint32_t flags, void 
 int srg_blit_wait(srg_ctx* ctx, srg_fb* fb)
 {
     if (!ctx || !fb) return SRG_ERR_INVALID;
+    if (!fb_known(ctx, fb)) return SRG_ERR_HANDLE;
     int st = finish_blit(fb->planes[0]);
     const int st1 = finish_blit(fb->planes[1]);
     return st ? st : st1;

@@ -126,3 +126,80 @@ def test_clear_list_clears_every_view():
         assert np.all(col == 0xABABABAB) and np.all(dep == 0.5)
         fb.destroy()
     ctx.Shutdown()
+
+
+SRG_ERR_HANDLE = -9
+
+
+def _draw_scene(ctx, sc, fb, texdata):
+    import softrast_amd as sr
+    for d in sc.draws:
+        call = sr.DrawCall().SetFrameBuffer(fb).SetMVP(sc.mvp)
+        call.SetIndexBuffer(d.indices, d.indices.dtype.itemsize, len(d.indices))
+        call.SetPositionBuffer(d.vertices, helpers.ATTR_STRIDE, len(d.vertices))
+        call.SetAttributeBuffer(d.vertices, helpers.ATTR_STRIDE, len(d.vertices), d.uv_offset)
+        call.SetPixelShader(sr.UnlitDiffuseShader, texdata[d.texture] if d.texture >= 0 else None)
+        ctx.DrawIndexed(call)
+
+
+def test_released_handles_fail_the_frame():
+    """A vertex buffer released, a texture released or a framebuffer destroyed
+    between DrawIndexed and EndFrame makes EndFrame fail with SRG_ERR_HANDLE
+    before any kernel reads the released memory; a clear list holding a
+    destroyed framebuffer fails the same way and clears none of its other
+    framebuffers. The context then renders a valid frame bit-exactly."""
+    import softrast_amd as sr
+    sc = helpers.random_scene(93, 1500, 256, 192)
+    tex = helpers.build_textures(sc)
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS)
+    fb = sr.FrameBuffer(ctx, sc.width, sc.height)
+
+    _say("[edge] case=released_buffer")
+    ctx.BeginFrame()
+    _draw_scene(ctx, sc, fb, tex)
+    ctx.invalidate(sc.draws[0].vertices)
+    with pytest.raises(sr.SrgError) as e:
+        ctx.EndFrame()
+    assert e.value.status == SRG_ERR_HANDLE
+
+    _say("[edge] case=released_texture")
+    dtex = [sr.DeviceTexture(ctx, t, t.shape[1], t.shape[0]) for t in sc.textures]
+    ctx.BeginFrame()
+    _draw_scene(ctx, sc, fb, dtex)
+    dtex[0].release()
+    with pytest.raises(sr.SrgError) as e:
+        ctx.EndFrame()
+    assert e.value.status == SRG_ERR_HANDLE
+
+    _say("[edge] case=destroyed_framebuffer")
+    fb2 = sr.FrameBuffer(ctx, sc.width, sc.height)
+    ctx.BeginFrame()
+    _draw_scene(ctx, sc, fb2, tex)
+    fb2.destroy()
+    with pytest.raises(sr.SrgError) as e:
+        ctx.EndFrame()
+    assert e.value.status == SRG_ERR_HANDLE
+
+    _say("[edge] case=clear_list_with_destroyed_framebuffer")
+    keep, gone = sr.FrameBuffer(ctx, 130, 61), sr.FrameBuffer(ctx, 64, 64)
+    cl = ctx.PrepareClearList([keep, gone])
+    col, dep = keep.read_tiles()
+    keep.write_tiles(np.full_like(col, 0x01020304), np.full_like(dep, 0.5))
+    gone.destroy()
+    ctx.BeginFrame()
+    with pytest.raises(sr.SrgError) as e:
+        ctx.ClearFrameBuffers(cl, 0x11223344)
+    assert e.value.status == SRG_ERR_HANDLE
+    ctx.EndFrame()
+    col, dep = keep.read_tiles()
+    assert np.all(col == 0x01020304) and np.all(dep == 0.5)
+
+    _say("[edge] case=valid_frame_after_errors")
+    ref, _ = helpers.oracle_render(sc, tex, nthreads=8)
+    helpers.gpu_render(sc, tex, ctx=ctx, fb=fb)
+    col, dep, pid = fb.read_tiles(primid=True)
+    d = helpers.compare(ref, col.reshape(-1), dep.reshape(-1), pid.reshape(-1))
+    assert not any(d.values()), d
+    for f in (fb, keep):
+        f.destroy()
+    ctx.Shutdown()

@@ -347,6 +347,38 @@ def test_async_blit_overlaps_next_frame_and_matches_sync_blit():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("w,h", [(130, 61), (65, 65), (300, 170), (1, 1)])
+def test_blit_rows_equal_oracle_blit(w, h):
+    """Blit, BlitAsync and BlitAsync with the BGRA present swizzle give the
+    oracle's blit (BlitJobFn: per tile, min(64, W - 64*tx) pixels of each of
+    min(64, H - 64*ty) rows, Renderer.cpp:319-347) of the oracle's frame, on
+    framebuffers with partial edge tiles."""
+    import softrast_amd as sr
+    from oracle.oracle import Oracle
+    sc = helpers.random_scene(60 + w, 3000, w, h, clip_heavy=True)
+    tex = helpers.build_textures(sc)
+    _say(f"[parity] case=blit_vs_oracle size={w}x{h}")
+    ref, _ = helpers.oracle_render(sc, tex, nthreads=8)
+    o = Oracle(nthreads=1)
+    want = o.blit(ref)
+    o.close()
+    assert want.shape == (h, w)
+    ctx = sr.RenderContext()
+    fb = sr.FrameBuffer(ctx, w, h)
+    helpers.gpu_render(sc, tex, ctx=ctx, fb=fb, flags=0)
+    got = ctx.Blit(fb).copy()
+    assert np.array_equal(got, want), int(np.count_nonzero(got != want))
+    swapped = want.view(np.uint8).reshape(h, w, 4)[:, :, [2, 1, 0, 3]].copy().view(np.uint32).reshape(h, w)
+    for bgra, exp in ((False, want), (True, swapped)):
+        helpers.gpu_render(sc, tex, ctx=ctx, fb=fb, flags=0)
+        rows = ctx.BlitAsync(fb, bgra=bgra)
+        ctx.BlitWait(fb)
+        assert np.array_equal(rows, exp), (bgra, int(np.count_nonzero(rows != exp)))
+    fb.destroy()
+    ctx.Shutdown()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("tile_order", [0, 1, 2])
 def test_clustered_scene_every_order(tile_order):
     """The clustered layout of the learned-scheduler workload (90% of the
