@@ -171,60 +171,71 @@ private:
         return v * sign;
     }
 
-    bool face(const char* p)
+    // One corner of an "f" line as written: position, uv and normal indices
+    // (0 where the corner names none).
+    struct Corner {
+        int32_t p = 0, t = 0, n = 0;
+    };
+
+    // Reads "p", "p/t", "p//n" or "p/t/n" at s. A position index of 0 (or no
+    // digits) is no corner: the face ends before it.
+    static bool read_corner(const char*& s, Corner& c)
     {
-        int32_t ip[6] = {}, it[6] = {}, in[6] = {};
-        int nv = 0;
-        while (*p && nv < 4) {
-            ip[nv] = read_int(p);
-            if (ip[nv] == 0) break;
-            const int c = nv++;
-            if (*p != '/') continue;
-            ++p;
-            if (*p == '/') {
-                ++p;
-                in[c] = read_int(p);
-                continue;
-            }
-            it[c] = read_int(p);
-            if (*p != '/') continue;
-            ++p;
-            in[c] = read_int(p);
+        c = Corner{};
+        c.p = read_int(s);
+        if (c.p == 0) return false;
+        if (*s != '/') return true;
+        ++s;
+        if (*s != '/') {  // "p/t" or "p/t/n"
+            c.t = read_int(s);
+            if (*s != '/') return true;
         }
-        if (nv == 4) {  // corners 0 1 2 3 -> triangles (0 1 2) (0 2 3)
-            ip[5] = ip[3]; it[5] = it[3]; in[5] = in[3];
-            ip[3] = ip[0]; it[3] = it[0]; in[3] = in[0];
-            ip[4] = ip[2]; it[4] = it[2]; in[4] = in[2];
-            nv = 6;
-        }
+        ++s;  // "p//n" or the second '/' of "p/t/n"
+        c.n = read_int(s);
+        return true;
+    }
+
+    // Appends the mesh vertex of corner c to the index list: the existing
+    // vertex of the same (position, uv, normal) triple, else a new one. A new
+    // vertex whose index lies past its list fails the load; an empty uv or
+    // normal list gives zeros.
+    bool emit_corner(const Corner& c)
+    {
         const uint32_t np = (uint32_t)(pos_.size() / 3), nt = (uint32_t)(uv_.size() / 2), nn = (uint32_t)(nrm_.size() / 3);
-        for (int c = 0; c < nv; ++c) {
-            const Triple k{ fix_index(ip[c], np), fix_index(it[c], nt), fix_index(in[c], nn) };
-            const auto found = map_.find(k);
-            if (found != map_.end()) {
-                idx_.push_back(found->second);
-                continue;
-            }
-            const uint32_t id = (uint32_t)verts_.size();
-            idx_.push_back(id);
-            map_.emplace(k, id);
-            if (k.p >= np) return false;
-            Vertex v;
-            std::memcpy(v.pos, &pos_[3 * (size_t)k.p], 12);
-            if (nn) {
-                if (k.n >= nn) return false;
-                std::memcpy(v.norm, &nrm_[3 * (size_t)k.n], 12);
-            } else {
-                std::memset(v.norm, 0, 12);
-            }
-            if (nt) {
-                if (k.t >= nt) return false;
-                std::memcpy(v.uv, &uv_[2 * (size_t)k.t], 8);
-            } else {
-                std::memset(v.uv, 0, 8);
-            }
-            verts_.push_back(v);
+        const Triple k{ fix_index(c.p, np), fix_index(c.t, nt), fix_index(c.n, nn) };
+        const auto found = map_.find(k);
+        if (found != map_.end()) {
+            idx_.push_back(found->second);
+            return true;
         }
+        if (k.p >= np || (nt && k.t >= nt) || (nn && k.n >= nn)) return false;
+        Vertex v{};
+        std::memcpy(v.pos, &pos_[3 * (size_t)k.p], 12);
+        if (nn) std::memcpy(v.norm, &nrm_[3 * (size_t)k.n], 12);
+        if (nt) std::memcpy(v.uv, &uv_[2 * (size_t)k.t], 8);
+        const uint32_t id = (uint32_t)verts_.size();
+        map_.emplace(k, id);
+        idx_.push_back(id);
+        verts_.push_back(v);
+        return true;
+    }
+
+    // "f": at most four corners are read. Three are one triangle; four are the
+    // triangles (0 1 2) and (0 2 3); one or two corners are appended as they
+    // are, so their indices stay in the mesh's index list.
+    bool face(const char* s)
+    {
+        Corner corner[4];
+        int n = 0;
+        while (*s && n < 4 && read_corner(s, corner[n])) ++n;
+        static const int kQuadFan[6] = { 0, 1, 2, 0, 2, 3 };
+        if (n == 4) {
+            for (int i : kQuadFan)
+                if (!emit_corner(corner[i])) return false;
+            return true;
+        }
+        for (int i = 0; i < n; ++i)
+            if (!emit_corner(corner[i])) return false;
         return true;
     }
 

@@ -32,6 +32,14 @@ def test_zero_index_ends_the_face():
     assert m.indices.tolist() == [0, 1, 2]
 
 
+def test_faces_of_one_or_two_corners_stay_in_the_index_list():
+    # no triangle is formed and nothing is dropped: the corners' vertices are
+    # appended as read, so later triangles start mid-triple
+    m = sr.obj_parse(QUAD + "f 1 2\nf 3 4 1\nf 2\n").meshes[0]
+    assert m.indices.tolist() == [0, 1, 2, 3, 0, 1]
+    assert _pos(m).tolist() == [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]
+
+
 def test_index_forms_and_missing_attribute_lists():
     text = QUAD + "vt 0.25 0.75\nvt 0.5 0.5\nvn 0 0 1\nvn 0 1 0\n"
     m = sr.obj_parse(text + "f 1/2/1 2//2 3/1\n").meshes[0]
