@@ -211,6 +211,29 @@ int srg_draw_indexed_n(srg_ctx* ctx, const srg_draw* draws, uint32_t n);
  * still registered, sizes still in range): a handle released since its
  * srg_draw_indexed fails the frame with SRG_ERR_HANDLE, drawing nothing. */
 int srg_end_frame(srg_ctx* ctx);
+/* srg_end_frame without allocating: no hipMalloc, hipHostMalloc, hipFree or
+ * graph instantiation, for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid allocation). No counterpart
+ * in the reference, whose EndFrame (Renderer.cpp:209-317) bump-allocates from
+ * per-thread arenas. The frame's working memory (tables, bin arrays, setup
+ * records, entry lists) is carved from a static device array of
+ * SRG_NOALLOC_SCRATCH_BYTES held by each GPU the library runs on until the
+ * process ends, and the tables are staged through SRG_NOALLOC_TABLE_BYTES of
+ * pinned host memory allocated by srg_create. Same image as srg_end_frame.
+ * Differences: the queue is in natural or entry-count order (a context
+ * configured for the learned order uses entry counts; its model is neither
+ * used nor trained), the autotuner's choice for the frame's signature is used
+ * only once it is locked (no sweep; otherwise the configured layout), and no
+ * graph is captured. A frame whose arenas do not fit returns SRG_ERR_OVERFLOW
+ * and draws nothing (its planes keep their pending clears); about 160 bytes per
+ * triangle plus 4 per triangle-bin entry, i.e. roughly 350k triangles per
+ * frame. Every call on one GPU shares that GPU's array: calls that overlap on
+ * one GPU (two threads, two contexts) race and must be serialised by the
+ * caller. Each call is synchronous and initialises every part of the array it
+ * reads. */
+#define SRG_NOALLOC_SCRATCH_BYTES (64ull << 20)
+#define SRG_NOALLOC_TABLE_BYTES (256u << 10)
+int srg_end_frame_noalloc(srg_ctx* ctx);
 /* Copies the write plane's colour to linear RGBA8 rows (width*height*4 bytes),
  * swaps the planes, then calls on_finish(user) (Renderer.cpp:350-372). */
 int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear_rgba8, void (*on_finish)(void*), void* user);

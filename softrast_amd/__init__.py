@@ -48,7 +48,7 @@ SRG_SYMBOLS = (
     "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
     "srg_get_tuning", "srg_draw_indexed_n", "srg_obj_load", "srg_obj_parse", "srg_obj_mesh_count",
     "srg_obj_get_mesh", "srg_obj_material_count", "srg_obj_material_name", "srg_obj_material_diffuse",
-    "srg_obj_free", "srg_set_sponza_constants", "srg_clear_n",
+    "srg_obj_free", "srg_set_sponza_constants", "srg_clear_n", "srg_end_frame_noalloc",
 )
 OBJ_FLIP_WINDING, OBJ_FLIP_UVS = 1, 4  # Obj::LoadFlags (Viewer/Obj.h:54-60)
 PLANE_WRITE = 0xFFFFFFFF
@@ -145,6 +145,7 @@ def load_library(path=None):
         "srg_clear": (i32, [vp, vp, u32, i32, i32]),
         "srg_draw_indexed": (i32, [vp, ctypes.POINTER(SrgDraw)]),
         "srg_end_frame": (i32, [vp]),
+        "srg_end_frame_noalloc": (i32, [vp]),
         "srg_blit": (i32, [vp, vp, vp, vp, vp]),
         "srg_get_frame_stats": (i32, [vp, ctypes.POINTER(SrgFrameStats)]),
         "srg_get_bin_stats": (i32, [vp, vp, vp, u32, ctypes.POINTER(u32)]),
@@ -617,8 +618,15 @@ class RenderContext:
         d.mvp[:] = [float(x) for x in mvp]
         return d
 
-    def EndFrame(self):
-        _check(self._lib.srg_end_frame(self._ctx), "EndFrame")
+    def EndFrame(self, noalloc=False):
+        """EndFrame; with noalloc, srg_end_frame_noalloc: the frame allocates
+        nothing and works in the library's fixed per-GPU scratch (entry-count
+        order, no graph, no autotuner sweep; SrgError with SRG_ERR_OVERFLOW
+        when the frame does not fit)."""
+        if noalloc:
+            _check(self._lib.srg_end_frame_noalloc(self._ctx), "EndFrame(noalloc)")
+        else:
+            _check(self._lib.srg_end_frame(self._ctx), "EndFrame")
 
     def Blit(self, fb, linear=None, onFinish=None, user=None):
         out = linear if linear is not None else np.empty((fb.m_height, fb.m_width), np.uint32)

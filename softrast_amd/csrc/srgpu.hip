@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <unordered_map>
@@ -79,6 +80,14 @@ __global__ void k_detile(const uint32_t* tiles, uint32_t* linear, uint32_t width
 }
 
 }  // namespace srg
+
+// Scratch memory of srg_end_frame_noalloc: the per-frame arenas of such a
+// frame (tables, bin arrays, records, entry lists) are carved from this array
+// instead of the context's own device allocations. Each GPU holds one copy,
+// from the library's first use on it until the process ends. Host code never
+// names it in a kernel argument or copy: it uses the device address that
+// hipGetSymbolAddress returned on that GPU (scratch_address below).
+static __device__ __align__(256) uint8_t scratch[SRG_NOALLOC_SCRATCH_BYTES];
 
 // ==========================================================================
 // host side
@@ -232,6 +241,7 @@ struct srg_ctx {
 
     hipEvent_t ev[4] = {};
     srg_frame_stats stats{};
+    void* scratch_dev = nullptr;  // this device's `scratch` (srg_end_frame_noalloc)
 };
 
 namespace {
@@ -251,6 +261,45 @@ int ensure_host_tables(srg_ctx* ctx, size_t n)
 }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Device address of `scratch` on each GPU, looked up once per device (with
+// that device current) by srg_create.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxDevices] = {};
+
+int scratch_address(int device, void** out)
+{
+    if (device < 0 || device >= kMaxDevices) return SRG_ERR_INVALID;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SRG_ERR_HIP;
+        g_scratch_dev[device] = p;
+    }
+    *out = g_scratch_dev[device];
+    return SRG_OK;
+}
+
+// Bump allocation of 256-byte-aligned pieces of a fixed block (the scratch).
+struct Carver {
+    uint8_t* base;
+    size_t cap;
+    size_t used = 0;
+    bool ok = true;
+    template <typename T>
+    T* take(size_t n)
+    {
+        const size_t o = align_up(used, 256), b = n * sizeof(T);
+        if (o > cap || b > cap - o) {
+            ok = false;
+            return nullptr;
+        }
+        used = o + b;
+        return (T*)(base + o);
+    }
+    size_t left() const { return align_up(used, 256) >= cap ? 0 : cap - align_up(used, 256); }
+};
 
 // Work on the library stream that touches a plane waits until an
 // asynchronous blit of it has read its tiles.
@@ -524,7 +573,28 @@ void tune_after_frame(srg_ctx* ctx, uint64_t sig, float ms)
     t.locked = true;
 }
 
-int run_frame(srg_ctx* ctx)
+// Device memory one frame works in: the context's arenas (grown as frames
+// need them), or pieces of the scratch array for srg_end_frame_noalloc.
+struct FrameArenas {
+    uint8_t* tables = nullptr;
+    uint4* rec_raster = nullptr;
+    float4* rec_shade = nullptr;
+    float4* rec_ext = nullptr;
+    uint32_t *rec_meta = nullptr, *clip_queue = nullptr, *clip_base = nullptr, *clip_order = nullptr;
+    uint32_t *bin_count = nullptr, *bin_start = nullptr, *part_count = nullptr, *part_cursor = nullptr;
+    uint32_t *part_off = nullptr, *queue = nullptr, *queue2 = nullptr, *entries = nullptr, *scan_blk = nullptr;
+    uint16_t* order_bkt = nullptr;
+    uint32_t* order_hist = nullptr;
+    uint64_t* bin_cost = nullptr;
+    uint32_t* counters = nullptr;
+};
+
+// One EndFrame. With noalloc, nothing is allocated: the arenas are carved
+// from the scratch array (entry lists take what the rest leaves), the queue
+// is in natural or entry-count order (the learned order's buffers are not
+// used), the autotuner's locked choice or the configured layout is used
+// without sweeping, and the launches run without a graph.
+int run_frame(srg_ctx* ctx, bool noalloc)
 {
     // ---- views: planes drawn to, plus planes with a pending clear ----
     std::vector<ViewKey> views;
@@ -551,10 +621,15 @@ int run_frame(srg_ctx* ctx)
         sig = mix64(sig, views.size());
         for (const ViewKey& v : views) sig = mix64(mix64(sig, v.fb->width), v.fb->height);
         for (size_t i = 0; i < ctx->draws.size(); ++i) sig = mix64(mix64(sig, ctx->draws[i].index_num), draw_view[i]);
-        L = tune_layout(ctx, sig);
-        ctx->tune_sig = sig;
+        if (noalloc) {
+            const auto it = ctx->tunes.find(sig);
+            if (it != ctx->tunes.end() && it->second.locked) L = it->second.chosen;
+        } else {
+            L = tune_layout(ctx, sig);
+            ctx->tune_sig = sig;
+        }
     }
-    const bool sweeping = autotune && ctx->tunes[sig].sweeping;
+    const bool sweeping = autotune && !noalloc && ctx->tunes[sig].sweeping;
     const uint32_t K = kBin / L.bin_size;
     std::vector<ViewDev> vdev(views.size());
     uint64_t total_bins = 0;
@@ -646,9 +721,17 @@ int run_frame(srg_ctx* ctx)
     const size_t o_view = off; off = align_up(off + sizeof(ViewDev) * vdev.size(), 256);
     const size_t o_tex = off;  off = align_up(off + sizeof(TexDev) * std::max<size_t>(tdev.size(), 1), 256);
     const size_t o_sponza = off; off = align_up(off + (wide ? sizeof(SponzaDev) : 0), 256);
-    int st = ensure_host_tables(ctx, off);
-    if (st) return st;
-    if ((st = ctx->tables.ensure(off))) return st;
+    int st = SRG_OK;
+    FrameArenas A;
+    Carver C{ (uint8_t*)ctx->scratch_dev, noalloc ? (size_t)SRG_NOALLOC_SCRATCH_BYTES : 0u };
+    if (noalloc) {
+        if (!ctx->scratch_dev) return SRG_ERR_HIP;
+        if (off > ctx->h_tables_cap) return SRG_ERR_OVERFLOW;  // pinned staging sized at srg_create
+        A.tables = C.take<uint8_t>(off);
+    } else {
+        if ((st = ensure_host_tables(ctx, off)) || (st = ctx->tables.ensure(off))) return st;
+        A.tables = ctx->tables.ptr;
+    }
     if (!ddev.empty()) std::memcpy(ctx->h_tables + o_draw, ddev.data(), sizeof(DrawDev) * ddev.size());
     std::memcpy(ctx->h_tables + o_view, vdev.data(), sizeof(ViewDev) * vdev.size());
     if (!tdev.empty()) std::memcpy(ctx->h_tables + o_tex, tdev.data(), sizeof(TexDev) * tdev.size());
@@ -656,19 +739,56 @@ int run_frame(srg_ctx* ctx)
 
     // ---- arenas ----
     const uint32_t TB = (uint32_t)std::max<uint64_t>(total_bins, 1);
-    if ((st = ctx->bin_count.ensure(TB)) || (st = ctx->bin_start.ensure(TB + 1)) ||
-        (st = ctx->part_count.ensure((size_t)kBinParts * TB)) || (st = ctx->part_cursor.ensure((size_t)kBinParts * TB)) ||
-        (st = ctx->part_off.ensure((size_t)kBinParts * TB)) ||
-        (st = ctx->queue.ensure(TB)) || (st = ctx->queue2.ensure(TB)) ||
-        (st = ctx->order_bkt.ensure(TB)) ||
-        (st = ctx->order_hist.ensure((size_t)((TB + kOrderBlock - 1) / kOrderBlock) * kOrderBuckets)) ||
-        (st = ctx->scan_blk.ensure(2 * ((TB + kScanThreads - 1) / kScanThreads))) || (st = ctx->bin_cost.ensure(TB)) || (st = ctx->counters.ensure(C_NCOUNTERS)) ||
-        (st = ctx->clip_queue.ensure(std::max<uint32_t>(NT, 1))) || (st = ctx->clip_base.ensure(std::max<uint32_t>(NT, 1))))
-        return st;
+    const uint32_t NQ = std::max<uint32_t>(NT, 1);
+    const size_t n_hist = (size_t)((TB + kOrderBlock - 1) / kOrderBlock) * kOrderBuckets;
+    const size_t n_scan = 2 * ((TB + kScanThreads - 1) / kScanThreads);
+    if (noalloc) {
+        A.counters = C.take<uint32_t>(C_NCOUNTERS);
+        A.bin_count = C.take<uint32_t>(TB);
+        A.bin_start = C.take<uint32_t>(TB + 1);
+        A.part_count = C.take<uint32_t>((size_t)kBinParts * TB);
+        A.part_cursor = C.take<uint32_t>((size_t)kBinParts * TB);
+        A.part_off = C.take<uint32_t>((size_t)kBinParts * TB);
+        A.queue = C.take<uint32_t>(TB);
+        A.queue2 = C.take<uint32_t>(TB);
+        A.order_bkt = C.take<uint16_t>(TB);
+        A.order_hist = C.take<uint32_t>(n_hist);
+        A.scan_blk = C.take<uint32_t>(n_scan);
+        A.bin_cost = C.take<uint64_t>(TB);
+        A.clip_queue = C.take<uint32_t>(NQ);
+        A.clip_base = C.take<uint32_t>(NQ);
+        if (!C.ok) return SRG_ERR_OVERFLOW;
+    } else {
+        if ((st = ctx->bin_count.ensure(TB)) || (st = ctx->bin_start.ensure(TB + 1)) ||
+            (st = ctx->part_count.ensure((size_t)kBinParts * TB)) ||
+            (st = ctx->part_cursor.ensure((size_t)kBinParts * TB)) ||
+            (st = ctx->part_off.ensure((size_t)kBinParts * TB)) || (st = ctx->queue.ensure(TB)) ||
+            (st = ctx->queue2.ensure(TB)) || (st = ctx->order_bkt.ensure(TB)) || (st = ctx->order_hist.ensure(n_hist)) ||
+            (st = ctx->scan_blk.ensure(n_scan)) || (st = ctx->bin_cost.ensure(TB)) ||
+            (st = ctx->counters.ensure(C_NCOUNTERS)) || (st = ctx->clip_queue.ensure(NQ)) ||
+            (st = ctx->clip_base.ensure(NQ)))
+            return st;
+        A.counters = ctx->counters.ptr;
+        A.bin_count = ctx->bin_count.ptr;
+        A.bin_start = ctx->bin_start.ptr;
+        A.part_count = ctx->part_count.ptr;
+        A.part_cursor = ctx->part_cursor.ptr;
+        A.part_off = ctx->part_off.ptr;
+        A.queue = ctx->queue.ptr;
+        A.queue2 = ctx->queue2.ptr;
+        A.order_bkt = ctx->order_bkt.ptr;
+        A.order_hist = ctx->order_hist.ptr;
+        A.scan_blk = ctx->scan_blk.ptr;
+        A.bin_cost = ctx->bin_cost.ptr;
+        A.clip_queue = ctx->clip_queue.ptr;
+        A.clip_base = ctx->clip_base.ptr;
+    }
     uint64_t want_clip = ctx->cfg.max_prims ? ctx->cfg.max_prims : (uint64_t)NT / 8 + 4096;
-    want_clip = std::max<uint64_t>(want_clip, ctx->clip_cap);
     uint64_t want_entries = ctx->cfg.max_entries ? ctx->cfg.max_entries : 2ull * NT + 4096;
-    want_entries = std::max<uint64_t>(want_entries, ctx->entry_cap);
+    if (!noalloc) {
+        want_clip = std::max<uint64_t>(want_clip, ctx->clip_cap);
+        want_entries = std::max<uint64_t>(want_entries, ctx->entry_cap);
+    }
 
     // planes an asynchronous blit is still reading are written only after it
     for (const ViewKey& v : views) stream_wait_blit_read(ctx, v.fb->planes[v.plane]);
@@ -676,43 +796,62 @@ int run_frame(srg_ctx* ctx)
     for (uint32_t attempt = 0;; ++attempt) {
         if (want_clip > 0xFFFFFFF0ull - NT || want_entries > 0xFFFFFFF0ull) return SRG_ERR_OVERFLOW;
         const uint64_t nrec = (uint64_t)NT + want_clip;
-        if ((st = ctx->rec_raster.ensure(nrec * 4)) || (st = ctx->rec_shade.ensure(nrec * 4)) ||
-            (st = ctx->rec_meta.ensure(nrec)) || (st = ctx->clip_order.ensure(want_clip)) ||
-            (st = ctx->entries.ensure(want_entries)) || (wide && (st = ctx->rec_ext.ensure(nrec * 3))))
-            return st;
-        ctx->clip_cap = (uint32_t)want_clip;
-        ctx->entry_cap = (uint32_t)want_entries;
+        if (noalloc) {
+            // records first; the entry lists take what the scratch has left
+            Carver R = C;
+            A.rec_raster = R.take<uint4>(nrec * 4);
+            A.rec_shade = R.take<float4>(nrec * 4);
+            A.rec_meta = R.take<uint32_t>(nrec);
+            A.clip_order = R.take<uint32_t>(want_clip);
+            A.rec_ext = wide ? R.take<float4>(nrec * 3) : nullptr;
+            want_entries = std::min<uint64_t>(R.left() / sizeof(uint32_t), 0xFFFFFFF0ull);
+            if (!R.ok || want_entries < 4096) return SRG_ERR_OVERFLOW;
+            A.entries = R.take<uint32_t>(want_entries);
+        } else {
+            if ((st = ctx->rec_raster.ensure(nrec * 4)) || (st = ctx->rec_shade.ensure(nrec * 4)) ||
+                (st = ctx->rec_meta.ensure(nrec)) || (st = ctx->clip_order.ensure(want_clip)) ||
+                (st = ctx->entries.ensure(want_entries)) || (wide && (st = ctx->rec_ext.ensure(nrec * 3))))
+                return st;
+            ctx->clip_cap = (uint32_t)want_clip;
+            ctx->entry_cap = (uint32_t)want_entries;
+            A.rec_raster = ctx->rec_raster.ptr;
+            A.rec_shade = ctx->rec_shade.ptr;
+            A.rec_meta = ctx->rec_meta.ptr;
+            A.clip_order = ctx->clip_order.ptr;
+            A.entries = ctx->entries.ptr;
+            A.rec_ext = wide ? ctx->rec_ext.ptr : nullptr;
+        }
 
         FrameDev F;
         std::memset(&F, 0, sizeof(F));
-        F.draws = (const DrawDev*)(ctx->tables.ptr + o_draw);
-        F.views = (const ViewDev*)(ctx->tables.ptr + o_view);
-        F.texs = (const TexDev*)(ctx->tables.ptr + o_tex);
+        F.draws = (const DrawDev*)(A.tables + o_draw);
+        F.views = (const ViewDev*)(A.tables + o_view);
+        F.texs = (const TexDev*)(A.tables + o_tex);
         F.ndraws = (uint32_t)ddev.size();
         F.nviews = (uint32_t)vdev.size();
         F.total_bins = (uint32_t)total_bins;
         F.ntris = NT;
         F.bin_k = K;
         F.bin_size = kBin / K;
-        F.clip_cap = ctx->clip_cap;
-        F.entry_cap = ctx->entry_cap;
-        F.rec_raster = ctx->rec_raster.ptr;
-        F.rec_shade = ctx->rec_shade.ptr;
-        F.rec_ext = wide ? ctx->rec_ext.ptr : nullptr;
-        F.sponza = wide ? (const SponzaDev*)(ctx->tables.ptr + o_sponza) : nullptr;
-        F.rec_meta = ctx->rec_meta.ptr;
-        F.clip_queue = ctx->clip_queue.ptr;
-        F.clip_base = ctx->clip_base.ptr;
-        F.clip_order = ctx->clip_order.ptr;
-        F.bin_count = ctx->bin_count.ptr;
-        F.bin_start = ctx->bin_start.ptr;
-        F.part_count = ctx->part_count.ptr;
-        F.part_cursor = ctx->part_cursor.ptr;
-        F.part_off = ctx->part_off.ptr;
-        F.entries = ctx->entries.ptr;
-        F.queue = ctx->queue.ptr;
-        F.bin_cost = ctx->bin_cost.ptr;
-        F.counters = ctx->counters.ptr;
+        F.clip_cap = (uint32_t)want_clip;
+        F.entry_cap = (uint32_t)want_entries;
+        F.rec_raster = A.rec_raster;
+        F.rec_shade = A.rec_shade;
+        F.rec_ext = A.rec_ext;
+        F.sponza = wide ? (const SponzaDev*)(A.tables + o_sponza) : nullptr;
+        F.rec_meta = A.rec_meta;
+        F.clip_queue = A.clip_queue;
+        F.clip_base = A.clip_base;
+        F.clip_order = A.clip_order;
+        F.bin_count = A.bin_count;
+        F.bin_start = A.bin_start;
+        F.part_count = A.part_count;
+        F.part_cursor = A.part_cursor;
+        F.part_off = A.part_off;
+        F.entries = A.entries;
+        F.queue = A.queue;
+        F.bin_cost = A.bin_cost;
+        F.counters = A.counters;
         F.flags = ((ctx->cfg.flags & SRG_CFG_STATS) ? FF_STATS : 0u) |
                   ((ctx->cfg.flags & SRG_CFG_KEEP_PRIMID) ? FF_PRIMID : 0u) |
                   ((ctx->cfg.flags & SRG_CFG_TILE_COSTS) ? FF_TILE_COSTS : 0u);
@@ -724,7 +863,7 @@ int run_frame(srg_ctx* ctx)
         const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
         // while the autotuner sweeps, the learned order falls back to entry
         // counts and records nothing (its samples belong to one bin size)
-        const bool learned = ctx->cfg.tile_order == SRG_ORDER_LEARNED && !sweeping;
+        const bool learned = ctx->cfg.tile_order == SRG_ORDER_LEARNED && !sweeping && !noalloc;
         if (learned) {
             if ((st = ensure_sched(ctx)) || (st = ctx->sched_feat.ensure(TB)) || (st = ctx->sched_pred.ensure(TB)))
                 return st;
@@ -749,9 +888,9 @@ int run_frame(srg_ctx* ctx)
         }
         // The whole frame on the stream: tables, six kernels, counters back.
         auto enqueue = [&](bool split_events) -> int {
-            HIP_TRY(hipMemcpyAsync(ctx->tables.ptr, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(k_frame_init, dim3(init_blocks), dim3(256), 0, ctx->stream, ctx->counters.ptr,
-                               ctx->part_count.ptr, ctx->part_cursor.ptr, ctx->bin_cost.ptr, TB,
+            HIP_TRY(hipMemcpyAsync(A.tables, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(k_frame_init, dim3(init_blocks), dim3(256), 0, ctx->stream, A.counters,
+                               A.part_count, A.part_cursor, A.bin_cost, TB,
                                learned ? ctx->sched_gate : nullptr, sched_every);
             if (ctx->cfg.flags & SRG_CFG_KEEP_PRIMID)
                 for (auto& v : vdev)
@@ -766,11 +905,11 @@ int run_frame(srg_ctx* ctx)
                 if (NT) hipLaunchKernelGGL(k_clip<kMaxSlots>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
             }
             FrameDev Fs = F;  // with ordering, the natural queue goes to the spare buffer first
-            if (ordered) Fs.queue = ctx->queue2.ptr;
-            hipLaunchKernelGGL(k_scan_blocks, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs, ctx->scan_blk.ptr);
-            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs, ctx->scan_blk.ptr, scan_blocks);
+            if (ordered) Fs.queue = A.queue2;
+            hipLaunchKernelGGL(k_scan_blocks, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs, A.scan_blk);
+            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs, A.scan_blk, scan_blocks);
             hipLaunchKernelGGL(k_scan_apply, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs,
-                               (const uint32_t*)ctx->scan_blk.ptr);
+                               (const uint32_t*)A.scan_blk);
             hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
             if (learned) {
                 hipLaunchKernelGGL(k_bin_predict, dim3(std::min<uint32_t>((TB + 3) / 4, 16384u)),
@@ -782,11 +921,11 @@ int run_frame(srg_ctx* ctx)
                 const float* pred = (const float*)(learned && use_model ? ctx->sched_pred.ptr : nullptr);
                 const uint32_t oblk = (TB + kOrderBlock - 1) / kOrderBlock;
                 hipLaunchKernelGGL(k_order_hist, dim3(oblk), dim3(kOrderBlock), 0, ctx->stream, F,
-                                   (const uint32_t*)ctx->queue2.ptr, pred, ctx->order_bkt.ptr, ctx->order_hist.ptr);
-                hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(kOrderBuckets), 0, ctx->stream, F, ctx->order_hist.ptr);
+                                   (const uint32_t*)A.queue2, pred, A.order_bkt, A.order_hist);
+                hipLaunchKernelGGL(k_order_offsets, dim3(1), dim3(kOrderBuckets), 0, ctx->stream, F, A.order_hist);
                 hipLaunchKernelGGL(k_order_scatter, dim3(oblk), dim3(kOrderBlock), 0, ctx->stream, F,
-                                   (const uint32_t*)ctx->queue2.ptr, (const uint16_t*)ctx->order_bkt.ptr,
-                                   (const uint32_t*)ctx->order_hist.ptr);
+                                   (const uint32_t*)A.queue2, (const uint16_t*)A.order_bkt,
+                                   (const uint32_t*)A.order_hist);
             }
             if (split_events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
             if (const int lst = launch_backend(ctx, L, F)) return lst;
@@ -796,19 +935,19 @@ int run_frame(srg_ctx* ctx)
                                    (const float*)(ctx->sched_trained ? ctx->sched_pred.ptr : nullptr), ctx->sched_ring,
                                    kSchedRing, ctx->sched_head);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipMemcpyAsync(ctx->h_counters, ctx->counters.ptr, sizeof(uint32_t) * C_NCOUNTERS,
+            HIP_TRY(hipMemcpyAsync(ctx->h_counters, A.counters, sizeof(uint32_t) * C_NCOUNTERS,
                                    hipMemcpyDeviceToHost, ctx->stream));
             return SRG_OK;
         };
 
         const bool timing = (ctx->cfg.flags & SRG_CFG_TIMING) != 0 || sweeping;
         bool split = false;
-        if (ctx->cfg.flags & SRG_CFG_GRAPH) {
+        if ((ctx->cfg.flags & SRG_CFG_GRAPH) && !noalloc) {
             // Replay the captured frame when nothing it depends on changed:
             // the table bytes, every pointer and capacity in F, and the
             // launch shapes.
             std::vector<uint8_t> sig(ctx->h_tables, ctx->h_tables + off);
-            const uint64_t extra[] = { (uint64_t)(uintptr_t)ctx->h_tables, (uint64_t)(uintptr_t)ctx->tables.ptr,
+            const uint64_t extra[] = { (uint64_t)(uintptr_t)ctx->h_tables, (uint64_t)(uintptr_t)A.tables,
                                        (uint64_t)(uintptr_t)ctx->h_counters, nblocks, emit_blocks, clip_blocks,
                                        init_blocks, TB, K, L.parts, ctx->cfg.flags, ctx->cfg.wg_per_cu,
                                        (uint64_t)ctx->sched_trained | ((uint64_t)ctx->sched_fallback << 1) |
@@ -849,6 +988,8 @@ int run_frame(srg_ctx* ctx)
         if (flags & FLAG_INDEX_RANGE) return SRG_ERR_INDEX_RANGE;
         if (flags & (FLAG_CLIP_OVERFLOW | FLAG_ENTRY_OVERFLOW)) {
             if (attempt >= 4) return SRG_ERR_OVERFLOW;
+            // the scratch cannot grow: only more fan records at the entries' expense
+            if (noalloc && (flags & FLAG_ENTRY_OVERFLOW)) return SRG_ERR_OVERFLOW;
             if (flags & FLAG_CLIP_OVERFLOW)
                 want_clip = std::max<uint64_t>(want_clip * 2, (uint64_t)ctx->h_counters[C_CLIPREC] + 64);
             if (flags & FLAG_ENTRY_OVERFLOW)
@@ -969,9 +1110,15 @@ int srg_create(const srg_config* cfg, srg_ctx** out)
     }
     for (auto& e : ctx->ev)
         if (hipEventCreate(&e) != hipSuccess) { srg_destroy(ctx); return SRG_ERR_HIP; }
-    if (hipHostMalloc((void**)&ctx->h_counters, sizeof(uint32_t) * C_NCOUNTERS, 0) != hipSuccess) {
+    if (hipHostMalloc((void**)&ctx->h_counters, sizeof(uint32_t) * C_NCOUNTERS, 0) != hipSuccess ||
+        ensure_host_tables(ctx, SRG_NOALLOC_TABLE_BYTES) != SRG_OK) {
         srg_destroy(ctx);
         return SRG_ERR_NOMEM;
+    }
+    // srg_end_frame_noalloc's scratch on this device (looked up once per device)
+    if (const int sst = scratch_address(ctx->device, &ctx->scratch_dev)) {
+        srg_destroy(ctx);
+        return sst;
     }
     *out = ctx;
     return SRG_OK;
@@ -1285,7 +1432,7 @@ int srg_draw_indexed_n(srg_ctx* ctx, const srg_draw* draws, uint32_t n)
     return SRG_OK;
 }
 
-int srg_end_frame(srg_ctx* ctx)
+static int end_frame(srg_ctx* ctx, bool noalloc)
 {
     if (!ctx) return SRG_ERR_INVALID;
     if (!ctx->in_frame) return SRG_ERR_STATE;
@@ -1299,10 +1446,14 @@ int srg_end_frame(srg_ctx* ctx)
             return vst;
         }
     }
-    int st = run_frame(ctx);
+    int st = run_frame(ctx, noalloc);
     ctx->draws.clear();
     return st;
 }
+
+int srg_end_frame(srg_ctx* ctx) { return end_frame(ctx, false); }
+
+int srg_end_frame_noalloc(srg_ctx* ctx) { return end_frame(ctx, true); }
 
 int srg_blit(srg_ctx* ctx, srg_fb* fb, void* linear, void (*on_finish)(void*), void* user)
 {

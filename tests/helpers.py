@@ -95,10 +95,12 @@ def oracle_render(scene, texdata, nthreads=1, clear=0, plane_init=None, fast=Fal
     return p, stats
 
 
-def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, fb=None, sponza=None, **ctx_kw):
+def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, fb=None, sponza=None, noalloc=False,
+               **ctx_kw):
     """Render one frame on the GPU through the C ABI; returns (colour, depth, primid, stats).
     With fb given, renders into it and returns the stats only. sponza:
-    SponzaConstants set before the frame."""
+    SponzaConstants set before the frame. noalloc: end the frame with
+    srg_end_frame_noalloc."""
     import softrast_amd as sr
     if flags is None:
         flags = sr.CFG_KEEP_PRIMID | sr.CFG_STATS
@@ -125,7 +127,7 @@ def gpu_render(scene, texdata, clear=0, plane_init=None, ctx=None, flags=None, f
                   3: sr.SponzaShader}[d.shader]
         call.SetPixelShader(shader, texdata[d.texture] if d.texture >= 0 else None)
         ctx.DrawIndexed(call)
-    ctx.EndFrame()
+    ctx.EndFrame(noalloc=noalloc)
     stats = ctx.frame_stats()
     if not own_fb:
         return stats
