@@ -17,11 +17,17 @@ starts. Two paths are timed:
 
 A step is one EndFrame; frames/s counts views completed per second
 (value = the batched path). Also reported:
-  roofline      HBM roofline of the dominant kernel (k_backend) in the plain
-                path, timed with HIP events on the library's stream
+  roofline      HBM roofline of the dominant kernel of the batched launch
+                (k_backend on the chosen layout): the algorithmic bytes of all
+                its views over the kernel's time, from HIP events around it on
+                the library's stream in launches of the same layout and order
+                without a graph; HBM traffic and VALU issue counts from the
+                rocprofv3 summaries in profiles/ (the plain path's figures
+                are under "plain")
   cpu_baseline  the CPU oracle (a C port of the reference path, -O3
                 AVX2/FMA build, reference task model) on the host cores
-  parity        differing-pixel counts of the benchmarked frame vs the oracle
+  parity        differing-pixel counts vs the oracle: the plain frame, and
+                the first, middle and last view of the timed batched launch
 """
 import argparse
 import json
@@ -100,20 +106,22 @@ def make_calls(sr, fb, scene, texs, mvp, shader=None):
     return calls
 
 
-def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup, shader=None, sponza=None):
+def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup, shader=None, sponza=None,
+              wg_per_tile=0):
     """Returns (frames/s, ms per EndFrame, per-frame stats, ctx, framebuffers).
     Untimed frames first: `warmup`, and with the autotuner or the learned
     order, until the layout is chosen and the model trained on it. shader:
     the pixel shader of every draw (UnlitDiffuse by default); sponza: the
     SponzaShader constants."""
-    ctx = sr.RenderContext(flags=flags, tile_order=tile_order, bin_size=bin_size, autotune_frames=3)
+    ctx = sr.RenderContext(flags=flags, tile_order=tile_order, bin_size=bin_size, wg_per_tile=wg_per_tile,
+                           autotune_frames=3)
     if sponza is not None:
         ctx.set_sponza_constants(sponza)
     fbs, calls = [], []
     for v in range(views):
         fb = sr.FrameBuffer(ctx, scene.width, scene.height)
         # cameras turned by up to +-2 degrees: every view sees nearly the plain view's scene
-        mvp = scene.mvp if views == 1 else view_mvp(scene, (v - (views - 1) / 2.0) * 4.0 / (views - 1))
+        mvp = batched_view_mvp(scene, views, v)
         fbs.append(fb)
         calls.append(make_calls(sr, fb, scene, texs, mvp, shader))
 
@@ -143,9 +151,16 @@ def time_path(sr, scene, texs, views, flags, tile_order, bin_size, steps, warmup
     return views * steps / elapsed, 1000.0 * elapsed / steps, stats, ctx, fbs
 
 
-def batched_frags(sr, scene, texs, views, bin_size):
-    """Rasterised fragments per view of the batched views, from one untimed
-    statistics frame (the views' cameras differ from the plain view's)."""
+def batched_view_mvp(scene, views, v):
+    """Camera of view v of a batch: turned by up to +-2 degrees, so every view
+    sees nearly the plain view's scene."""
+    return scene.mvp if views == 1 else view_mvp(scene, (v - (views - 1) / 2.0) * 4.0 / (views - 1))
+
+
+def batched_stats(sr, scene, texs, views, bin_size):
+    """Rasterised fragments per view and covered pixels of each view of the
+    batched views, from one untimed statistics frame (the views' cameras
+    differ from the plain view's)."""
     ctx = sr.RenderContext(flags=sr.CFG_STATS, bin_size=bin_size)
     fbs = []
     ctx.BeginFrame()
@@ -153,14 +168,16 @@ def batched_frags(sr, scene, texs, views, bin_size):
         fb = sr.FrameBuffer(ctx, scene.width, scene.height)
         fbs.append(fb)
         ctx.ClearFrameBuffer(fb, 0)
-        for c in make_calls(sr, fb, scene, texs, view_mvp(scene, (v - (views - 1) / 2.0) * 4.0 / (views - 1))):
+        for c in make_calls(sr, fb, scene, texs, batched_view_mvp(scene, views, v)):
             ctx.DrawIndexed(c)
     ctx.EndFrame()
     frags = ctx.frame_stats()["frags_raster"]
+    covered = []
     for fb in fbs:
+        covered.append(int(np.count_nonzero(fb.read_tiles()[1].reshape(-1).view(np.uint32) != 0)))
         fb.destroy()
     ctx.Shutdown()
-    return frags / views
+    return frags / views, covered
 
 
 def load_profile_value(name, key):
@@ -511,6 +528,19 @@ def sponza_workload(sr, args):
             dd = int(np.count_nonzero(plane.depth.view(np.uint32) != depth.reshape(-1).view(np.uint32)))
             dc = int(np.count_nonzero(plane.colour != colour.reshape(-1)))
             out["parity"] = {"depth_px_diff": dd, "rgba8_px_diff": dc, "vs": "oracle", "frame": "plain path"}
+            # sampled views of the batched launch `value` measures, against the
+            # oracle's render of each view's camera
+            bp = {"views": sorted(sampled), "depth_px_diff": 0, "rgba8_px_diff": 0, "vs": "oracle"}
+            for v, (bcol, bdep) in sorted(sampled.items()):
+                vdraws = [dict(d, mvp=batched_view_mvp(scene, args.views, v)) for d in odraws]
+                vplane = Plane(scene.width, scene.height)
+                orc.clear(vplane, 0)
+                orc.render(vdraws, [vplane])
+                bp["depth_px_diff"] += int(np.count_nonzero(vplane.depth.view(np.uint32) !=
+                                                            bdep.reshape(-1).view(np.uint32)))
+                bp["rgba8_px_diff"] += int(np.count_nonzero(vplane.colour != bcol.reshape(-1)))
+            if sampled:
+                out["batched"]["parity"] = bp
         if not args.no_cpu_baseline:
             times = []
             for _ in range(3 + args.cpu_frames):
@@ -621,11 +651,12 @@ def main():
     fe_bytes = 4 * 3 * scene.num_tris + 32 * sum(len(d.vertices) for d in scene.draws)
     be_bytes = 8 * scene.width * scene.height + 16 * covered
     achieved = be_bytes / (be_ms * 1e-3) / 1e9
-    traffic = load_traffic("k_backend_spheres_1080p_1m_traffic.md")
 
     # ---- batched path ----
     fps_batched = ms_batched = dev_batched = None
     batched_runs = {}
+    sampled = {}  # view -> (colour, depth) tiles of the headline batched path, for parity
+    be_ms_batched = None
     if not args.no_batched:
         bflags = sr.CFG_GRAPH | (0 if args.no_autotune else sr.CFG_AUTOTUNE)
         orders = [args.tile_order] + ([sr.ORDER_TRICOUNT] if args.tile_order != sr.ORDER_TRICOUNT else [])
@@ -650,9 +681,23 @@ def main():
             batched_runs[run["tile_order"]] = run
             if order == args.tile_order:
                 fps_batched, ms_batched, dev_batched = fps, ms, run["device_ms_per_launch"]
+                for v in sorted({0, (args.views - 1) // 2, args.views - 1}):
+                    sampled[v] = bfbs[v].read_tiles()
             for fb in bfbs:
                 fb.destroy()
             bctx.Shutdown()
+            if order == args.tile_order:
+                # the back-end kernel of this path alone: HIP events around it on
+                # the library's stream, in launches without a graph on the same
+                # layout and order
+                lay = run["layout"]
+                _, _, st_k, kctx, kfbs = time_path(sr, scene, texs, args.views, sr.CFG_TIMING, order, lay["bin_size"],
+                                                   10, args.warmup, wg_per_tile=lay["wg_per_tile"])
+                be_ms_batched = statistics.mean(s["ms_backend"] for s in st_k)
+                run["backend_ms_per_launch"] = round(be_ms_batched, 4)
+                for fb in kfbs:
+                    fb.destroy()
+                kctx.Shutdown()
             print(f"[bench] batched x{args.views} {run['tile_order']}: {fps:.1f} frames/s, layout {run['layout']}",
                   file=sys.stderr, flush=True)
 
@@ -680,25 +725,53 @@ def main():
         "plain": {"frames_per_s": round(fps_plain, 2), "ms_per_frame": round(ms_plain, 4),
                   "device_ms": {"frontend": round(fe_ms, 4), "backend": round(be_ms, 4), "total": round(dev_ms, 4)}},
         "mtris_per_s": round(scene.num_tris * value / 1e6, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_backend", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": be_bytes,
-                     "traffic": traffic},
         "frame_algorithmic_bytes": fe_bytes + be_bytes,
     }
-    # the back end is bound by instruction issue, not HBM: its VALU wave
-    # instructions per launch (SQ_INSTS_VALU, profiles/) at 4 cycles each on
-    # 1,024 SIMDs at 2.4 GHz, against the live back-end time
-    valu = load_profile_value("k_backend_spheres_1080p_1m_sq_counters.md", "valu_insts_per_launch")
-    if valu:
+
+    def valu_issue(kernel, profile, measured_ms):
+        # the back end is bound by instruction issue, not HBM: its VALU wave
+        # instructions per launch (SQ_INSTS_VALU, profiles/) at 4 cycles each
+        # on 1,024 SIMDs at 2.4 GHz, against the live kernel time
+        valu = load_profile_value(profile, "valu_insts_per_launch")
+        if not valu:
+            return None
         floor_us = valu * 4.0 / 1024 / 2.4e3
-        out["valu_issue"] = {"kernel": "k_backend", "valu_insts_per_launch": valu, "issue_floor_us": round(floor_us, 1),
-                             "measured_us": round(be_ms * 1e3, 1), "frac": round(floor_us / (be_ms * 1e3), 3)}
+        return {"kernel": kernel, "valu_insts_per_launch": valu, "issue_floor_us": round(floor_us, 1),
+                "measured_us": round(measured_ms * 1e3, 1), "frac": round(floor_us / (measured_ms * 1e3), 3),
+                "source": f"profiles/{profile}"}
+
+    plain_kernel = f"k_backend<{st_plain[-1]['bin_size']}, {st_plain[-1]['bin_size'] // st_plain[-1]['wg_per_tile']}>"
+    out["plain"]["roofline"] = {
+        "bound": "hbm", "kernel": plain_kernel + " (one view)", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "algorithmic_bytes": be_bytes,
+        "traffic": load_traffic("k_backend_spheres_1080p_1m_traffic.md")}
+    vi = valu_issue(plain_kernel, "k_backend_spheres_1080p_1m_sq_counters.md", be_ms)
+    if vi:
+        out["plain"]["valu_issue"] = vi
     if fps_batched:
         out["batched"] = {"views": args.views, "frames_per_s": round(fps_batched, 2),
                           "ms_per_launch": round(ms_batched, 4), "device_ms_per_launch": round(dev_batched, 4),
                           "vs_plain": round(fps_batched / fps_plain, 3), "runs": batched_runs}
-        if args.views > 1:
-            out["batched"]["frags_raster_per_view"] = int(batched_frags(sr, scene, texs, args.views, args.bin_size))
+        per_view, covered_views = batched_stats(sr, scene, texs, args.views, args.bin_size)
+        out["batched"]["frags_raster_per_view"] = int(per_view)
+        # the dominant kernel of the path `value` measures: the batched back end,
+        # its algorithmic bytes summed over the launch's views
+        lay = batched_runs[("natural", "tricount", "learned")[args.tile_order]]["layout"]
+        kname = f"k_backend<{lay['bin_size']}, {lay['bin_size'] // lay['wg_per_tile']}>"
+        bb = sum(8 * scene.width * scene.height + 16 * c for c in covered_views)
+        ach_b = bb / (be_ms_batched * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "kernel": f"{kname} (batched x{args.views})", "achieved": round(ach_b, 2),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach_b / HBM_PEAK_GBS, 5),
+                           "algorithmic_bytes": bb, "launch_us": round(be_ms_batched * 1e3, 1),
+                           "traffic": load_traffic("k_backend_spheres_1080p_1m_x64_traffic.md"),
+                           "traffic_source": "profiles/k_backend_spheres_1080p_1m_x64_traffic.md"}
+        vi = valu_issue(kname, "k_backend_spheres_1080p_1m_x64_sq_counters.md", be_ms_batched)
+        if vi:
+            out["valu_issue"] = vi
+    else:
+        out["roofline"] = dict(out["plain"]["roofline"])
+        if "valu_issue" in out["plain"]:
+            out["valu_issue"] = out["plain"]["valu_issue"]
 
     if not args.no_parity or not args.no_cpu_baseline:
         from oracle.oracle import Oracle, Plane, make_texture

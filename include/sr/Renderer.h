@@ -6,7 +6,10 @@
 // Viewer/Scene.cpp:32-65 compiles unchanged apart from the include and the
 // matrix type. Differences a caller can observe:
 //   * RenderContext methods that can fail return an int status (0 = ok)
-//     instead of asserting; FrameBuffer takes the context that owns it.
+//     instead of asserting. A FrameBuffer lives in one context's device
+//     memory: FrameBuffer(w, h) binds to the most recently created live
+//     RenderContext, or, when none exists yet, to the first context that
+//     clears, draws to or blits it; FrameBuffer(ctx, w, h) names the context.
 //   * A PixelShaderFn pointer selects a device shader: the three shaders of
 //     Viewer/Shaders.h are provided here; any other pointer makes DrawIndexed
 //     return SRG_ERR_UNKNOWN_SHADER (there is no CPU fallback).
@@ -17,7 +20,10 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
+#include <cstddef>
 #include <map>
+#include <memory>
 #include <utility>
 #include <vector>
 
@@ -102,15 +108,19 @@ class RenderContext;
 
 // FrameBufferPlane / FrameBuffer (Renderer.h:58-108): double-buffered
 // planes of 64x64 tiles held in device memory.
+struct FrameBuffer;
+
 struct FrameBufferPlane
 {
     srg_fb* m_fb = nullptr;
     uint32_t m_plane = 0;
     uint32_t m_width = 0, m_height = 0, m_tilesX = 0, m_tilesY = 0;
+    FrameBuffer* m_owner = nullptr;  // binds the framebuffer when a draw reaches it unbound
 };
 
 struct FrameBuffer
 {
+    FrameBuffer(uint32_t _width, uint32_t _height, bool _colour = true, bool _depth = true);
     FrameBuffer(RenderContext& _ctx, uint32_t _width, uint32_t _height, bool _colour = true, bool _depth = true);
     ~FrameBuffer();
     FrameBuffer(FrameBuffer const&) = delete;
@@ -133,6 +143,45 @@ struct FrameBuffer
     srg_fb* m_fb = nullptr;
     RenderContext* m_ctx = nullptr;
     int m_status = 0;
+    uint32_t m_width = 0, m_height = 0, m_flags = 0;
+
+private:
+    void Init(uint32_t _width, uint32_t _height, bool _colour, bool _depth);
+    int BindTo(RenderContext& _ctx);
+    friend class RenderContext;
+};
+
+// TaskSystem's per-thread scratch allocator (TaskSystem.h:13, kt::LinearAllocator):
+// a host bump allocator per thread for host code written against
+// RenderContext::ThreadAllocator. The device path does not use it: its
+// scratch is the library's device arenas.
+class ThreadScratchAllocator
+{
+public:
+    void* Alloc(size_t _bytes, size_t _align = 16)
+    {
+        size_t at = (m_used + _align - 1) / _align * _align;
+        if (m_blocks.empty() || at + _bytes > m_cap) {
+            const size_t cap = _bytes + _align > (size_t(1) << 20) ? _bytes + _align : (size_t(1) << 20);
+            m_blocks.emplace_back(new uint8_t[cap]);
+            m_cap = cap;
+            m_used = 0;
+            const uintptr_t base = (uintptr_t)m_blocks.back().get();
+            at = (size_t)(((base + _align - 1) / _align * _align) - base);
+        }
+        m_used = at + _bytes;
+        return m_blocks.back().get() + at;
+    }
+    // Frees everything allocated since the last Reset (the frame's scratch).
+    void Reset()
+    {
+        if (m_blocks.size() > 1) m_blocks.erase(m_blocks.begin(), m_blocks.end() - 1);
+        m_used = 0;
+    }
+
+private:
+    std::vector<std::unique_ptr<uint8_t[]>> m_blocks;
+    size_t m_cap = 0, m_used = 0;
 };
 
 struct GenericDrawBuffer
@@ -207,16 +256,40 @@ public:
         cfg.struct_size = sizeof(cfg);
         if (_cfg) cfg = *_cfg;
         m_status = srg_create(&cfg, &m_ctx);
+        if (!m_status) m_prevCurrent = std::exchange(CurrentSlot(), this);
     }
-    ~RenderContext() { Shutdown(); }
+    ~RenderContext()
+    {
+        Shutdown();
+        if (CurrentSlot() == this) CurrentSlot() = m_prevCurrent;
+    }
     RenderContext(RenderContext const&) = delete;
     RenderContext& operator=(RenderContext const&) = delete;
 
     int Status() const { return m_status; }
     srg_ctx* Handle() const { return m_ctx; }
 
+    // The most recently created RenderContext still alive (FrameBuffer(w, h)
+    // binds to it), or nullptr.
+    static RenderContext* Current() { return CurrentSlot(); }
+
+    // RenderContext::ThreadAllocator (Renderer.h:165): the calling thread's
+    // host scratch allocator.
+    ThreadScratchAllocator& ThreadAllocator()
+    {
+        thread_local ThreadScratchAllocator alloc;
+        return alloc;
+    }
+
     void Shutdown()
     {
+        // srg_destroy frees the framebuffers' planes: detach them first
+        for (FrameBuffer* fb : m_fbs) {
+            fb->m_fb = nullptr;
+            fb->m_ctx = nullptr;
+            fb->m_bufferedPlanes[0].m_fb = fb->m_bufferedPlanes[1].m_fb = nullptr;
+        }
+        m_fbs.clear();
         if (m_ctx) srg_destroy(m_ctx);
         m_ctx = nullptr;
         m_buffers.clear();
@@ -236,12 +309,17 @@ public:
     int ClearFrameBuffer(FrameBuffer& _buffer, uint32_t _color = 0x00000000, bool _clearColour = true,
                          bool _clearDepth = true)
     {
+        if (int st = Bind(_buffer)) return st;
         return srg_clear(m_ctx, _buffer.m_fb, _color, _clearColour ? 1 : 0, _clearDepth ? 1 : 0);
     }
 
     int DrawIndexed(DrawCall const& _call)
     {
         if (!_call.m_indexBuffer.m_ptr || !_call.m_frameBuffer) return SRG_ERR_INVALID;
+        FrameBuffer* owner = _call.m_frameBuffer->m_owner;
+        if (owner && !owner->m_fb) {
+            if (int st = Bind(*owner)) return st;
+        }
         srg_draw d{};
         if (_call.m_pixelShader == &shader::UnlitDiffuseShader) d.shader = SRG_SHADER_UNLIT_DIFFUSE;
         else if (_call.m_pixelShader == &shader::VisualizeNormalsShader) d.shader = SRG_SHADER_VISUALIZE_NORMALS;
@@ -266,7 +344,7 @@ public:
         d.attribute_stride = ab.m_stride;
         d.attribute_num = ab.m_num;
         d.uv_offset = _call.m_uvOffset;
-        d.fb = _call.m_frameBuffer->m_fb;
+        d.fb = owner ? owner->m_fb : _call.m_frameBuffer->m_fb;
         d.plane = _call.m_frameBuffer->m_plane;
         memcpy(d.mvp, _call.m_mvp.m, sizeof(d.mvp));
         ++m_numDraws;
@@ -278,6 +356,7 @@ public:
     int Blit(FrameBuffer& _fb, uint8_t* _linearPixels, void (*_onFinishBlit)(void*) = nullptr,
              void* _onFinishUser = nullptr)
     {
+        if (int bst = Bind(_fb)) return bst;
         int st = srg_blit(m_ctx, _fb.m_fb, _linearPixels, _onFinishBlit, _onFinishUser);
         if (!st) _fb.m_writePlane ^= 1;
         return st;
@@ -296,6 +375,20 @@ public:
     }
 
 private:
+    static RenderContext*& CurrentSlot()
+    {
+        static RenderContext* current = nullptr;
+        return current;
+    }
+
+    // A framebuffer made without a context is created in this one on first
+    // use; one bound to another context is an error.
+    int Bind(FrameBuffer& _fb)
+    {
+        if (_fb.m_fb) return _fb.m_ctx == this ? SRG_OK : SRG_ERR_HANDLE;
+        return _fb.BindTo(*this);
+    }
+
     int Buffer(void const* _ptr, size_t _bytes, srg_handle* _out)
     {
         auto key = std::make_pair(_ptr, _bytes);
@@ -327,28 +420,59 @@ private:
     uint32_t m_numDraws = 0;
     std::map<std::pair<void const*, size_t>, srg_handle> m_buffers;
     std::map<Tex::TextureData const*, srg_handle> m_textures;
+    RenderContext* m_prevCurrent = nullptr;
+    std::vector<FrameBuffer*> m_fbs;  // framebuffers bound to this context
 
     friend struct FrameBuffer;
 };
 
-inline FrameBuffer::FrameBuffer(RenderContext& _ctx, uint32_t _width, uint32_t _height, bool _colour, bool _depth)
-    : m_ctx(&_ctx)
+inline void FrameBuffer::Init(uint32_t _width, uint32_t _height, bool _colour, bool _depth)
 {
-    uint32_t flags = (_colour ? SRG_FB_COLOUR : 0u) | (_depth ? SRG_FB_DEPTH : 0u);
-    m_status = srg_fb_create(_ctx.m_ctx, _width, _height, flags, &m_fb);
+    m_width = _width;
+    m_height = _height;
+    m_flags = (_colour ? SRG_FB_COLOUR : 0u) | (_depth ? SRG_FB_DEPTH : 0u);
     for (uint32_t i = 0; i < 2; ++i) {
-        m_bufferedPlanes[i].m_fb = m_fb;
         m_bufferedPlanes[i].m_plane = i;
         m_bufferedPlanes[i].m_width = _width;
         m_bufferedPlanes[i].m_height = _height;
         m_bufferedPlanes[i].m_tilesX = (_width + 63) / 64;
         m_bufferedPlanes[i].m_tilesY = (_height + 63) / 64;
+        m_bufferedPlanes[i].m_owner = this;
     }
+}
+
+inline int FrameBuffer::BindTo(RenderContext& _ctx)
+{
+    m_status = srg_fb_create(_ctx.m_ctx, m_width, m_height, m_flags, &m_fb);
+    if (m_status) {
+        m_fb = nullptr;
+        return m_status;
+    }
+    m_ctx = &_ctx;
+    _ctx.m_fbs.push_back(this);
+    for (uint32_t i = 0; i < 2; ++i) m_bufferedPlanes[i].m_fb = m_fb;
+    if (m_writePlane) srg_fb_swap_planes(m_fb);  // swapped before it was bound
+    return SRG_OK;
+}
+
+inline FrameBuffer::FrameBuffer(uint32_t _width, uint32_t _height, bool _colour, bool _depth)
+{
+    Init(_width, _height, _colour, _depth);
+    if (RenderContext* ctx = RenderContext::Current()) BindTo(*ctx);
+}
+
+inline FrameBuffer::FrameBuffer(RenderContext& _ctx, uint32_t _width, uint32_t _height, bool _colour, bool _depth)
+{
+    Init(_width, _height, _colour, _depth);
+    BindTo(_ctx);
 }
 
 inline FrameBuffer::~FrameBuffer()
 {
-    if (m_fb && m_ctx && m_ctx->m_ctx) srg_fb_destroy(m_ctx->m_ctx, m_fb);
+    if (!m_ctx) return;  // never bound, or its context has shut down
+    if (m_ctx->m_ctx) srg_fb_destroy(m_ctx->m_ctx, m_fb);
+    auto& v = m_ctx->m_fbs;
+    v.erase(std::remove(v.begin(), v.end(), this), v.end());
 }
 
 }  // namespace sr

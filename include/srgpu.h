@@ -131,7 +131,7 @@ typedef struct srg_frame_stats {
     uint32_t retries;         /* frame re-runs after arena growth */
     uint32_t graph_captures;  /* SRG_CFG_GRAPH: graphs (re)captured by this frame */
     float ms_frontend;        /* SRG_CFG_TIMING without SRG_CFG_GRAPH: setup + clip + scan + emit */
-    float ms_backend;         /* SRG_CFG_TIMING without SRG_CFG_GRAPH: raster + shade kernel */
+    float ms_backend;         /* SRG_CFG_TIMING without SRG_CFG_GRAPH: the raster + shade kernel alone */
     float ms_total;           /* SRG_CFG_TIMING: whole frame on the device */
     float sched_err;          /* SRG_ORDER_LEARNED: mean |log2 predicted - log2 measured| bin cost */
     uint32_t sched_trained;   /* SRG_ORDER_LEARNED: 1 once the model orders the queue */
@@ -225,9 +225,9 @@ int srg_end_frame(srg_ctx* ctx);
  * used nor trained), the autotuner's choice for the frame's signature is used
  * only once it is locked (no sweep; otherwise the configured layout), and no
  * graph is captured. A frame whose arenas do not fit returns SRG_ERR_OVERFLOW
- * and draws nothing (its planes keep their pending clears); about 160 bytes per
- * triangle plus 4 per triangle-bin entry, i.e. roughly 350k triangles per
- * frame. Every call on one GPU shares that GPU's array: calls that overlap on
+ * and draws nothing (its planes keep their pending clears); about 110-130
+ * bytes per triangle plus 16 per triangle-bin entry, i.e. roughly 350k
+ * triangles per frame. Every call on one GPU shares that GPU's array: calls that overlap on
  * one GPU (two threads, two contexts) race and must be serialised by the
  * caller. Each call is synchronous and initialises every part of the array it
  * reads. */
@@ -264,6 +264,16 @@ int srg_get_bin_stats(srg_ctx* ctx, uint32_t* counts, uint64_t* costs, uint32_t 
 #define SRG_SCHED_PARAMS 1249u
 int srg_sched_get_model(srg_ctx* ctx, float* params, uint32_t count);
 int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count);
+/* Test access to the model's device kernels (no counterpart in the
+ * reference). srg_sched_eval: the predicted log2 cost of n rows of 4
+ * features (the inference the learned order runs per bin), with the current
+ * parameters. srg_sched_train: replaces the recorded samples with n rows of
+ * (4 features, log2 cost), clears the Adam state and runs `steps` steps of
+ * the device's mini-batch Adam (batch 256, each thread drawing its sample
+ * with the generator seeded by `seed`) at learning rate lr; the parameters
+ * are then read with srg_sched_get_model. n <= 262144. */
+int srg_sched_eval(srg_ctx* ctx, const float* features, uint32_t n, float* log2_cost);
+int srg_sched_train(srg_ctx* ctx, const float* samples, uint32_t n, uint32_t steps, float lr, uint32_t seed);
 /* The autotuner's table for the last frame's signature (SRG_CFG_AUTOTUNE).
  * No counterpart in the reference, whose bin size is a compile-time constant
  * (Config.h:18-21) and whose tiles each take one task (Renderer.cpp:264-306). */

@@ -48,7 +48,8 @@ SRG_SYMBOLS = (
     "srg_texture_build_rgba8", "srg_texture_create_rgba8", "srg_texture_read", "srg_blit_async", "srg_blit_wait",
     "srg_get_tuning", "srg_draw_indexed_n", "srg_obj_load", "srg_obj_parse", "srg_obj_mesh_count",
     "srg_obj_get_mesh", "srg_obj_material_count", "srg_obj_material_name", "srg_obj_material_diffuse",
-    "srg_obj_free", "srg_set_sponza_constants", "srg_clear_n", "srg_end_frame_noalloc",
+    "srg_obj_free", "srg_set_sponza_constants", "srg_clear_n", "srg_end_frame_noalloc", "srg_sched_eval",
+    "srg_sched_train",
 )
 OBJ_FLIP_WINDING, OBJ_FLIP_UVS = 1, 4  # Obj::LoadFlags (Viewer/Obj.h:54-60)
 PLANE_WRITE = 0xFFFFFFFF
@@ -151,6 +152,8 @@ def load_library(path=None):
         "srg_get_bin_stats": (i32, [vp, vp, vp, u32, ctypes.POINTER(u32)]),
         "srg_sched_get_model": (i32, [vp, vp, u32]),
         "srg_sched_set_model": (i32, [vp, vp, u32]),
+        "srg_sched_eval": (i32, [vp, vp, u32, vp]),
+        "srg_sched_train": (i32, [vp, vp, u32, u32, ctypes.c_float, u32]),
         "srg_get_tuning": (i32, [vp, ctypes.POINTER(SrgTuning)]),
         "srg_draw_indexed_n": (i32, [vp, vp, u32]),
         "srg_obj_load": (i32, [ctypes.c_char_p, u32, ctypes.POINTER(vp)]),
@@ -694,6 +697,19 @@ class RenderContext:
         if buf.size != 6 + SPONZA_LIGHTS * 8:
             raise ValueError("SponzaScene::Constants is 134 floats")
         _check(self._lib.srg_set_sponza_constants(self._ctx, _ptr(buf)), "set_sponza_constants")
+
+    def sched_eval(self, features):
+        """The model's predicted log2 cost of rows of 4 features (srg_sched_eval)."""
+        x = np.ascontiguousarray(features, np.float32).reshape(-1, 4)
+        out = np.zeros(len(x), np.float32)
+        _check(self._lib.srg_sched_eval(self._ctx, _ptr(x), len(x), _ptr(out)), "sched_eval")
+        return out
+
+    def sched_train(self, samples, steps, lr, seed):
+        """Runs the device's Adam on rows of (4 features, log2 cost) from a
+        cleared optimiser state (srg_sched_train)."""
+        x = np.ascontiguousarray(samples, np.float32).reshape(-1, 5)
+        _check(self._lib.srg_sched_train(self._ctx, _ptr(x), len(x), steps, lr, seed), "sched_train")
 
     def tuning(self):
         """Autotuner state for the last frame's signature: {"state": "off" |

@@ -50,9 +50,10 @@ __device__ __forceinline__ uint32_t shade_sponza(const FrameDev& F, const DrawDe
     if (D.tex == kNoTex) return 0xFFFFFFFFu;
     const TexDev& T = F.texs[D.tex];
     if (T.empty) return 0xFFFFFFFFu;
-    const float4* Sp = F.rec_shade + (size_t)rec * 4;
+    const float* Sp = F.rec_shade + (size_t)rec * F.shade_stride;
     const float4* Ep = F.rec_ext + (size_t)rec * 3;
-    const float4 s0 = Sp[0], s1 = Sp[1], s2 = Sp[2], s3 = Sp[3];
+    const float4 s0 = make_float4(Sp[0], Sp[1], Sp[2], Sp[3]), s1 = make_float4(Sp[4], Sp[5], Sp[6], Sp[7]);
+    const float4 s2 = make_float4(Sp[8], Sp[9], Sp[10], Sp[11]), s3 = make_float4(Sp[12], Sp[13], Sp[14], 0.0f);
     const float4 e0 = Ep[0], e1 = Ep[1], e2 = Ep[2];
     const uint4 q2 = F.rec_raster[(size_t)rec * 4 + 2];
     const uint4 q3 = F.rec_raster[(size_t)rec * 4 + 3];
@@ -152,9 +153,10 @@ __device__ __forceinline__ uint32_t shade_pixel(const FrameDev& F, uint32_t rec,
     const uint32_t d = F.rec_meta[rec];
     const DrawDev& D = F.draws[d];
     if (WIDE && D.shader == SRG_SHADER_SPONZA) return shade_sponza(F, D, rec, X0, Y0, px, py);
-    const float4* Sp = F.rec_shade + (size_t)rec * 4;
-    const float4 s0 = Sp[0], s1 = Sp[1];
-    const float pf01 = ((const float*)Sp)[8];
+    // shade record: (wdx, wdy, w0), then (dx, dy, value) of each kept slot
+    const float* Sp = F.rec_shade + (size_t)rec * F.shade_stride;
+    const float4 s0 = make_float4(Sp[0], Sp[1], Sp[2], Sp[3]), s1 = make_float4(Sp[4], Sp[5], Sp[6], Sp[7]);
+    const float pf01 = Sp[8];
     const uint32_t* Rw = (const uint32_t*)(F.rec_raster + (size_t)rec * 4);
     // plane constants re-based to the tile origin (Binning.cpp:435-452)
     const float sx = (float)X0 - ffrom(Rw[11]), sy = (float)Y0 - ffrom(Rw[12]);
@@ -171,7 +173,7 @@ __device__ __forceinline__ uint32_t shade_pixel(const FrameDev& F, uint32_t rec,
     const float u = tv[0], v = tv[1];
     switch (D.shader) {
     case SRG_SHADER_VISUALIZE_NORMALS: {
-        const float4 s2 = Sp[2];
+        const float4 s2 = make_float4(Sp[8], Sp[9], Sp[10], Sp[11]);
         const float c2 = s2.y * sx + s2.z * sy + s2.w;
         const float n2 = iq * __builtin_fmaf(s2.z, Y, __builtin_fmaf(s2.y, X, c2));
         return pack_rgba(__builtin_fmaf(u, 0.5f, 0.5f), __builtin_fmaf(v, 0.5f, 0.5f), __builtin_fmaf(n2, 0.5f, 0.5f),
@@ -185,7 +187,7 @@ __device__ __forceinline__ uint32_t shade_pixel(const FrameDev& F, uint32_t rec,
     const TexDev& T = F.texs[D.tex];
     if (T.empty) return 0xFFFFFFFFu;
     if (D.deriv_slot != 0) {  // uv_offset != 6: derivatives from slots 2 and 3
-        const float4 s2 = Sp[2], s3 = Sp[3];
+        const float4 s2 = make_float4(Sp[8], Sp[9], Sp[10], Sp[11]), s3 = make_float4(Sp[12], Sp[13], Sp[14], 0.0f);
         tdx[0] = s2.y; tdy[0] = s2.z; tdx[1] = s3.x; tdy[1] = s3.y;
         tc[0] = tdx[0] * sx + tdy[0] * sy + s2.w;
         tc[1] = tdx[1] * sx + tdy[1] * sy + s3.z;
@@ -348,14 +350,23 @@ __device__ __forceinline__ EntryRaster entry_raster(uint4 q0, uint4 q1, uint4 q2
 //
 // A block the reference rejects by its 64-wide corner test is skipped and a
 // fully accepted one is covered without per-pixel tests (Rasterizer.cpp:
-// 230-272). For a tight primitive (tight_bound in sr_frontend.h) only the
-// rows and columns of its bound are visited, and the corner test is needed
-// only where some edge value is exactly 0: with exact edge values a rejected
-// block has no pixel with all three values positive, and a fully accepted
-// block no pixel with any negative.
+// 230-272); otherwise a pixel is covered when its three edge values are
+// >= 0. Per item this is one threshold on min(E0, E1, E2): INT_MIN (full),
+// 0 (test pixels). For a tight primitive (tight_bound in sr_frontend.h) only
+// the rows and columns of its bound are visited and the corner test matters
+// only where some edge value is exactly 0 (with exact edge values a rejected
+// block has no pixel whose three values are > 0, a fully accepted block no
+// pixel with a negative one): its threshold is 1, and pixels whose minimum
+// is exactly 0 are collected and decided by the corner test after the rows,
+// with the depth the row walk gives them.
+// Words of a wave's LDS staging area: 15 fields of each of its 64 entries,
+// field-major (field f of lane j at f * 64 + j), then the 64 owner marks.
+constexpr uint32_t kStageFields = 15;
+constexpr uint32_t kStageWords = (kStageFields + 1u) * 64u;
+
 template <uint32_t S, bool STATS>
 __device__ __forceinline__ void raster_wave(unsigned long long* keys, const EntryRaster& e, bool has, uint32_t sx0,
-                                            uint32_t sy0, unsigned long long& frags, uint32_t* mark)
+                                            uint32_t sy0, unsigned long long& frags, uint32_t* stage)
 {
     const uint32_t lane = lane_id();
     const uint32_t cw = e.cx1 > e.cx0 ? e.cx1 - e.cx0 : 0u;
@@ -368,56 +379,60 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         if (lane >= o) incl += y;
     }
     const uint32_t total = bcast(incl, 63);
-    // per-entry values the column workers need, packed
-    const uint32_t magic = (65536u + max(np, 1u) - 1u) / max(np, 1u);  // k / np == (k * magic) >> 16 for k < 512
+    // per-entry values the column workers need, packed; the first block row
+    // is cy0 rounded down to 8
+    const uint32_t npp = max(np, 1u);
+    const uint32_t magic = (65536u + npp - 1u) / npp;  // k / np == (k * magic) >> 16 for k < 512
     const uint32_t pk_cand = e.cx0 | (cw << 7) | (e.cy0 << 14) | (e.cy1 << 21) | ((e.tight ? 1u : 0u) << 28);
-    const uint32_t pk_div = e.by0 | (magic << 7);
+    const uint32_t pk_div = magic | (npp << 17);
     const uint32_t first = incl - n;
-    const uint32_t low = (uint32_t)e.low;
-    uint32_t carry = 0;  // 1 + owner of the previous pass's last item
+    // each lane's entry into the wave's staging area: items read their
+    // owner's fields from there (a broadcast where lanes share an owner)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the previous entries' reads are issued
+    {
+        const uint32_t v[kStageFields] = { e.ct[0], e.ct[1], e.ct[2], e.dxu[0], e.dxu[1], e.dxu[2], e.dyu[0], e.dyu[1],
+                                           e.dyu[2], pk_cand, pk_div, fbits(e.zdx), fbits(e.zdy), fbits(e.zc0),
+                                           (uint32_t)e.low };
+#pragma unroll
+        for (uint32_t f = 0; f < kStageFields; ++f) stage[f * 64u + lane] = v[f];
+    }
+    uint32_t* mark = stage + kStageFields * 64u;
+    uint32_t carry = 0;  // mark of the previous pass's last item
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t it = base + lane;
         const bool valid = it < total;
-        const uint32_t m = wave_owner(mark, n, first, base, carry);
-        const int src = (int)(valid && m ? min(m - 1u, 63u) : 0u);
+        const uint32_t m = wave_owner_first(mark, n, first, base, carry);
+        const uint32_t mo = m & 127u;
+        const int src = (int)(valid && mo ? min(mo - 1u, 63u) : 0u);
+        const uint32_t* o = stage + src;  // the owner's fields (wave_owner_first fenced the stores)
         uint32_t ct[3], dxu[3], dyu[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            ct[k] = __shfl(e.ct[k], src, 64);
-            dxu[k] = __shfl(e.dxu[k], src, 64);
-            dyu[k] = __shfl(e.dyu[k], src, 64);
+            ct[k] = o[k * 64];
+            dxu[k] = o[(3 + k) * 64];
+            dyu[k] = o[(6 + k) * 64];
         }
-        const uint32_t pc = __shfl(pk_cand, src, 64), pd = __shfl(pk_div, src, 64);
-        const float zdx = __shfl(e.zdx, src, 64), zdy = __shfl(e.zdy, src, 64), zc0 = __shfl(e.zc0, src, 64);
-        const unsigned long long lw = (unsigned long long)__shfl(low, src, 64);
-        const uint32_t k = it - __shfl(first, src, 64);
+        const uint32_t pc = o[9 * 64], pd = o[10 * 64];
+        const float zdx = ffrom(o[11 * 64]), zdy = ffrom(o[12 * 64]), zc0 = ffrom(o[13 * 64]);
+        const unsigned long long lw = (unsigned long long)o[14 * 64];
+        const uint32_t k = it - (m >> 7);
         const uint32_t cx0 = pc & 127u, ccw = (pc >> 7) & 127u, cy0 = (pc >> 14) & 127u, cy1 = (pc >> 21) & 127u;
         const bool tight = ((pc >> 28) & 1u) != 0;
-        const uint32_t snp = ccw ? ((cx0 + ccw - 1u) >> 1) - (cx0 >> 1) + 1u : 1u;
-        const uint32_t brow = (k * (pd >> 7)) >> 16;
-        const uint32_t x = 2u * ((cx0 >> 1) + (k - brow * snp)), yb = (pd & 127u) + 8u * brow;
+        const uint32_t snp = pd >> 17;
+        const uint32_t brow = (k * (pd & 0x1FFFFu)) >> 16;
+        const uint32_t x = 2u * ((cx0 >> 1) + (k - brow * snp)), yb = (cy0 & ~7u) + 8u * brow;
         const uint32_t xb = x & ~7u, l = x & 7u;
         // the pair's columns inside the owner's candidate rectangle (checked, not assumed)
         const bool in0 = x - cx0 < ccw, in1 = x + 1u - cx0 < ccw;
         const bool inside = valid && yb < cy1 && (in0 || in1);
         const uint32_t r_lo = inside && cy0 > yb ? cy0 - yb : 0u;
-        const uint32_t r_hi = inside ? min(cy1 - yb, 8u) : 0u;
-        // per-pixel rule: covered = full || (test && all edge values >= 0)
-        bool full = false, test = true;
+        uint32_t r_hi = inside ? min(cy1 - yb, 8u) : 0u;
+        int32_t thr = 1;  // tight: all three edge values > 0; an exact 0 is decided below
         if (inside && !tight) {
             const uint32_t cls = block_class(ct, dxu, dyu, xb, yb);
-            full = cls == 2u;
-            test = cls == 1u;
+            thr = cls == 2u ? INT32_MIN : 0;
+            if (cls == 0u) r_hi = r_lo;  // rejected block
         }
-        float z0 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
-        float z1 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)(l + 1u), zdx, zc0)) + (float)xb * zdx;
-        for (uint32_t r = 0; r < r_lo; ++r) {
-            z0 = z0 + zdy;
-            z1 = z1 + zdy;
-        }
-        uint32_t E[3];
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * (yb + r_lo);
         // key_slot of (x - sx0, yb + r - sy0): rows of S keys, the in-group
         // rotation alternating with the row's parity (yb is a multiple of 8);
         // x is even, so x + 1 is in the same group of G keys
@@ -425,29 +440,48 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         const uint32_t sxl = x - sx0, ybl = yb - sy0;
         const uint32_t t = sxl + 2u * (ybl >> 3) + (sxl >> 4);
         const uint32_t grp = ybl * S + (sxl & ~(G - 1u));
-        for (uint32_t r = r_lo; r < r_hi; ++r) {
-            const uint32_t tr = t + ((r & 1u) << 3), ro = grp + r * S;
-            const uint32_t F0 = E[0] + dyu[0], F1 = E[1] + dyu[1], F2 = E[2] + dyu[2];
-            const bool pos0 = (int32_t)(E[0] | E[1] | E[2]) >= 0;
-            const bool pos1 = (int32_t)(F0 | F1 | F2) >= 0;
-            bool cov0 = in0 & (full | (test & pos0));
-            bool cov1 = in1 & (full | (test & pos1));
-            if (tight & ((in0 & pos0 & (min(min(E[0], E[1]), E[2]) == 0u)) | (in1 & pos1 & (min(min(F0, F1), F2) == 0u)))) {
-                // an edge through a pixel: the block test decides that pixel
-                const bool bc = block_class(ct, dxu, dyu, xb, yb) != 0u;
-                if (in0 & pos0 & (min(min(E[0], E[1]), E[2]) == 0u)) cov0 = bc;
-                if (in1 & pos1 & (min(min(F0, F1), F2) == 0u)) cov1 = bc;
+        const bool t0 = tight & in0, t1 = tight & in1;
+        // One walk down the block row. The first covers the pixels whose
+        // minimum edge value reaches thr and reports whether a tight pixel has
+        // a minimum of exactly 0; the second, taken only where the corner test
+        // accepts the block, covers those pixels.
+        auto walk = [&](bool second) -> bool {
+            float z0 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
+            float z1 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)(l + 1u), zdx, zc0)) + (float)xb * zdx;
+            for (uint32_t r = 0; r < r_lo; ++r) {
+                z0 = z0 + zdy;
+                z1 = z1 + zdy;
             }
-            if (STATS) frags += (cov0 ? 1u : 0u) + (cov1 ? 1u : 0u);
-            if (cov0 & (z0 > 0.0f))
-                atomicMax(&keys[ro + (tr & (G - 1u))], ((unsigned long long)fbits(z0) << 32) | lw);
-            if (cov1 & (z1 > 0.0f))
-                atomicMax(&keys[ro + ((tr + 1u) & (G - 1u))], ((unsigned long long)fbits(z1) << 32) | lw);
-            z0 = z0 + zdy;
-            z1 = z1 + zdy;
+            uint32_t E[3];
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) E[kk] += dxu[kk];
-        }
+            for (int kk = 0; kk < 3; ++kk) E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * (yb + r_lo);
+            bool zero = false;
+            for (uint32_t r = r_lo; r < r_hi; ++r) {
+                const uint32_t tr = t + ((r & 1u) << 3), ro = grp + r * S;
+                const int32_t m0 = min(min((int32_t)E[0], (int32_t)E[1]), (int32_t)E[2]);
+                const int32_t m1 =
+                    min(min((int32_t)(E[0] + dyu[0]), (int32_t)(E[1] + dyu[1])), (int32_t)(E[2] + dyu[2]));
+                const bool e0 = t0 & (m0 == 0), e1 = t1 & (m1 == 0);
+                const bool cov0 = second ? e0 : in0 & (m0 >= thr);
+                const bool cov1 = second ? e1 : in1 & (m1 >= thr);
+                zero = zero | e0 | e1;
+                if (STATS) frags += (cov0 ? 1u : 0u) + (cov1 ? 1u : 0u);
+                if (cov0 & (z0 > 0.0f))
+                    atomicMax(&keys[ro + (tr & (G - 1u))], ((unsigned long long)fbits(z0) << 32) | lw);
+                if (cov1 & (z1 > 0.0f))
+                    atomicMax(&keys[ro + ((tr + 1u) & (G - 1u))], ((unsigned long long)fbits(z1) << 32) | lw);
+                z0 = z0 + zdy;
+                z1 = z1 + zdy;
+#pragma unroll
+                for (int kk = 0; kk < 3; ++kk) E[kk] += dxu[kk];
+            }
+            return zero;
+        };
+        const bool zero = walk(false);
+        // an edge exactly through a pixel of a tight primitive: the corner
+        // test decides it (rare; a wave-uniform branch)
+        if (__ballot(zero))
+            if (zero && block_class(ct, dxu, dyu, xb, yb) != 0u) (void)walk(true);
     }
 }
 
@@ -464,8 +498,10 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
     static_assert(H % 8u == 0 && S % H == 0, "strips are whole 8-pixel block rows of the bin");
     static_assert(NPIX % T == 0 && PPT % BATCH == 0, "strip must hold a whole number of pixels per thread");
     __shared__ unsigned long long keys[NPIX];
-    __shared__ uint32_t s_q;
-    __shared__ uint32_t s_mark[W * 64];
+    __shared__ uint32_t s_stage[W * kStageWords];
+    // the next queue item, handed to the workgroup through the first key slot
+    // (free between the end of one item's shading and its key initialisation)
+    uint32_t& s_q = *(uint32_t*)&keys[0];
     const uint32_t tid = threadIdx.x, lane = tid & 63u;
     const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(tid >> 6));
     unsigned long long frags = 0;
@@ -512,7 +548,7 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
                 order = r < F.ntris ? ((r << 3) | kFanUnclipped) : F.clip_order[r - F.ntris];
             }
             const EntryRaster e = entry_raster(q0, q1, q2, q3, order, X0, Y0, sx0, sy0, S, H);
-            raster_wave<S, STATS>(keys, e, has, sx0, sy0, frags, s_mark + wave * 64u);
+            raster_wave<S, STATS>(keys, e, has, sx0, sy0, frags, s_stage + wave * kStageWords);
         }
         __syncthreads();
 

@@ -61,6 +61,8 @@ enum : uint32_t {
     C_FRAGS = 8,       // 64-bit rasterised fragment count (words 8, 9)
     C_SCHED_ERR = 10,  // float: summed |log2 predicted - log2 measured| bin cost
     C_SCHED_N = 11,    // bins recorded by the scheduler this frame
+    C_PAIR_NEED = 12,  // on a pair region overflow: the largest count a region reached
+    C_CLIP_PAIR_NEED = 13, // on an overflow of k_clip's pair region: the count it reached
     C_NCOUNTERS = 16,
 };
 
@@ -85,7 +87,9 @@ struct FrameDev {
     uint32_t ndraws, nviews, total_bins, ntris;
     uint32_t bin_k, bin_size, clip_cap, entry_cap;
     uint4* rec_raster;     // 4 x uint4 per record: edges, bbox, r0, z plane
-    float4* rec_shade;     // 4 x float4 per record: 1/w plane and attribute planes 0-3
+    float* rec_shade;      // shade_stride floats per record: 1/w plane, then attribute planes 0-1 (stride 9)
+                           // or 0-3 (stride 15), each plane (dx, dy, value at v0)
+    uint32_t shade_stride;
     float4* rec_ext;       // 3 x float4 per record: attribute planes 4-7 (wide frames only)
     const SponzaDev* sponza;  // SponzaShader constants (wide frames only)
     const uint32_t* sched_gate;  // learned order: [1] != 0 on frames that predict and record (else null)
@@ -95,15 +99,33 @@ struct FrameDev {
     uint32_t* clip_order;  // per fan record: order key
     uint32_t* bin_count;   // entries per bin (written by the scan)
     uint32_t* bin_start;   // total_bins + 1
-    uint32_t* part_count;  // kBinParts x total_bins: entries per (part, bin)
-    uint32_t* part_cursor; // kBinParts x total_bins: append cursors
+    uint32_t* part_count;  // kBinParts x total_bins: entries per (part, bin); the count walks' slot cursors
     uint32_t* part_off;    // kBinParts x total_bins: first entry of a part inside its bin's list
     uint32_t* entries;
+    // (record, part-bin key, slot in the part's list) of every entry, as the
+    // count walks find them: three arrays of pair_total words, one after
+    // another; k_scatter writes entries from them after the scan. Region r <
+    // pair_regions (k_setup's workgroup index mod pair_regions) fills
+    // [r * pair_region_cap, (r + 1) * pair_region_cap) through its own
+    // counter pair_count[r * kPairStride]; k_clip's workgroups fill the
+    // kClipPairRegions regions after them (workgroup index mod
+    // kClipPairRegions), clip_pair_cap words each. k_setup's regions
+    // therefore get the same pairs every frame; k_clip takes its triangles in
+    // queue order, and its regions are sized generously.
+    uint32_t* pairs;
+    uint32_t* pair_count;
+    uint32_t pair_regions, pair_region_cap, clip_pair_cap, pair_total;
     uint32_t* queue;
     uint64_t* bin_cost;    // optional per bin (s_memtime ticks)
     uint32_t* counters;
     uint32_t flags;
 };
+
+// Pair regions: one counter per region on its own 64-byte line, so that the
+// walks of different workgroups do not contend on one address. A frame uses
+// as many regions as k_setup has workgroups, at most kPairRegions, so that
+// every region holds a share of the frame's pairs.
+constexpr uint32_t kPairRegions = 256, kClipPairRegions = 8, kPairStride = 16;
 
 // A bin's counter and append cursor are split into kBinParts parts, the part
 // chosen by the entry's record (64 consecutive records share one), so the
@@ -219,6 +241,25 @@ __device__ __forceinline__ uint32_t wave_owner(uint32_t* mark, uint32_t n, uint3
     mark[lane] = 0;
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (n && first >= base && first - base < 64u) mark[first - base] = lane + 1u;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const uint32_t m = max(wave_incl_max(mark[lane]), carry);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    carry = readlane_u32(m, 63);
+    return m;
+}
+
+// wave_owner that also hands each item its owner's first item: an owner's
+// mark is (first << 7) | (lane + 1). Owners' first items strictly increase
+// with the lane, so the marks do too and the same max-scan yields both:
+// owner = (m & 127) - 1, first = m >> 7 (first < 2^25). Same contract as
+// wave_owner.
+__device__ __forceinline__ uint32_t wave_owner_first(uint32_t* mark, uint32_t n, uint32_t first, uint32_t base,
+                                                     uint32_t& carry)
+{
+    const uint32_t lane = lane_id();
+    mark[lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (n && first >= base && first - base < 64u) mark[first - base] = (first << 7) | (lane + 1u);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const uint32_t m = max(wave_incl_max(mark[lane]), carry);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");

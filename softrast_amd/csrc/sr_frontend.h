@@ -6,7 +6,8 @@
 //   k_clip   slow path over the queued triangles: Sutherland-Hodgman clip,
 //            fan, setup of each fan triangle, bin counts
 //   k_scan   exclusive scan of bin counts + the back end's work queue
-//   k_emit   one lane per record: writes the record index into each bin list
+//   k_scatter one lane per (record, bin) pair found by the count walks:
+//            writes the record index into its bin list
 #pragma once
 #include "sr_frame.h"
 
@@ -146,6 +147,35 @@ __device__ __forceinline__ bool setup_prim(Prim& s, const float4 v[3], const flo
     return true;
 }
 
+// True when a set-up primitive provably covers no pixel, so that it needs no
+// record, no bin entry and no back-end work. With a tight bound (closed on
+// every side) the reference can accept only pixels of that bound lying in the
+// blocks it walks: columns from xmin & ~7 up to, not including, xmax rounded
+// up to a multiple of 8 (the exclusive block end), rows likewise. It accepts a
+// pixel only when the pixel's three edge values E_k = c_k + dy_k x + dx_k y
+// (int32, wrapping) are >= 0: all > 0, or some = 0 and the block test passes.
+// So a primitive whose candidate pixels (at most four are tested) all have a
+// negative edge value draws nothing.
+__device__ __forceinline__ bool covers_nothing(const Prim& s)
+{
+    if (!(s.tight & 1u) || (s.tight & 0x18u)) return false;  // no bound, or open on a side
+    const int32_t x0 = max(s.xmin - (int32_t)((s.tight >> 1) & 1u), s.xmin & ~7);
+    const int32_t x1 = min(s.xmax, ((s.xmax + 7) & ~7) - 1);
+    const int32_t y0 = max(s.ymin - (int32_t)((s.tight >> 2) & 1u), s.ymin & ~7);
+    const int32_t y1 = min(s.ymax, ((s.ymax + 7) & ~7) - 1);
+    if (x0 > x1 || y0 > y1) return true;
+    if ((x1 - x0 + 1) * (y1 - y0 + 1) > 4) return false;
+    for (int32_t y = y0; y <= y1; ++y)
+        for (int32_t x = x0; x <= x1; ++x) {
+            int32_t m = INT32_MAX;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                m = min(m, (int32_t)((uint32_t)s.c[k] + (uint32_t)s.dy[k] * (uint32_t)x + (uint32_t)s.dx[k] * (uint32_t)y));
+            if (m >= 0) return false;
+        }
+    return true;
+}
+
 // 64x64 bin corner test (Binning.cpp:376-409): false if for some edge no
 // corner of the bin is strictly inside.
 __device__ __forceinline__ bool tile_accepts(const int32_t c[3], const int32_t dx[3], const int32_t dy[3], uint32_t X0,
@@ -208,8 +238,10 @@ __device__ __forceinline__ uint32_t reference_tile_entries(const Prim& s)
     return n;
 }
 
-// Counts (emit = false) or appends (emit = true) a primitive's bin entries,
-// wave-aggregated. Uniform control flow required; `active` lanes carry a
+// Counts a primitive's bin entries, wave-aggregated, and records each
+// (record, bin) pair with its slot in the (part, bin) list: the count
+// atomic's old value plus the lane's rank among the wave's lanes adding to
+// the same counter. Uniform control flow required; `active` lanes carry a
 // primitive described by (c, dx, dy, bbox) in view V.
 //
 // The wave's (primitive, candidate bin) pairs are compacted: an inclusive
@@ -219,7 +251,7 @@ __device__ __forceinline__ uint32_t reference_tile_entries(const Prim& s)
 // its largest primitive has candidates.
 __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, bool active, const int32_t c[3],
                                           const int32_t dx[3], const int32_t dy[3], int32_t xmin, int32_t xmax,
-                                          int32_t ymin, int32_t ymax, bool emit, uint32_t rec, uint32_t* ht)
+                                          int32_t ymin, int32_t ymax, uint32_t rec, uint32_t* ht, bool clip)
 {
     BinWalk w;
     if (active) {
@@ -267,11 +299,22 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
         }
         const uint32_t key = act ? bb + sby * vbx + sbx : 0u;
         const uint32_t pkey = part_of(orec) * F.total_bins + key;  // the (part, bin) counter
-        if (!emit) {
-            wave_count_lds(ht, F.part_count, pkey, act);
-        } else {
-            const uint32_t slot = wave_add_slot_lds(ht, F.part_cursor, pkey, act);
-            if (act) F.entries[F.bin_start[key] + F.part_off[pkey] + slot] = orec;
+        const uint32_t slot = wave_add_slot_lds(ht, F.part_count, pkey, act);
+        const uint32_t region = clip ? F.pair_regions + blockIdx.x % kClipPairRegions : blockIdx.x % F.pair_regions;
+        const uint32_t cap = clip ? F.clip_pair_cap : F.pair_region_cap;
+        const uint32_t first_p = clip ? F.pair_regions * F.pair_region_cap + (region - F.pair_regions) * F.clip_pair_cap
+                                      : region * F.pair_region_cap;
+        const uint32_t ri = wave_add_amount(&F.pair_count[region * kPairStride], act ? 1u : 0u);
+        if (act) {
+            if (ri < cap) {
+                const uint32_t p = first_p + ri, np = F.pair_total;
+                F.pairs[p] = orec;
+                F.pairs[np + p] = pkey;
+                F.pairs[2u * np + p] = slot;
+            } else {
+                atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
+                atomicMax(&F.counters[clip ? C_CLIP_PAIR_NEED : C_PAIR_NEED], ri + 1u);
+            }
         }
     }
 }
@@ -290,11 +333,16 @@ __device__ __forceinline__ void write_record(const FrameDev& F, uint32_t r, cons
                           (((s.tight >> 4) & 1u) << 30),
                       fbits(s.r0x));
     R[3] = make_uint4(fbits(s.r0y), fbits(s.zdx), fbits(s.zdy), fbits(s.zf0));
-    float4* S = F.rec_shade + (size_t)r * 4;
-    S[0] = make_float4(s.wdx, s.wdy, s.wf0, s.adx[0]);
-    S[1] = make_float4(s.ady[0], s.af0[0], s.adx[1], s.ady[1]);
-    S[2] = make_float4(s.af0[1], s.adx[2], s.ady[2], s.af0[2]);
-    S[3] = make_float4(s.adx[3], s.ady[3], s.af0[3], 0.0f);
+    // the 1/w plane and the kept attribute planes, packed: 9 floats for a
+    // two-plane frame (NS = 2), 15 otherwise
+    float* S = F.rec_shade + (size_t)r * (NS <= 2 ? 9u : 15u);
+    S[0] = s.wdx; S[1] = s.wdy; S[2] = s.wf0;
+#pragma unroll
+    for (uint32_t k = 0; k < (NS <= 2 ? 2u : 4u); ++k) {
+        S[3 + 3 * k] = s.adx[k];
+        S[4 + 3 * k] = s.ady[k];
+        S[5 + 3 * k] = s.af0[k];
+    }
     if (NS > kMaxSlots) {
         float4* E = F.rec_ext + (size_t)r * 3;
         E[0] = make_float4(s.adx[4], s.ady[4], s.af0[4], s.adx[5]);
@@ -388,12 +436,14 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
     float4 v[3];
     float a[3][kWideSlots];
     Prim s;
-    bool survive = false, clipped = false;
+    bool set_up = false, survive = false, clipped = false;
     if (valid && fetch_triangle<NS>(F, D, tri, v, a)) {
         const uint32_t c0 = clip_codes(v[0]), c1 = clip_codes(v[1]), c2 = clip_codes(v[2]);
         const uint32_t mor = c0 | c1 | c2;
-        if (mor == 0) survive = setup_prim<NS>(s, v, a, D.nslots, V.width, V.height);
+        if (mor == 0) set_up = setup_prim<NS>(s, v, a, D.nslots, V.width, V.height);
         else clipped = (c0 & c1 & c2) == 0;
+        // a primitive drawing nothing still counts as binned by the reference
+        survive = set_up && !covers_nothing(s);
     }
     if (valid) {
         if (survive) write_record<NS>(F, g, s, d);
@@ -402,8 +452,8 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
     const uint32_t q = wave_add_amount(&F.counters[C_CLIPQ], clipped ? 1u : 0u);
     if (clipped) F.clip_queue[q] = g;  // at most one slot per triangle: never overflows
 
-    if (stats) block_stats_add(&bs, survive ? reference_tile_entries(s) : 0u, survive ? 1u : 0u);
-    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, g, ht);
+    if (stats) block_stats_add(&bs, set_up ? reference_tile_entries(s) : 0u, set_up ? 1u : 0u);
+    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, g, ht, false);
     if (stats) {
         __syncthreads();
         block_stats_flush(F, &bs);
@@ -584,7 +634,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
                 if (lane == 0 && te) atomicAdd(&F.counters[C_TILE_ENTRIES], te);
                 if (lane == 0 && pr) atomicAdd(&F.counters[C_PRIMS], pr);
             }
-            walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, false, r, ht);
+            walk_bins(F, V, ok, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, r, ht, true);
         }
         __syncthreads();  // before the polygons are overwritten
     }
@@ -795,32 +845,25 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(FrameDev F, const
     if (i < n) F.queue[atomicAdd(&off[bkt[i]], 1u)] = in_queue[i];
 }
 
-__global__ __launch_bounds__(256) void k_emit(FrameDev F)
+// Bin lists from the pairs of the count walks: entry (bin start + the part's
+// offset in the bin + the pair's slot) = record. An overflowed frame (pairs or
+// entries past the arena) is rendered again with larger arenas.
+// grid: (blocks per region, pair_regions + kClipPairRegions)
+__global__ __launch_bounds__(256) void k_scatter(FrameDev F)
 {
-    __shared__ uint32_t s_hash[256 / 64 * kWaveScratch];
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
-    uint32_t* ht = s_hash + (threadIdx.x >> 6) * kWaveScratch;
-    wave_hash_init(ht);
-    const uint32_t nrec = F.ntris + F.counters[C_CLIPREC];
+    const uint32_t region = blockIdx.y;
+    const uint32_t n = F.pair_count[region * kPairStride];  // within the region's capacity without overflow
+    const uint32_t first_p = region < F.pair_regions
+                                 ? region * F.pair_region_cap
+                                 : F.pair_regions * F.pair_region_cap + (region - F.pair_regions) * F.clip_pair_cap;
+    const uint32_t np = F.pair_total;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base_r = blockIdx.x * blockDim.x; base_r < nrec; base_r += stride) {
-        const uint32_t r = base_r + threadIdx.x;
-        uint32_t d = 0xFFFFFFFFu;
-        if (r < nrec) d = F.rec_meta[r];
-        const bool active = d != 0xFFFFFFFFu;
-        int32_t c[3] = { 0, 0, 0 }, dx[3] = { 0, 0, 0 }, dy[3] = { 0, 0, 0 };
-        int32_t xmin = 0, xmax = 0, ymin = 0, ymax = 0;
-        uint32_t view = 0;
-        if (active) {
-            const uint4* R = F.rec_raster + (size_t)r * 4;
-            const uint4 q0 = R[0], q1 = R[1], q2 = R[2];
-            c[0] = (int32_t)q0.x; c[1] = (int32_t)q0.y; c[2] = (int32_t)q0.z;
-            dx[0] = (int32_t)q0.w; dx[1] = (int32_t)q1.x; dx[2] = (int32_t)q1.y;
-            dy[0] = (int32_t)q1.z; dy[1] = (int32_t)q1.w; dy[2] = (int32_t)q2.x;
-            xmin = q2.y & 0x3FFF; xmax = (q2.y >> 16) & 0x3FFF; ymin = q2.z & 0x3FFF; ymax = (q2.z >> 16) & 0x3FFF;
-            view = F.draws[d].view;
-        }
-        walk_bins(F, F.views[view], active, c, dx, dy, xmin, xmax, ymin, ymax, true, r, ht);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t p = first_p + i;
+        const uint32_t rec = F.pairs[p], pkey = F.pairs[np + p], slot = F.pairs[2u * np + p];
+        const uint32_t key = pkey - part_of(rec) * F.total_bins;
+        F.entries[F.bin_start[key] + F.part_off[pkey] + slot] = rec;
     }
 }
 

@@ -56,6 +56,41 @@ struct MlpLds {
     float b3;
 };
 
+// The model's output, the predicted log2 cost, for features x (one wave, the
+// result in every lane): hidden unit j = lane & 31 (the upper half repeats
+// it); layer 2 reads unit i of layer 1 from lane i.
+__device__ __forceinline__ float mlp_infer_wave(const MlpLds& P, const float x[kFeat], uint32_t lane)
+{
+    const uint32_t j = lane & 31u;
+    float a = P.b1[j];
+#pragma unroll
+    for (uint32_t k = 0; k < kFeat; ++k) a += P.w1[j][k] * x[k];
+    const float h1 = a > 0.0f ? a : 0.0f;
+    float b = P.b2[j];
+#pragma unroll
+    for (uint32_t i = 0; i < kHid; ++i) b += P.w2t[i][j] * __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, h1), (int)i));
+    const float h2 = b > 0.0f ? b : 0.0f;
+    float o = lane < kHid ? P.w3[j] * h2 : 0.0f;
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) o += __shfl_xor(o, m, 64);
+    return P.b3 + o;
+}
+
+// The parameters into LDS (layer 2 transposed); the whole workgroup takes
+// part, a barrier follows.
+__device__ __forceinline__ void load_mlp_lds(MlpLds& P, const MlpParams* params)
+{
+    for (uint32_t i = threadIdx.x; i < kHid * kFeat; i += blockDim.x) (&P.w1[0][0])[i] = (&params->w1[0][0])[i];
+    for (uint32_t i = threadIdx.x; i < kHid * kHid; i += blockDim.x) P.w2t[i % kHid][i / kHid] = (&params->w2[0][0])[i];
+    if (threadIdx.x < kHid) {
+        P.b1[threadIdx.x] = params->b1[threadIdx.x];
+        P.b2[threadIdx.x] = params->b2[threadIdx.x];
+        P.w3[threadIdx.x] = params->w3[threadIdx.x];
+    }
+    if (threadIdx.x == 0) P.b3 = params->b3;
+    __syncthreads();
+}
+
 // Features of bin gb and, once trained, its predicted cost (one wave).
 __device__ __forceinline__ void bin_predict_one(const FrameDev& F, const MlpLds& P, bool trained, uint32_t gb,
                                                 uint32_t lane, float4* feat, float* pred)
@@ -93,7 +128,8 @@ __device__ __forceinline__ void bin_predict_one(const FrameDev& F, const MlpLds&
             zext = z1 - z0;
         }
         // mip level from the uv planes at the first vertex (slots 0 and 1)
-        const float4 s0 = F.rec_shade[(size_t)rec * 4], s1 = F.rec_shade[(size_t)rec * 4 + 1];
+        const float* Sp = F.rec_shade + (size_t)rec * F.shade_stride;
+        const float4 s0 = make_float4(Sp[0], Sp[1], Sp[2], Sp[3]), s1 = make_float4(Sp[4], Sp[5], Sp[6], Sp[7]);
         const DrawDev& D = F.draws[F.rec_meta[rec]];
         const uint32_t ti = D.tex_info;
         if (ti && D.deriv_slot == 0) {
@@ -116,21 +152,8 @@ __device__ __forceinline__ void bin_predict_one(const FrameDev& F, const MlpLds&
                              __log2f(1.0f + scale * overlap), textured > 0.0f ? mip / textured : 0.0f };
     if (lane == 0) feat[gb] = make_float4(x[0], x[1], x[2], x[3]);
     if (!trained) return;
-    // hidden unit j = lane & 31 (the upper half repeats it); layer 2 reads
-    // unit i of layer 1 from lane i
-    const uint32_t j = lane & 31u;
-    float a = P.b1[j];
-#pragma unroll
-    for (uint32_t k = 0; k < kFeat; ++k) a += P.w1[j][k] * x[k];
-    const float h1 = a > 0.0f ? a : 0.0f;
-    float b = P.b2[j];
-#pragma unroll
-    for (uint32_t i = 0; i < kHid; ++i) b += P.w2t[i][j] * __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, h1), (int)i));
-    const float h2 = b > 0.0f ? b : 0.0f;
-    float o = lane < kHid ? P.w3[j] * h2 : 0.0f;
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1) o += __shfl_xor(o, m, 64);
-    if (lane == 0) pred[gb] = cnt ? exp2f(fminf(P.b3 + o, 120.0f)) : 0.0f;
+    const float o = mlp_infer_wave(P, x, lane);
+    if (lane == 0) pred[gb] = cnt ? exp2f(fminf(o, 120.0f)) : 0.0f;
 }
 
 __device__ __forceinline__ float mlp_forward(const MlpParams& P, const float x[kFeat], float h1[kHid], float h2[kHid])
@@ -164,20 +187,25 @@ __global__ __launch_bounds__(256) void k_bin_predict(FrameDev F, float4* feat, c
     // lists (k_emit skipped it); it is rendered again with larger arenas.
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
     if (F.sched_gate && !F.sched_gate[1]) return;  // a frame between recordings keeps the last prediction
-    if (params) {
-        for (uint32_t i = threadIdx.x; i < kHid * kFeat; i += 256) (&P.w1[0][0])[i] = (&params->w1[0][0])[i];
-        for (uint32_t i = threadIdx.x; i < kHid * kHid; i += 256) P.w2t[i % kHid][i / kHid] = (&params->w2[0][0])[i];
-        if (threadIdx.x < kHid) {
-            P.b1[threadIdx.x] = params->b1[threadIdx.x];
-            P.b2[threadIdx.x] = params->b2[threadIdx.x];
-            P.w3[threadIdx.x] = params->w3[threadIdx.x];
-        }
-        if (threadIdx.x == 0) P.b3 = params->b3;
-        __syncthreads();
-    }
+    if (params) load_mlp_lds(P, params);
     const uint32_t lane = threadIdx.x & 63u;
     for (uint32_t gb = blockIdx.x * 4u + (threadIdx.x >> 6); gb < F.total_bins; gb += gridDim.x * 4u)
         bin_predict_one(F, P, params != nullptr, gb, lane, feat, pred);
+}
+
+// The model's output for n given feature rows (srg_sched_eval): the inference
+// k_bin_predict runs, one wave per row.
+__global__ __launch_bounds__(256) void k_mlp_eval(const MlpParams* params, const float4* x, uint32_t n, float* out)
+{
+    __shared__ MlpLds P;
+    load_mlp_lds(P, params);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t r = blockIdx.x * 4u + (threadIdx.x >> 6); r < n; r += gridDim.x * 4u) {
+        const float4 f = x[r];
+        const float xx[kFeat] = { f.x, f.y, f.z, f.w };
+        const float o = mlp_infer_wave(P, xx, lane);
+        if (lane == 0) out[r] = o;
+    }
 }
 
 // After the back end: append (features, log2 cost) of every non-empty bin to

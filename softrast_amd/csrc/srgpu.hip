@@ -6,7 +6,7 @@
 //   k_setup       one lane per submitted triangle (sr_frontend.h)
 //   k_clip        slow path for triangles crossing a clip plane
 //   k_scan_*      bin offsets and the back end's work queue (three passes)
-//   k_emit        bin lists
+//   k_scatter     bin lists from the (record, bin) pairs the count walks found
 //   k_backend<S>  persistent raster + resolve + shade per S x S bin (sr_backend.h)
 //
 // Visibility is resolved per pixel as the max of (depth bits << 32 |
@@ -34,22 +34,27 @@
 
 namespace srg {
 
-// Frame-start fill: counters, per-(part, bin) counts and cursors, per-bin
-// costs (graph-safe). With the learned order, gate[1] says whether this frame
+// Frame-start fill: counters, per-(part, bin) counts, per-bin costs
+// (graph-safe). With the learned order, gate[1] says whether this frame
 // predicts and records (every `every`-th frame, counted in gate[0] on the
 // device so that replayed graphs count too).
-__global__ void k_frame_init(uint32_t* counters, uint32_t* part_count, uint32_t* part_cursor, uint64_t* bin_cost,
-                             uint32_t nbins, uint32_t* gate, uint32_t every)
+// Counter words of a frame: the C_NCOUNTERS frame counters (copied back to
+// the host), then from word 64 the counters of the pair regions (at most
+// kPairRegions for k_setup, kClipPairRegions more for k_clip).
+constexpr uint32_t kPairCounterBase = 64,
+                   kCounterWords = kPairCounterBase + (kPairRegions + kClipPairRegions) * kPairStride;
+
+__global__ void k_frame_init(uint32_t* counters, uint32_t* part_count, uint64_t* bin_cost, uint32_t nbins,
+                             uint32_t* gate, uint32_t every)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (gate && i == 0) {
         gate[1] = (every <= 1u || gate[0] % every == 0u) ? 1u : 0u;
         gate[0] = gate[0] + 1u;
     }
-    if (i < C_NCOUNTERS) counters[i] = 0;
+    for (uint32_t t = i; t < kCounterWords; t += gridDim.x * blockDim.x) counters[t] = 0;
     for (uint32_t t = i; t < nbins * kBinParts; t += gridDim.x * blockDim.x) {
         part_count[t] = 0;
-        part_cursor[t] = 0;
         if (t < nbins) bin_cost[t] = 0;
     }
 }
@@ -207,11 +212,11 @@ struct srg_ctx {
     size_t h_tables_cap = 0;
     // arenas
     DevArray<uint4> rec_raster;
-    DevArray<float4> rec_shade;
+    DevArray<float> rec_shade;
     DevArray<float4> rec_ext;         // attribute planes 4-7, frames with a SponzaShader draw
     SponzaDev sponza{};               // SponzaShader constants (srg_set_sponza_constants)
     DevArray<uint32_t> rec_meta, clip_queue, clip_base, clip_order;
-    DevArray<uint32_t> bin_count, bin_start, part_count, part_cursor, part_off, queue, queue2, entries, scan_blk;
+    DevArray<uint32_t> bin_count, bin_start, part_count, part_off, queue, queue2, entries, pairs, scan_blk;
     DevArray<uint16_t> order_bkt;   // per queue entry: its cost bucket
     DevArray<uint32_t> order_hist;  // per (queue block, bucket): count, then offset
     DevArray<uint64_t> bin_cost;
@@ -229,6 +234,7 @@ struct srg_ctx {
     uint32_t* sched_gate = nullptr;   // [0] frame count, [1] this frame predicts and records
     float sched_last_err = 0.0f;      // error of the last frame that recorded
     bool sched_force_record = true;   // the next frame predicts and records (the model just changed)
+    uint64_t sched_layout = 0;        // bin layout (views' bin ranges) of the last learned-order frame
     DevArray<float4> sched_feat;
     DevArray<float> sched_pred;
     bool sched_trained = false;
@@ -239,7 +245,7 @@ struct srg_ctx {
     uint64_t tune_sig = 0;                          // signature of the last frame
     std::vector<uint8_t> graph_sig;
 
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[5] = {};  // frame start, back-end start, frame end, blit, back-end end
     srg_frame_stats stats{};
     void* scratch_dev = nullptr;  // this device's `scratch` (srg_end_frame_noalloc)
 };
@@ -419,7 +425,7 @@ constexpr float kSchedFallbackErr = 1.0f;
 // Once trained, the model's features, prediction and cost recording run on
 // every kSchedEvery-th frame; the frames between order their queue by the
 // last prediction (bins keep their numbers across frames of the same views).
-constexpr uint32_t kSchedEvery = 4;
+constexpr uint32_t kSchedEvery = 16;
 
 // A new bin size: the recorded samples and the model describe other bins, so
 // the model starts again from its initial weights and an empty ring.
@@ -578,10 +584,10 @@ void tune_after_frame(srg_ctx* ctx, uint64_t sig, float ms)
 struct FrameArenas {
     uint8_t* tables = nullptr;
     uint4* rec_raster = nullptr;
-    float4* rec_shade = nullptr;
+    float* rec_shade = nullptr;
     float4* rec_ext = nullptr;
     uint32_t *rec_meta = nullptr, *clip_queue = nullptr, *clip_base = nullptr, *clip_order = nullptr;
-    uint32_t *bin_count = nullptr, *bin_start = nullptr, *part_count = nullptr, *part_cursor = nullptr;
+    uint32_t *bin_count = nullptr, *bin_start = nullptr, *part_count = nullptr, *pairs = nullptr;
     uint32_t *part_off = nullptr, *queue = nullptr, *queue2 = nullptr, *entries = nullptr, *scan_blk = nullptr;
     uint16_t* order_bkt = nullptr;
     uint32_t* order_hist = nullptr;
@@ -664,6 +670,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
     uint64_t total_tris = 0;
     uint32_t nblocks = 0;
     bool wide = false;  // a draw keeps more than kMaxSlots attribute planes
+    uint32_t max_slots = 0;
     for (size_t i = 0; i < ctx->draws.size(); ++i) {
         const srg_draw& s = ctx->draws[i];
         DrawDev& d = ddev[i];
@@ -686,6 +693,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         int st = shader_slots(s.shader, s.uv_offset, s.attribute_stride / 4u, d.slot_attr, d.nslots, d.deriv_slot);
         if (st) return st;
         wide = wide || d.nslots > kMaxSlots;
+        max_slots = std::max(max_slots, d.nslots);
         d.tex = kNoTex;
         if (shader_samples(s.shader) && s.texture) {
             auto it = tex_index.find(s.texture);
@@ -714,6 +722,11 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         total_tris += d.ntris;
     }
     if (total_tris >= (1ull << 29)) return SRG_ERR_INVALID;  // order key is (triangle << 3) | fan
+    // setup records keep two attribute planes (9 floats of shade record) when
+    // no draw of the frame reads more, four (15 floats) otherwise; planes 4-7
+    // of a frame with a SponzaShader draw go to rec_ext
+    const bool narrow = !wide && max_slots <= 2;
+    const uint32_t shade_stride = narrow ? 9u : 15u;
     const uint32_t NT = (uint32_t)total_tris;
 
     size_t off = 0;
@@ -743,11 +756,10 @@ int run_frame(srg_ctx* ctx, bool noalloc)
     const size_t n_hist = (size_t)((TB + kOrderBlock - 1) / kOrderBlock) * kOrderBuckets;
     const size_t n_scan = 2 * ((TB + kScanThreads - 1) / kScanThreads);
     if (noalloc) {
-        A.counters = C.take<uint32_t>(C_NCOUNTERS);
+        A.counters = C.take<uint32_t>(kCounterWords);
         A.bin_count = C.take<uint32_t>(TB);
         A.bin_start = C.take<uint32_t>(TB + 1);
         A.part_count = C.take<uint32_t>((size_t)kBinParts * TB);
-        A.part_cursor = C.take<uint32_t>((size_t)kBinParts * TB);
         A.part_off = C.take<uint32_t>((size_t)kBinParts * TB);
         A.queue = C.take<uint32_t>(TB);
         A.queue2 = C.take<uint32_t>(TB);
@@ -761,18 +773,16 @@ int run_frame(srg_ctx* ctx, bool noalloc)
     } else {
         if ((st = ctx->bin_count.ensure(TB)) || (st = ctx->bin_start.ensure(TB + 1)) ||
             (st = ctx->part_count.ensure((size_t)kBinParts * TB)) ||
-            (st = ctx->part_cursor.ensure((size_t)kBinParts * TB)) ||
             (st = ctx->part_off.ensure((size_t)kBinParts * TB)) || (st = ctx->queue.ensure(TB)) ||
             (st = ctx->queue2.ensure(TB)) || (st = ctx->order_bkt.ensure(TB)) || (st = ctx->order_hist.ensure(n_hist)) ||
             (st = ctx->scan_blk.ensure(n_scan)) || (st = ctx->bin_cost.ensure(TB)) ||
-            (st = ctx->counters.ensure(C_NCOUNTERS)) || (st = ctx->clip_queue.ensure(NQ)) ||
+            (st = ctx->counters.ensure(kCounterWords)) || (st = ctx->clip_queue.ensure(NQ)) ||
             (st = ctx->clip_base.ensure(NQ)))
             return st;
         A.counters = ctx->counters.ptr;
         A.bin_count = ctx->bin_count.ptr;
         A.bin_start = ctx->bin_start.ptr;
         A.part_count = ctx->part_count.ptr;
-        A.part_cursor = ctx->part_cursor.ptr;
         A.part_off = ctx->part_off.ptr;
         A.queue = ctx->queue.ptr;
         A.queue2 = ctx->queue2.ptr;
@@ -800,17 +810,24 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             // records first; the entry lists take what the scratch has left
             Carver R = C;
             A.rec_raster = R.take<uint4>(nrec * 4);
-            A.rec_shade = R.take<float4>(nrec * 4);
+            A.rec_shade = R.take<float>(nrec * shade_stride);
             A.rec_meta = R.take<uint32_t>(nrec);
             A.clip_order = R.take<uint32_t>(want_clip);
             A.rec_ext = wide ? R.take<float4>(nrec * 3) : nullptr;
-            want_entries = std::min<uint64_t>(R.left() / sizeof(uint32_t), 0xFFFFFFF0ull);
-            if (!R.ok || want_entries < 4096) return SRG_ERR_OVERFLOW;
+            // each entry takes 4 bytes of list and 18 of (record, key, slot)
+            // pairs (setup regions plus k_clip's half-size region), 22 in
+            // all; 64 entries' worth is left for the alignment of the two
+            const size_t left = R.left();
+            if (!R.ok || left < (4096 + 64) * 22) return SRG_ERR_OVERFLOW;
+            want_entries = std::min<uint64_t>(left / 22 - 64, 0xFFFFFFF0ull);
             A.entries = R.take<uint32_t>(want_entries);
+            A.pairs = R.take<uint32_t>(3 * (want_entries + want_entries / 2));
+            if (!R.ok) return SRG_ERR_OVERFLOW;
         } else {
-            if ((st = ctx->rec_raster.ensure(nrec * 4)) || (st = ctx->rec_shade.ensure(nrec * 4)) ||
+            if ((st = ctx->rec_raster.ensure(nrec * 4)) || (st = ctx->rec_shade.ensure(nrec * shade_stride)) ||
                 (st = ctx->rec_meta.ensure(nrec)) || (st = ctx->clip_order.ensure(want_clip)) ||
-                (st = ctx->entries.ensure(want_entries)) || (wide && (st = ctx->rec_ext.ensure(nrec * 3))))
+                (st = ctx->entries.ensure(want_entries)) || (st = ctx->pairs.ensure(3 * (want_entries + want_entries / 2))) ||
+                (wide && (st = ctx->rec_ext.ensure(nrec * 3))))
                 return st;
             ctx->clip_cap = (uint32_t)want_clip;
             ctx->entry_cap = (uint32_t)want_entries;
@@ -819,6 +836,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             A.rec_meta = ctx->rec_meta.ptr;
             A.clip_order = ctx->clip_order.ptr;
             A.entries = ctx->entries.ptr;
+            A.pairs = ctx->pairs.ptr;
             A.rec_ext = wide ? ctx->rec_ext.ptr : nullptr;
         }
 
@@ -837,6 +855,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         F.entry_cap = (uint32_t)want_entries;
         F.rec_raster = A.rec_raster;
         F.rec_shade = A.rec_shade;
+        F.shade_stride = shade_stride;
         F.rec_ext = A.rec_ext;
         F.sponza = wide ? (const SponzaDev*)(A.tables + o_sponza) : nullptr;
         F.rec_meta = A.rec_meta;
@@ -846,9 +865,14 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         F.bin_count = A.bin_count;
         F.bin_start = A.bin_start;
         F.part_count = A.part_count;
-        F.part_cursor = A.part_cursor;
         F.part_off = A.part_off;
         F.entries = A.entries;
+        F.pairs = A.pairs;
+        F.pair_count = A.counters + kPairCounterBase;
+        F.pair_regions = std::max<uint32_t>(1, std::min<uint32_t>(nblocks, kPairRegions));
+        F.pair_region_cap = (uint32_t)(want_entries / F.pair_regions);
+        F.clip_pair_cap = (uint32_t)(want_entries / 2 / kClipPairRegions);
+        F.pair_total = F.pair_regions * F.pair_region_cap + kClipPairRegions * F.clip_pair_cap;
         F.queue = A.queue;
         F.bin_cost = A.bin_cost;
         F.counters = A.counters;
@@ -856,7 +880,8 @@ int run_frame(srg_ctx* ctx, bool noalloc)
                   ((ctx->cfg.flags & SRG_CFG_KEEP_PRIMID) ? FF_PRIMID : 0u) |
                   ((ctx->cfg.flags & SRG_CFG_TILE_COSTS) ? FF_TILE_COSTS : 0u);
 
-        const uint32_t emit_blocks = std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((nrec + 255) / 256, 16384));
+        const uint32_t emit_blocks =
+            std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((F.pair_region_cap + 255) / 256, 64));
         const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + kClipThreads - 1) / kClipThreads, 2048));
         const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((kBinParts * TB + 255) / 256, 1024));
         const uint32_t scan_blocks = (TB + kScanThreads - 1) / kScanThreads;
@@ -882,6 +907,16 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         // that asked for tile costs gets them every frame
         const uint32_t sched_every = (!ctx->sched_trained || (ctx->cfg.flags & SRG_CFG_TILE_COSTS)) ? 1u : kSchedEvery;
         F.sched_gate = learned ? ctx->sched_gate : nullptr;
+        if (learned) {
+            // predictions are per bin: a frame whose bins are laid out
+            // differently from the last one's (other views, sizes or bin
+            // size) predicts afresh, and so does a frame rendered again after
+            // an overflow (its first attempt advanced the frame count)
+            uint64_t lay = mix64(vdev.size(), K);
+            for (const ViewDev& v : vdev) lay = mix64(mix64(mix64(lay, v.bin_base), v.bins_x), v.bins_y);
+            if (lay != ctx->sched_layout || attempt > 0) ctx->sched_force_record = true;
+            ctx->sched_layout = lay;
+        }
         if (learned && ctx->sched_force_record) {
             hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, ctx->stream, ctx->sched_gate, 0u, (size_t)1);
             HIP_TRY(hipGetLastError());
@@ -890,7 +925,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         auto enqueue = [&](bool split_events) -> int {
             HIP_TRY(hipMemcpyAsync(A.tables, ctx->h_tables, off, hipMemcpyHostToDevice, ctx->stream));
             hipLaunchKernelGGL(k_frame_init, dim3(init_blocks), dim3(256), 0, ctx->stream, A.counters,
-                               A.part_count, A.part_cursor, A.bin_cost, TB,
+                               A.part_count, A.bin_cost, TB,
                                learned ? ctx->sched_gate : nullptr, sched_every);
             if (ctx->cfg.flags & SRG_CFG_KEEP_PRIMID)
                 for (auto& v : vdev)
@@ -900,6 +935,9 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             if (wide) {
                 if (nblocks) hipLaunchKernelGGL(k_setup<kWideSlots>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
                 if (NT) hipLaunchKernelGGL(k_clip<kWideSlots>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
+            } else if (narrow) {
+                if (nblocks) hipLaunchKernelGGL(k_setup<2>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
+                if (NT) hipLaunchKernelGGL(k_clip<2>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
             } else {
                 if (nblocks) hipLaunchKernelGGL(k_setup<kMaxSlots>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
                 if (NT) hipLaunchKernelGGL(k_clip<kMaxSlots>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
@@ -910,7 +948,8 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs, A.scan_blk, scan_blocks);
             hipLaunchKernelGGL(k_scan_apply, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs,
                                (const uint32_t*)A.scan_blk);
-            hipLaunchKernelGGL(k_emit, dim3(emit_blocks), dim3(256), 0, ctx->stream, F);
+            hipLaunchKernelGGL(k_scatter, dim3(emit_blocks, F.pair_regions + kClipPairRegions), dim3(256), 0,
+                               ctx->stream, F);
             if (learned) {
                 hipLaunchKernelGGL(k_bin_predict, dim3(std::min<uint32_t>((TB + 3) / 4, 16384u)),
                                    dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr,
@@ -929,6 +968,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             }
             if (split_events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
             if (const int lst = launch_backend(ctx, L, F)) return lst;
+            if (split_events) HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream));
             if (learned)
                 hipLaunchKernelGGL(k_mlp_record, dim3((TB + 255) / 256), dim3(256), 0, ctx->stream, F,
                                    (const float4*)ctx->sched_feat.ptr,
@@ -993,7 +1033,9 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             if (flags & FLAG_CLIP_OVERFLOW)
                 want_clip = std::max<uint64_t>(want_clip * 2, (uint64_t)ctx->h_counters[C_CLIPREC] + 64);
             if (flags & FLAG_ENTRY_OVERFLOW)
-                want_entries = std::max<uint64_t>(want_entries * 2, (uint64_t)ctx->h_counters[C_ENTRIES] * 5 / 4);
+                want_entries = std::max({ want_entries * 2, (uint64_t)ctx->h_counters[C_ENTRIES] * 5 / 4,
+                                          (uint64_t)ctx->h_counters[C_PAIR_NEED] * F.pair_regions * 5 / 4,
+                                          (uint64_t)ctx->h_counters[C_CLIP_PAIR_NEED] * 2 * kClipPairRegions * 5 / 4 });
             ctx->stats.retries++;
             continue;
         }
@@ -1008,7 +1050,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         if (timing) {
             if (split) {
                 (void)hipEventElapsedTime(&ctx->stats.ms_frontend, ctx->ev[0], ctx->ev[1]);
-                (void)hipEventElapsedTime(&ctx->stats.ms_backend, ctx->ev[1], ctx->ev[2]);
+                (void)hipEventElapsedTime(&ctx->stats.ms_backend, ctx->ev[1], ctx->ev[4]);
             }
             (void)hipEventElapsedTime(&ctx->stats.ms_total, ctx->ev[0], ctx->ev[2]);
         }
@@ -1151,7 +1193,7 @@ void srg_destroy(srg_ctx* ctx)
     ctx->bin_count.release();
     ctx->bin_start.release();
     ctx->part_count.release();
-    ctx->part_cursor.release();
+    ctx->pairs.release();
     ctx->part_off.release();
     ctx->queue.release();
     ctx->queue2.release();
@@ -1572,6 +1614,53 @@ int srg_sched_set_model(srg_ctx* ctx, const float* params, uint32_t count)
     int st = ensure_sched(ctx);
     if (st) return st;
     HIP_TRY(hipMemcpyAsync(ctx->sched_params, params, sizeof(MlpParams), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->sched_trained = true;
+    ctx->sched_fallback = false;
+    ctx->sched_force_record = true;
+    return SRG_OK;
+}
+
+int srg_sched_eval(srg_ctx* ctx, const float* features, uint32_t n, float* log2_cost)
+{
+    if (!ctx || (n && (!features || !log2_cost))) return SRG_ERR_INVALID;
+    int st = ensure_sched(ctx);
+    if (st || !n) return st;
+    DevArray<float4> x;
+    DevArray<float> out;
+    if ((st = x.ensure(n)) || (st = out.ensure(n))) return st;
+    int rc = SRG_OK;
+    if (hipMemcpyAsync(x.ptr, features, sizeof(float4) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        rc = SRG_ERR_HIP;
+    if (!rc) {
+        hipLaunchKernelGGL(k_mlp_eval, dim3(std::min<uint32_t>((n + 3) / 4, 4096u)), dim3(256), 0, ctx->stream,
+                           (const MlpParams*)ctx->sched_params, (const float4*)x.ptr, n, out.ptr);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(log2_cost, out.ptr, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = SRG_ERR_HIP;
+    }
+    (void)hipStreamSynchronize(ctx->stream);
+    x.release();
+    out.release();
+    return rc;
+}
+
+int srg_sched_train(srg_ctx* ctx, const float* samples, uint32_t n, uint32_t steps, float lr, uint32_t seed)
+{
+    if (!ctx || !samples || n == 0 || n > kSchedRing) return SRG_ERR_INVALID;
+    int st = ensure_sched(ctx);
+    if (st) return st;
+    static_assert(sizeof(SchedSample) == 5 * sizeof(float), "sample rows of 5 floats");
+    static_assert(sizeof(AdamState) % sizeof(uint32_t) == 0, "Adam state in words");
+    HIP_TRY(hipMemcpyAsync(ctx->sched_ring, samples, sizeof(SchedSample) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->sched_head, &n, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_fill_u32, dim3(16), dim3(256), 0, ctx->stream, (uint32_t*)ctx->sched_adam, 0u,
+                       sizeof(AdamState) / sizeof(uint32_t));
+    hipLaunchKernelGGL(k_mlp_train, dim3(1), dim3(kTrainBatch), 0, ctx->stream, (MlpParams*)ctx->sched_params,
+                       (AdamState*)ctx->sched_adam, (const SchedSample*)ctx->sched_ring,
+                       (const uint32_t*)ctx->sched_head, kSchedRing, steps, lr, seed);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->sched_trained = true;
     ctx->sched_fallback = false;

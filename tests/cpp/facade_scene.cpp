@@ -3,7 +3,10 @@
 // include/sr/Renderer.h. Draws a small quad grid textured with a checker
 // (UnlitDiffuseShader, then SponzaShader with constants set the way
 // SponzaScene::Init / Update set them) and prints the number of non-clear
-// pixels after the last blit.
+// pixels after the last blit. With a path argument it also writes its inputs
+// and both blitted frames there, as records of (u32 byte count, bytes):
+// width and height, vertices, u16 indices, the checker's RGBA8 texels, the
+// MVP, the Sponza constants, frame 0's rows, frame 1's rows.
 // Exit codes: 0 = rendered (or no GPU: prints "no-device"), 1 = failure.
 #include <cstdio>
 #include <vector>
@@ -12,8 +15,19 @@
 
 struct Vertex { float pos[3]; float norm[3]; float uv[2]; };
 
-int main()
+static void put(std::FILE* f, const void* p, size_t n)
 {
+    const uint32_t k = (uint32_t)n;
+    std::fwrite(&k, 4, 1, f);
+    std::fwrite(p, 1, n, f);
+}
+
+int main(int argc, char** argv)
+{
+    // made before any context exists: bound by the first context that uses
+    // it, and destroyed after that context shut down
+    sr::FrameBuffer early(100, 70);
+    if (early.m_ctx) return 1;
     sr::RenderContext renderCtx;
     if (renderCtx.Status() == SRG_ERR_NO_DEVICE) {
         std::printf("no-device\n");
@@ -21,8 +35,23 @@ int main()
     }
     if (renderCtx.Status()) return 1;
     const uint32_t W = 320, H = 200;
-    sr::FrameBuffer framebuffer(renderCtx, W, H);
-    if (framebuffer.m_status) return 1;
+    // as Viewer/Main.cpp:41-42: the framebuffer is made after the context, without naming it
+    sr::FrameBuffer framebuffer(W, H);
+    if (framebuffer.m_status || framebuffer.m_ctx != &renderCtx) return 1;
+    // per-thread host scratch, as the reference's tasks take it
+    sr::ThreadScratchAllocator& scratch = renderCtx.ThreadAllocator();
+    float* tmp = (float*)scratch.Alloc(4096 * sizeof(float), 64);
+    if (!tmp || ((uintptr_t)tmp & 63u)) return 1;
+    tmp[4095] = 1.0f;
+    scratch.Reset();
+    if (renderCtx.BeginFrame() || renderCtx.ClearFrameBuffer(early, 0x5A) || renderCtx.EndFrame()) return 1;
+    if (early.m_ctx != &renderCtx || !early.m_fb) return 1;
+    {
+        std::vector<uint32_t> rows(100 * 70);
+        if (renderCtx.Blit(early, (uint8_t*)rows.data())) return 1;
+        for (uint32_t p : rows)
+            if (p != 0x5A5A5A5Au) return 1;  // ClearFrameBuffer's low-byte memset
+    }
 
     std::vector<uint8_t> checker(64 * 64 * 4);
     for (uint32_t y = 0; y < 64; ++y)
@@ -68,7 +97,7 @@ int main()
     }
     if (int st = renderCtx.SetSponzaConstants(sponza)) { std::printf("SetSponzaConstants: %s\n", srg_status_string(st)); return 1; }
 
-    std::vector<uint32_t> pixels(W * H);
+    std::vector<uint32_t> pixels(W * H), frames[2];
     for (int frame = 0; frame < 2; ++frame) {
         renderCtx.BeginFrame();
         renderCtx.ClearFrameBuffer(framebuffer, 0);
@@ -87,6 +116,21 @@ int main()
         if (int st = renderCtx.DrawIndexed(call)) { std::printf("DrawIndexed: %s\n", srg_status_string(st)); return 1; }
         if (int st = renderCtx.EndFrame()) { std::printf("EndFrame: %s\n", srg_status_string(st)); return 1; }
         if (int st = renderCtx.Blit(framebuffer, (uint8_t*)pixels.data())) { std::printf("Blit: %s\n", srg_status_string(st)); return 1; }
+        frames[frame] = pixels;
+    }
+    if (argc > 1) {
+        std::FILE* f = std::fopen(argv[1], "wb");
+        if (!f) return 1;
+        const uint32_t wh[2] = { W, H };
+        put(f, wh, sizeof(wh));
+        put(f, verts.data(), verts.size() * sizeof(Vertex));
+        put(f, indices.data(), indices.size() * sizeof(uint16_t));
+        put(f, checker.data(), checker.size());
+        put(f, m, sizeof(m));
+        put(f, &sponza, sizeof(sponza));
+        put(f, frames[0].data(), frames[0].size() * 4);
+        put(f, frames[1].data(), frames[1].size() * 4);
+        std::fclose(f);
     }
     uint32_t drawn = 0;
     for (uint32_t p : pixels) drawn += p != 0;

@@ -177,6 +177,38 @@ def oracle_render_views(viewset, texdata, nthreads=8, clear=0, sponza=None):
     return planes, stats
 
 
+def oracle_render_views_each(viewset, texdata, nthreads=16, clear=0, fast=False):
+    """The oracle's image of every view of a scenes.ViewSet as one EndFrame
+    draws them, rendered one view per oracle frame so that memory holds one
+    view's bins at a time. A view rendered alone numbers its triangles from
+    0; its primitive ids are shifted to the batch's numbering, (index of the
+    view's first triangle among all views' draws) << 3, the order key's
+    triangle field. Returns (planes, summed stats)."""
+    from oracle.oracle import Oracle, Plane, make_texture
+    o = Oracle(nthreads=nthreads, fast=fast)
+    stex = [make_texture(t) for t in texdata]
+    planes, total, base = [], {}, 0
+    for v in viewset.views:
+        p = Plane(v.width, v.height)
+        o.clear(p, clear)
+        draws = []
+        for d in v.draws:
+            vf = d.vertices.view(np.float32).reshape(-1, 8)
+            draws.append(dict(shader=d.shader, tex=stex[d.texture][0] if d.texture >= 0 else None, idx=d.indices,
+                              idx_stride=d.indices.dtype.itemsize, pos=vf, pos_stride=ATTR_STRIDE, pos_num=len(vf),
+                              attr=vf, attr_stride=ATTR_STRIDE, attr_num=len(vf), uv_offset=d.uv_offset, plane=0,
+                              mvp=v.mvp))
+        st = o.render(draws, [p])
+        covered = p.primid != 0xFFFFFFFF
+        p.primid[covered] += np.uint32(base << 3)
+        base += sum(len(d.indices) // 3 for d in v.draws)
+        for k, val in st.items():
+            total[k] = total.get(k, 0) + val
+        planes.append(p)
+    o.close()
+    return planes, total
+
+
 def gpu_render_views(viewset, texdata, ctx, fbs, clear=0, sponza=None):
     """Every view of a scenes.ViewSet in one EndFrame, view k into fbs[k];
     `texdata` holds TextureData or device textures by ViewSet texture index."""

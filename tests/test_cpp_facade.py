@@ -1,8 +1,10 @@
 """The C++ facade (include/sr/Renderer.h) compiles against the C ABI and
-drives the library with reference-style host code (tests/cpp/facade_scene.cpp)."""
+drives the library with reference-style host code (tests/cpp/facade_scene.cpp);
+on a GPU its blitted frames are compared with the oracle's."""
 import pathlib
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
@@ -24,11 +26,46 @@ def test_facade_compiles_and_fails_cleanly_without_gpu(tmp_path):
     assert res.stdout.strip() == "no-device" or res.stdout.startswith("drawn=")
 
 
+def _records(path):
+    data = path.read_bytes()
+    out, o = [], 0
+    while o < len(data):
+        n = int(np.frombuffer(data, np.uint32, 1, o)[0])
+        out.append(data[o + 4:o + 4 + n])
+        o += 4 + n
+    return out
+
+
 @pytest.mark.gpu
 def test_facade_renders_on_gpu(tmp_path):
+    """The C++ program's two blitted frames (UnlitDiffuse, then SponzaShader)
+    equal the oracle's render of the inputs it wrote, blitted the way
+    BlitJobFn de-tiles (Renderer.cpp:319-347): every RGBA8 pixel identical."""
+    import softrast_amd as sr
+    from oracle.oracle import Oracle, Plane, make_texture
     exe = _build(tmp_path)
+    dump = tmp_path / "facade.bin"
     print("[facade] running reference-style C++ host program", flush=True)
-    res = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    res = subprocess.run([str(exe), str(dump)], capture_output=True, text=True, timeout=120)
     print(res.stdout, flush=True)
     assert res.returncode == 0, res.stdout + res.stderr
     assert res.stdout.startswith("drawn=")
+    wh, verts, idx, checker, mvp, sponza, f0, f1 = _records(dump)
+    w, h = (int(x) for x in np.frombuffer(wh, np.uint32))
+    vf = np.frombuffer(verts, np.float32).reshape(-1, 8)
+    idx = np.frombuffer(idx, np.uint16)
+    tex = sr.TextureData().CreateFromRGBA8(np.frombuffer(checker, np.uint8), 64, 64, True)
+    o = Oracle(nthreads=4)
+    o.set_sponza_constants(sponza)
+    stex = make_texture(tex)
+    for shader, rows in ((0, f0), (3, f1)):
+        plane = Plane(w, h)
+        o.clear(plane, 0)
+        o.render([dict(shader=shader, tex=stex[0], idx=idx, idx_stride=2, pos=vf, pos_stride=32, pos_num=len(vf),
+                       attr=vf, attr_stride=32, attr_num=len(vf), uv_offset=6, plane=0,
+                       mvp=np.frombuffer(mvp, np.float32))], [plane])
+        want = o.blit(plane)
+        got = np.frombuffer(rows, np.uint32).reshape(h, w)
+        print(f"[facade] shader {shader}: {int(np.count_nonzero(got != want))} pixels differ", flush=True)
+        assert np.array_equal(got, want)
+    o.close()

@@ -203,3 +203,28 @@ def test_released_handles_fail_the_frame():
     for f in (fb, keep):
         f.destroy()
     ctx.Shutdown()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_tiny_lattice_triangles(seed):
+    """20,000 triangles of up to a few px^2 with vertices on a half-pixel
+    lattice (identity MVP, 256x256, so edges pass exactly through pixel
+    centres and edge values hit 0): many survivors cover one pixel or none.
+    The front end's no-coverage skip and the back end's exact-zero handling
+    give the oracle's image and rasterised-fragment count."""
+    from softrast_amd import scenes
+    rng = np.random.default_rng(300 + seed)
+    n, W = 20000, 256
+    centre = rng.integers(16, 2 * W - 16, (n, 1, 2)) * 0.5
+    pts = centre + rng.integers(-3, 4, (n, 3, 2)) * 0.5
+    v = np.zeros((n * 3, 8), np.float32)
+    v[:, 0] = ((pts[..., 0] - 128.0) / 128.0).reshape(-1)
+    v[:, 1] = ((128.0 - pts[..., 1]) / 128.0).reshape(-1)
+    v[:, 2] = np.repeat(rng.uniform(0.1, 0.9, n), 3)
+    v[:, 3:6] = rng.normal(0, 1, (n * 3, 3))
+    v[:, 6:8] = rng.uniform(-2, 2, (n * 3, 2))
+    assert np.array_equal(v[:, 0] * 128.0 + 128.0, pts[..., 0].reshape(-1).astype(np.float32))
+    draw = scenes.Draw(v.view(scenes.VERTEX_DTYPE).reshape(-1), np.arange(n * 3, dtype=np.uint32), 0, 0, 6)
+    sc = scenes.Scene(W, W, np.eye(4, dtype=np.float32).reshape(-1), [draw],
+                      [rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)], {"config": "lattice"})
+    _check(sc, f"lattice_{seed}")

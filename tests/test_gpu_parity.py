@@ -156,7 +156,7 @@ def test_layouts_same_image(bin_size, wg_per_tile):
 def test_layouts_clustered_learned(bin_size, wg_per_tile):
     """Strips and one-wave bins under the learned order: the image is the
     oracle's, and on every frame that records (each frame until trained, the
-    frame after training, then every fourth) every non-empty bin has a
+    frame after training, then one in sixteen) every non-empty bin has a
     recorded cost (summed over its strips)."""
     import softrast_amd as sr
     from softrast_amd import scenes
@@ -178,7 +178,7 @@ def test_layouts_clustered_learned(bin_size, wg_per_tile):
             assert np.all(costs[counts > 0] > 0), frame
             recorded.append(frame)
     assert st["frags_raster"] == rstats["frags_raster"] and st["sched_trained"]
-    # frames 0 (untrained) and 1 (right after training) record, then one in four
+    # frames 0 (untrained) and 1 (right after training) record, then one in sixteen
     assert recorded[:2] == [0, 1] and 2 <= len(recorded) <= 4, recorded
     ctx.Shutdown()
 
@@ -584,4 +584,35 @@ def test_draw_list_follows_write_plane():
         ctx.Blit(fb)
         assert fb.m_writePlane == plane ^ 1
     fb.destroy()
+    ctx.Shutdown()
+
+
+def test_learned_order_view_count_changes():
+    """A trained learned-order context whose frames change the number and
+    sizes of views (so every bin's number changes): the frame after each
+    change predicts afresh, and every view of every frame is the oracle's
+    image."""
+    import softrast_amd as sr
+    from softrast_amd import scenes
+    pool = [helpers.random_scene(700 + k, 2500, w, h, clip_heavy=(k % 2 == 1), tex_size=(64, 64))
+            for k, (w, h) in enumerate(((320, 200), (256, 256), (400, 180), (200, 300), (333, 222)))]
+    ctx = sr.RenderContext(flags=sr.CFG_KEEP_PRIMID | sr.CFG_STATS, tile_order=sr.ORDER_LEARNED,
+                           sched_warmup_frames=2)
+    fbs = {}
+    for frame, pick in enumerate(((0, 1, 2), (0, 1, 2), (0, 1, 2), (3, 4, 0, 1, 2), (2, 4), (3, 4, 0, 1, 2), (1,))):
+        vs = scenes.ViewSet([pool[k] for k in pick], pool[0].textures)
+        for v in vs.views:
+            for d in v.draws:
+                d.texture = 0
+        tex = _viewset_textures(vs)
+        views = [fbs.setdefault(k, sr.FrameBuffer(ctx, pool[k].width, pool[k].height)) for k in pick]
+        _say(f"[parity] case=learned_view_changes frame={frame} views={pick}")
+        st = helpers.gpu_render_views(vs, tex, ctx, views)
+        planes, rstats = helpers.oracle_render_views(vs, tex)
+        _compare_views(vs, planes, views, frame)
+        assert st["frags_raster"] == rstats["frags_raster"]
+        if frame >= 3:
+            assert st["sched_trained"]
+    for fb in fbs.values():
+        fb.destroy()
     ctx.Shutdown()
