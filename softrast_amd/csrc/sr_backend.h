@@ -6,14 +6,16 @@
 // or four workgroups per bin, sharing its entry list). A 64-bit visibility
 // key per pixel lives in LDS. Waves
 // stride over the bin's entries 64 at a time, each lane loading one
-// primitive record so the loads overlap; the entries' 8x8 blocks then form a
-// queue that groups of 8 lanes take one block at a time, a lane per pixel
-// column (RasterizeTrisInBin_OutputFragments and ComputeBlockMask8x8,
-// Rasterizer.cpp:97-304). Then each lane resolves its pixels (the key names
+// primitive record so the loads overlap; the entries' column pairs of 8x8
+// block rows then form a queue the wave takes 64 at a time, a lane per pair
+// walking its eight rows (RasterizeTrisInBin_OutputFragments and
+// ComputeBlockMask8x8, Rasterizer.cpp:97-304). Then each lane resolves its pixels (the key names
 // the fragment the reference would have shaded last) and shades them
 // (ComputeInterpolantsDrawCallImpl :306-419, Shaders.h, Texture.cpp),
 // writing RGBA8 and depth once.
 #pragma once
+#include <type_traits>
+
 #include "sr_frontend.h"
 
 namespace srg {
@@ -236,6 +238,22 @@ __device__ __forceinline__ uint32_t view_of_bin(const FrameDev& F, uint32_t gb)
     return lo;
 }
 
+// v, computed here into a VGPR. An empty asm statement that takes and
+// returns the register: the compiler can neither recompute v next to each
+// use nor defer the wait for the loads v depends on into the conditional
+// blocks that use it.
+__device__ __forceinline__ uint32_t pin(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long pin(unsigned long long v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
@@ -243,22 +261,25 @@ __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane)
 
 // 64-wide block reject / full-accept test of RasterizeTrisInBin_OutputFragments
 // (Rasterizer.cpp:230-272): corners {xb, xb+64} x {yb, yb+64} of the block,
-// tile-relative. Returns 0 = reject, 1 = test pixels, 2 = every pixel covered.
+// tile-relative, each edge value in wrapping int32 as the reference computes
+// it. The block is rejected when some edge has no corner > 0 (the largest of
+// its four corner values is <= 0) and fully covered when every corner of
+// every edge is > 0 (the smallest of all twelve is > 0).
+// Returns 0 = reject, 1 = test pixels, 2 = every pixel covered.
 __device__ __forceinline__ uint32_t block_class(const uint32_t ct[3], const uint32_t dxu[3], const uint32_t dyu[3],
                                                 uint32_t xb, uint32_t yb)
 {
-    const uint32_t Xa = xb, Xb = xb + kBin, Ya = yb, Yb = yb + kBin;
-    bool reject = false, full = true;
+    int32_t rej = INT32_MAX, full = INT32_MAX;  // min over edges of the largest corner; min over all corners
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const uint32_t m = ((int32_t)(ct[k] + dyu[k] * Xa + dxu[k] * Ya) > 0 ? 1u : 0u) |
-                           ((int32_t)(ct[k] + dyu[k] * Xa + dxu[k] * Yb) > 0 ? 2u : 0u) |
-                           ((int32_t)(ct[k] + dyu[k] * Xb + dxu[k] * Ya) > 0 ? 4u : 0u) |
-                           ((int32_t)(ct[k] + dyu[k] * Xb + dxu[k] * Yb) > 0 ? 8u : 0u);
-        reject = reject || m == 0;
-        full = full && m == 0xFu;
+        const uint32_t a = ct[k] + dyu[k] * xb + dxu[k] * yb;  // (xb, yb)
+        const uint32_t b = a + dxu[k] * kBin;                  // (xb, yb + 64)
+        const uint32_t c = a + dyu[k] * kBin;                  // (xb + 64, yb)
+        const uint32_t d = c + dxu[k] * kBin;                  // (xb + 64, yb + 64)
+        rej = min(rej, max(max((int32_t)a, (int32_t)b), max((int32_t)c, (int32_t)d)));
+        full = min(full, min(min((int32_t)a, (int32_t)b), min((int32_t)c, (int32_t)d)));
     }
-    return reject ? 0u : (full ? 2u : 1u);
+    return rej <= 0 ? 0u : (full > 0 ? 2u : 1u);
 }
 
 // LDS slot of the key of strip-local pixel (x, y): rows of S keys, and inside
@@ -414,74 +435,92 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         }
         const uint32_t pc = o[9 * 64], pd = o[10 * 64];
         const float zdx = ffrom(o[11 * 64]), zdy = ffrom(o[12 * 64]), zc0 = ffrom(o[13 * 64]);
-        const unsigned long long lw = (unsigned long long)o[14 * 64];
+        const unsigned long long lw = (unsigned long long)pin(o[14 * 64]);
         const uint32_t k = it - (m >> 7);
         const uint32_t cx0 = pc & 127u, ccw = (pc >> 7) & 127u, cy0 = (pc >> 14) & 127u, cy1 = (pc >> 21) & 127u;
         const bool tight = ((pc >> 28) & 1u) != 0;
         const uint32_t snp = pd >> 17;
-        const uint32_t brow = (k * (pd & 0x1FFFFu)) >> 16;
+        const uint32_t brow = __umul24(k, pd & 0x1FFFFu) >> 16;  // k < 512, magic < 2^17
         const uint32_t x = 2u * ((cx0 >> 1) + (k - brow * snp)), yb = (cy0 & ~7u) + 8u * brow;
         const uint32_t xb = x & ~7u, l = x & 7u;
         // the pair's columns inside the owner's candidate rectangle (checked, not assumed)
         const bool in0 = x - cx0 < ccw, in1 = x + 1u - cx0 < ccw;
         const bool inside = valid && yb < cy1 && (in0 || in1);
-        const uint32_t r_lo = inside && cy0 > yb ? cy0 - yb : 0u;
-        uint32_t r_hi = inside ? min(cy1 - yb, 8u) : 0u;
         int32_t thr = 1;  // tight: all three edge values > 0; an exact 0 is decided below
+        bool walk_rows = inside;
         if (inside && !tight) {
             const uint32_t cls = block_class(ct, dxu, dyu, xb, yb);
             thr = cls == 2u ? INT32_MIN : 0;
-            if (cls == 0u) r_hi = r_lo;  // rejected block
+            walk_rows = cls != 0u;  // a rejected block draws nothing
         }
         // key_slot of (x - sx0, yb + r - sy0): rows of S keys, the in-group
-        // rotation alternating with the row's parity (yb is a multiple of 8);
-        // x is even, so x + 1 is in the same group of G keys
+        // rotation alternating with the row's parity (yb is a multiple of 8),
+        // so each column has one slot base for even rows and one for odd
+        // rows, and row r adds r * S keys (a constant offset). x is even, so
+        // x + 1 is in the same group of G keys.
         constexpr uint32_t G = S < 16u ? S : 16u;
         const uint32_t sxl = x - sx0, ybl = yb - sy0;
         const uint32_t t = sxl + 2u * (ybl >> 3) + (sxl >> 4);
         const uint32_t grp = ybl * S + (sxl & ~(G - 1u));
-        const bool t0 = tight & in0, t1 = tight & in1;
-        // One walk down the block row. The first covers the pixels whose
-        // minimum edge value reaches thr and reports whether a tight pixel has
-        // a minimum of exactly 0; the second, taken only where the corner test
-        // accepts the block, covers those pixels.
-        auto walk = [&](bool second) -> bool {
-            float z0 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
-            float z1 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)(l + 1u), zdx, zc0)) + (float)xb * zdx;
-            for (uint32_t r = 0; r < r_lo; ++r) {
-                z0 = z0 + zdy;
-                z1 = z1 + zdy;
-            }
-            uint32_t E[3];
+        // byte offsets into keys of the four slot bases
+        const uint32_t b0e = pin(8u * (grp + (t & (G - 1u)))), b1e = pin(8u * (grp + ((t + 1u) & (G - 1u))));
+        const uint32_t b0o = pin(8u * (grp + ((t + 8u) & (G - 1u)))), b1o = pin(8u * (grp + ((t + 9u) & (G - 1u))));
+        char* const kb = reinterpret_cast<char*>(keys);
+        // One walk down all eight rows of the block row. Rows and columns
+        // outside cy0..cy1 x cx0..cx1 are walked too: for a primitive without
+        // a tight bound the rectangle is whole blocks, so there are none, and
+        // for a tight one every pixel of the walked blocks outside its bound
+        // has a negative edge value (the bound holds every pixel whose three
+        // values are >= 0), so they cover nothing and have no exact zero.
+        // The first walk covers the pixels whose minimum edge value reaches
+        // thr and returns the smallest minimum as unsigned (0 iff some pixel
+        // has a minimum of exactly 0); the second, taken only where the corner
+        // test accepts the block, covers those pixels.
+        auto walk = [&](auto second_walk) -> uint32_t {
+            constexpr bool second = decltype(second_walk)::value;
+            const float z0 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)l, zdx, zc0)) + (float)xb * zdx;
+            const float z1 = __builtin_fmaf((float)yb, zdy, __builtin_fmaf((float)(l + 1u), zdx, zc0)) + (float)xb * zdx;
+            // the keys of the pair's pixels in the current row: depth bits
+            // above the order word, the depth advanced in place
+            unsigned long long q0 = pin(((unsigned long long)fbits(z0) << 32) | lw);
+            unsigned long long q1 = pin(((unsigned long long)fbits(z1) << 32) | lw);
+            // edge values of the pair's two pixels in the current row
+            uint32_t E[3], F[3];
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * (yb + r_lo);
-            bool zero = false;
-            for (uint32_t r = r_lo; r < r_hi; ++r) {
-                const uint32_t tr = t + ((r & 1u) << 3), ro = grp + r * S;
+            for (int kk = 0; kk < 3; ++kk) {
+                E[kk] = ct[kk] + dyu[kk] * x + dxu[kk] * yb;
+                F[kk] = E[kk] + dyu[kk];
+            }
+            uint32_t zmin = 0xFFFFFFFFu;
+#pragma unroll
+            for (uint32_t r = 0; r < 8u; ++r) {
                 const int32_t m0 = min(min((int32_t)E[0], (int32_t)E[1]), (int32_t)E[2]);
-                const int32_t m1 =
-                    min(min((int32_t)(E[0] + dyu[0]), (int32_t)(E[1] + dyu[1])), (int32_t)(E[2] + dyu[2]));
-                const bool e0 = t0 & (m0 == 0), e1 = t1 & (m1 == 0);
-                const bool cov0 = second ? e0 : in0 & (m0 >= thr);
-                const bool cov1 = second ? e1 : in1 & (m1 >= thr);
-                zero = zero | e0 | e1;
+                const int32_t m1 = min(min((int32_t)F[0], (int32_t)F[1]), (int32_t)F[2]);
+                const bool cov0 = second ? m0 == 0 : m0 >= thr;
+                const bool cov1 = second ? m1 == 0 : m1 >= thr;
+                if (!second) zmin = min(zmin, min((uint32_t)m0, (uint32_t)m1));
                 if (STATS) frags += (cov0 ? 1u : 0u) + (cov1 ? 1u : 0u);
-                if (cov0 & (z0 > 0.0f))
-                    atomicMax(&keys[ro + (tr & (G - 1u))], ((unsigned long long)fbits(z0) << 32) | lw);
-                if (cov1 & (z1 > 0.0f))
-                    atomicMax(&keys[ro + ((tr + 1u) & (G - 1u))], ((unsigned long long)fbits(z1) << 32) | lw);
-                z0 = z0 + zdy;
-                z1 = z1 + zdy;
+                const uint32_t ro = 8u * S * r;
+                auto* const k0 = reinterpret_cast<unsigned long long*>(kb + ((r & 1u) ? b0o : b0e) + ro);
+                auto* const k1 = reinterpret_cast<unsigned long long*>(kb + ((r & 1u) ? b1o : b1e) + ro);
+                if (cov0 & (ffrom((uint32_t)(q0 >> 32)) > 0.0f)) atomicMax(k0, q0);
+                if (cov1 & (ffrom((uint32_t)(q1 >> 32)) > 0.0f)) atomicMax(k1, q1);
+                q0 = pin(((unsigned long long)fbits(ffrom((uint32_t)(q0 >> 32)) + zdy) << 32) | (uint32_t)q0);
+                q1 = pin(((unsigned long long)fbits(ffrom((uint32_t)(q1 >> 32)) + zdy) << 32) | (uint32_t)q1);
 #pragma unroll
-                for (int kk = 0; kk < 3; ++kk) E[kk] += dxu[kk];
+                for (int kk = 0; kk < 3; ++kk) {
+                    E[kk] += dxu[kk];
+                    F[kk] += dxu[kk];
+                }
             }
-            return zero;
+            return zmin;
         };
-        const bool zero = walk(false);
+        bool zero = false;
+        if (walk_rows) zero = walk(std::false_type{}) == 0u && tight;
         // an edge exactly through a pixel of a tight primitive: the corner
         // test decides it (rare; a wave-uniform branch)
         if (__ballot(zero))
-            if (zero && block_class(ct, dxu, dyu, xb, yb) != 0u) (void)walk(true);
+            if (zero && block_class(ct, dxu, dyu, xb, yb) != 0u) (void)walk(std::true_type{});
     }
 }
 

@@ -1032,10 +1032,23 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             if (noalloc && (flags & FLAG_ENTRY_OVERFLOW)) return SRG_ERR_OVERFLOW;
             if (flags & FLAG_CLIP_OVERFLOW)
                 want_clip = std::max<uint64_t>(want_clip * 2, (uint64_t)ctx->h_counters[C_CLIPREC] + 64);
-            if (flags & FLAG_ENTRY_OVERFLOW)
+            if (flags & FLAG_ENTRY_OVERFLOW) {
+                // The clip fans' pairs reach their regions in the order the
+                // clip queue was filled, which changes from frame to frame:
+                // size every clip region for all of them, so that the next
+                // frame's distribution fits too. The counters kept counting
+                // past the caps.
+                uint32_t cc[kClipPairRegions * kPairStride];
+                HIP_TRY(hipMemcpyAsync(cc, F.pair_count + F.pair_regions * kPairStride, sizeof(cc),
+                                       hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                uint64_t clip_pairs = 0;
+                for (uint32_t i = 0; i < kClipPairRegions; ++i) clip_pairs += cc[i * kPairStride];
                 want_entries = std::max({ want_entries * 2, (uint64_t)ctx->h_counters[C_ENTRIES] * 5 / 4,
                                           (uint64_t)ctx->h_counters[C_PAIR_NEED] * F.pair_regions * 5 / 4,
-                                          (uint64_t)ctx->h_counters[C_CLIP_PAIR_NEED] * 2 * kClipPairRegions * 5 / 4 });
+                                          (uint64_t)ctx->h_counters[C_CLIP_PAIR_NEED] * 2 * kClipPairRegions * 5 / 4,
+                                          clip_pairs * 2 * kClipPairRegions * 5 / 4 });
+            }
             ctx->stats.retries++;
             continue;
         }
