@@ -569,7 +569,7 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
 
         for (uint32_t p = tid; p < NPIX; p += T) {
             const uint32_t x = sx0 + p % S, y = sy0 + p / S;
-            keys[key_slot<S>(p % S, p / S)] = clear_d ? 0ull : init_key(V.depth[tofs + y * kBin + x]);
+            keys[key_slot<S>(p % S, p / S)] = clear_d ? 0ull : init_key(global_ptr(V.depth)[tofs + y * kBin + x]);
         }
         __syncthreads();
 
@@ -613,13 +613,15 @@ __global__ __launch_bounds__(backend_threads(S, H)) void k_backend(FrameDev F)
             for (uint32_t i = 0; i < BATCH; ++i) {
                 const uint32_t p = tid + (p0 + i) * T;
                 const size_t o = tofs + (sy0 + p / S) * kBin + sx0 + p % S;
+                const auto colour = global_ptr(V.colour);
+                const auto depth = global_ptr(V.depth);
                 if (win[i]) {
-                    if (V.flags & VF_HAS_COLOUR) V.colour[o] = col[i];
-                    V.depth[o] = ffrom((uint32_t)(kk[i] >> 32));
-                    if (V.primid) V.primid[o] = 0xFFFFFFFEu - (uint32_t)kk[i];
+                    if (V.flags & VF_HAS_COLOUR) colour[o] = col[i];
+                    depth[o] = ffrom((uint32_t)(kk[i] >> 32));
+                    if (V.primid) global_ptr(V.primid)[o] = 0xFFFFFFFEu - (uint32_t)kk[i];
                 } else {
-                    if (clear_c) V.colour[o] = col[i];
-                    if (clear_d) V.depth[o] = 0.0f;
+                    if (clear_c) colour[o] = col[i];
+                    if (clear_d) depth[o] = 0.0f;
                 }
             }
         }

@@ -120,6 +120,16 @@ __device__ __forceinline__ uint32_t spread5(uint32_t v)
 }
 __device__ __forceinline__ uint32_t morton5(uint32_t x, uint32_t y) { return spread5(x) | (spread5(y) << 1); }
 
+// Device-memory view of a pointer read from a table: accesses through it
+// compile to global memory instructions. Through a generic pointer they are
+// flat instructions, which the wait counters track together with LDS
+// operations, so a wait for an LDS result also waits for them.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* global_ptr(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 struct TexDev {
     const uint8_t* texels;
     uint32_t mip_offsets[14];
@@ -168,7 +178,7 @@ __device__ __forceinline__ void sample_wrap(const TexDev& T, float u, float v, f
     const uint32_t x1 = (x0 + 1u) & (uint32_t)(mw - 1), y1 = (y0 + 1u) & (uint32_t)(mh - 1);
 
     const uint32_t tw = (uint32_t)(mw > 32 ? mw : 32) >> 5;
-    const uint32_t* base = (const uint32_t*)(T.texels + T.mip_offsets[mip]);
+    const auto base = global_ptr((const uint32_t*)(T.texels + T.mip_offsets[mip]));
     const uint32_t ty0 = (y0 >> 5) * tw, ty1 = (y1 >> 5) * tw;
     const uint32_t my0 = spread5(y0) << 1, my1 = spread5(y1) << 1;
     const uint32_t mx0 = spread5(x0), mx1 = spread5(x1);

@@ -63,6 +63,7 @@ enum : uint32_t {
     C_SCHED_N = 11,    // bins recorded by the scheduler this frame
     C_PAIR_NEED = 12,  // on a pair region overflow: the largest count a region reached
     C_CLIP_PAIR_NEED = 13, // on an overflow of k_clip's pair region: the count it reached
+    C_CLIP_PAIRS = 14, // pairs k_clip found, summed over its regions
     C_NCOUNTERS = 16,
 };
 

@@ -362,9 +362,9 @@ __device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev&
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const size_t i = (size_t)tri * 3u + (size_t)k;
-        idx[k] = D.idx_stride == 2 ? (uint32_t)((const uint16_t*)D.idx)[i]
-               : D.idx_stride == 4 ? ((const uint32_t*)D.idx)[i]
-                                   : (uint32_t)D.idx[i];
+        idx[k] = D.idx_stride == 2 ? (uint32_t)global_ptr((const uint16_t*)D.idx)[i]
+               : D.idx_stride == 4 ? global_ptr((const uint32_t*)D.idx)[i]
+                                   : (uint32_t)global_ptr(D.idx)[i];
     }
     const bool ok = idx[0] < D.pos_num && idx[1] < D.pos_num && idx[2] < D.pos_num && idx[0] < D.attr_num &&
                     idx[1] < D.attr_num && idx[2] < D.attr_num;
@@ -374,9 +374,9 @@ __device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev&
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float* p = (const float*)(D.pos + (size_t)idx[k] * D.pos_stride);
+        const auto p = global_ptr((const float*)(D.pos + (size_t)idx[k] * D.pos_stride));
         v[k] = transform(D.mvp, p[0], p[1], p[2]);
-        const float* at = (const float*)(D.attr + (size_t)idx[k] * D.attr_stride);
+        const auto at = global_ptr((const float*)(D.attr + (size_t)idx[k] * D.attr_stride));
 #pragma unroll
         for (uint32_t s = 0; s < NS; ++s) a[k][s] = s < D.nslots ? at[D.slot_attr[s]] : 0.0f;
     }
@@ -427,8 +427,11 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
         if (F.draws[mid].block_base <= blockIdx.x) d = mid;
         else hi = mid;
     }
-    const DrawDev& D = F.draws[d];
-    const ViewDev& V = F.views[D.view];
+    // copies, read before the kernel stores anything: the compiler then
+    // keeps them in scalar registers (scalar loads need memory that no
+    // store of the kernel may have changed)
+    const DrawDev D = F.draws[d];
+    const ViewDev V = F.views[D.view];
     const uint32_t tri = (blockIdx.x - D.block_base) * kSetupThreads + threadIdx.x;
     const bool valid = tri < D.ntris;
     const uint32_t g = D.tri_base + tri;
@@ -741,6 +744,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(FrameDev F, uint32_t*
         F.counters[C_ENTRIES] = tc;
         F.counters[C_QLEN] = tq;
         if (tc > F.entry_cap) atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
+        uint32_t cp = 0;
+        for (uint32_t i = 0; i < kClipPairRegions; ++i) cp += F.pair_count[(F.pair_regions + i) * kPairStride];
+        F.counters[C_CLIP_PAIRS] = cp;
     }
 }
 

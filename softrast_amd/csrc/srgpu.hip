@@ -44,6 +44,13 @@ namespace srg {
 constexpr uint32_t kPairCounterBase = 64,
                    kCounterWords = kPairCounterBase + (kPairRegions + kClipPairRegions) * kPairStride;
 
+// Entry-arena size at which each k_clip pair region (entries / 2 /
+// kClipPairRegions pairs) holds `clip_pairs` pairs with a quarter to spare.
+static uint64_t clip_pair_want(uint32_t clip_pairs)
+{
+    return (uint64_t)clip_pairs * 2 * kClipPairRegions * 5 / 4;
+}
+
 __global__ void k_frame_init(uint32_t* counters, uint32_t* part_count, uint64_t* bin_cost, uint32_t nbins,
                              uint32_t* gate, uint32_t every)
 {
@@ -1032,26 +1039,20 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             if (noalloc && (flags & FLAG_ENTRY_OVERFLOW)) return SRG_ERR_OVERFLOW;
             if (flags & FLAG_CLIP_OVERFLOW)
                 want_clip = std::max<uint64_t>(want_clip * 2, (uint64_t)ctx->h_counters[C_CLIPREC] + 64);
-            if (flags & FLAG_ENTRY_OVERFLOW) {
-                // The clip fans' pairs reach their regions in the order the
-                // clip queue was filled, which changes from frame to frame:
-                // size every clip region for all of them, so that the next
-                // frame's distribution fits too. The counters kept counting
-                // past the caps.
-                uint32_t cc[kClipPairRegions * kPairStride];
-                HIP_TRY(hipMemcpyAsync(cc, F.pair_count + F.pair_regions * kPairStride, sizeof(cc),
-                                       hipMemcpyDeviceToHost, ctx->stream));
-                HIP_TRY(hipStreamSynchronize(ctx->stream));
-                uint64_t clip_pairs = 0;
-                for (uint32_t i = 0; i < kClipPairRegions; ++i) clip_pairs += cc[i * kPairStride];
+            if (flags & FLAG_ENTRY_OVERFLOW)
                 want_entries = std::max({ want_entries * 2, (uint64_t)ctx->h_counters[C_ENTRIES] * 5 / 4,
                                           (uint64_t)ctx->h_counters[C_PAIR_NEED] * F.pair_regions * 5 / 4,
                                           (uint64_t)ctx->h_counters[C_CLIP_PAIR_NEED] * 2 * kClipPairRegions * 5 / 4,
-                                          clip_pairs * 2 * kClipPairRegions * 5 / 4 });
-            }
+                                          clip_pair_want(ctx->h_counters[C_CLIP_PAIRS]) });
             ctx->stats.retries++;
             continue;
         }
+        // The clip fans' pairs reach their regions in the order the clip
+        // queue was filled, which changes from frame to frame: when a region
+        // could not hold all of them, the next frame sizes every region so
+        // (the total is the same for the same triangles).
+        if (!noalloc && clip_pair_want(ctx->h_counters[C_CLIP_PAIRS]) > want_entries)
+            ctx->entry_cap = (uint32_t)std::min<uint64_t>(clip_pair_want(ctx->h_counters[C_CLIP_PAIRS]), 0xFFFFFFF0ull);
         ctx->stats.tris_in = total_tris;
         ctx->stats.prims = ctx->h_counters[C_PRIMS];
         ctx->stats.bin_entries = ctx->h_counters[C_TILE_ENTRIES];
