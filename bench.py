@@ -490,6 +490,8 @@ def sponza_workload(sr, args):
     fps_b, ms_b, st_b, bctx, bfbs = time_path(sr, scene, texs, args.views, bflags, args.tile_order, args.bin_size,
                                               max(20, args.steps // args.views), args.warmup, sr.SponzaShader, k)
     layout = {"bin_size": st_b[-1]["bin_size"], "wg_per_tile": st_b[-1]["wg_per_tile"]}
+    # views of the batched launch to check against the oracle
+    sampled = {v: bfbs[v].read_tiles() for v in sorted({0, (args.views - 1) // 2, args.views - 1})}
     for fb in bfbs:
         fb.destroy()
     bctx.Shutdown()

@@ -28,6 +28,10 @@ HIP_FLAGS = [
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # atomics on one address are aggregated over the wave by hand where it
+    # pays; the compiler's own rewrite adds a wave scan and a broadcast that
+    # waits for the atomic's return at once
+    "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
     "-Wno-unused-command-line-argument",
     f"-I{ROOT / 'include'}",
     f"-I{CSRC}",

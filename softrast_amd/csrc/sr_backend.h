@@ -238,22 +238,6 @@ __device__ __forceinline__ uint32_t view_of_bin(const FrameDev& F, uint32_t gb)
     return lo;
 }
 
-// v, computed here into a VGPR. An empty asm statement that takes and
-// returns the register: the compiler can neither recompute v next to each
-// use nor defer the wait for the loads v depends on into the conditional
-// blocks that use it.
-__device__ __forceinline__ uint32_t pin(uint32_t v)
-{
-    asm volatile("" : "+v"(v));
-    return v;
-}
-
-__device__ __forceinline__ unsigned long long pin(unsigned long long v)
-{
-    asm volatile("" : "+v"(v));
-    return v;
-}
-
 __device__ __forceinline__ uint32_t bcast(uint32_t v, uint32_t lane)
 {
     return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)lane);
@@ -393,12 +377,7 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     const uint32_t cw = e.cx1 > e.cx0 ? e.cx1 - e.cx0 : 0u;
     const uint32_t np = cw ? ((e.cx1 - 1u) >> 1) - (e.cx0 >> 1) + 1u : 0u;  // column pairs per block row
     const uint32_t n = has ? e.nby * np : 0u;
-    uint32_t incl = n;
-#pragma unroll
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
+    const uint32_t incl = wave_incl_sum(n);
     const uint32_t total = bcast(incl, 63);
     // per-entry values the column workers need, packed; the first block row
     // is cy0 rounded down to 8
@@ -462,9 +441,11 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         const uint32_t sxl = x - sx0, ybl = yb - sy0;
         const uint32_t t = sxl + 2u * (ybl >> 3) + (sxl >> 4);
         const uint32_t grp = ybl * S + (sxl & ~(G - 1u));
-        // byte offsets into keys of the four slot bases
+        // byte offsets into keys of the four slot bases; grp is a multiple
+        // of G, so the odd rows' rotation by 8 flips bit 3 of the slot (G =
+        // 16) or changes nothing (G = 8)
         const uint32_t b0e = pin(8u * (grp + (t & (G - 1u)))), b1e = pin(8u * (grp + ((t + 1u) & (G - 1u))));
-        const uint32_t b0o = pin(8u * (grp + ((t + 8u) & (G - 1u)))), b1o = pin(8u * (grp + ((t + 9u) & (G - 1u))));
+        const uint32_t b0o = pin(G == 16u ? b0e ^ 64u : b0e), b1o = pin(G == 16u ? b1e ^ 64u : b1e);
         char* const kb = reinterpret_cast<char*>(keys);
         // One walk down all eight rows of the block row. Rows and columns
         // outside cy0..cy1 x cx0..cx1 are walked too: for a primitive without

@@ -76,6 +76,7 @@ enum : uint32_t {
 enum : uint32_t { FF_STATS = 1u, FF_PRIMID = 2u, FF_TILE_COSTS = 4u };
 
 constexpr uint32_t kFanUnclipped = 7u;  // fan field of an unclipped triangle's order key
+constexpr uint32_t kNoPair = 0xFFFFFFFFu;  // record field of a pair slot left empty
 
 // Primitive records. Index space: [0, ntris) one slot per submitted triangle
 // (its unclipped primitive), [ntris, ntris + clip_cap) fan triangles of
@@ -87,6 +88,7 @@ struct FrameDev {
     const TexDev* texs;
     uint32_t ndraws, nviews, total_bins, ntris;
     uint32_t bin_k, bin_size, clip_cap, entry_cap;
+    uint32_t bin_log2, bin_klog2;  // log2 of bin_size and of bin_k (both powers of two)
     uint4* rec_raster;     // 4 x uint4 per record: edges, bbox, r0, z plane
     float* rec_shade;      // shade_stride floats per record: 1/w plane, then attribute planes 0-1 (stride 9)
                            // or 0-3 (stride 15), each plane (dx, dy, value at v0)
@@ -106,13 +108,15 @@ struct FrameDev {
     // (record, part-bin key, slot in the part's list) of every entry, as the
     // count walks find them: three arrays of pair_total words, one after
     // another; k_scatter writes entries from them after the scan. Region r <
-    // pair_regions (k_setup's workgroup index mod pair_regions) fills
+    // pair_regions (a power of two; k_setup's workgroup index mod
+    // pair_regions) fills
     // [r * pair_region_cap, (r + 1) * pair_region_cap) through its own
     // counter pair_count[r * kPairStride]; k_clip's workgroups fill the
     // kClipPairRegions regions after them (workgroup index mod
     // kClipPairRegions), clip_pair_cap words each. k_setup's regions
     // therefore get the same pairs every frame; k_clip takes its triangles in
-    // queue order, and its regions are sized generously.
+    // queue order, so after a frame whose clip regions could not each hold
+    // all of its clip pairs the host sizes them so (C_CLIP_PAIRS).
     uint32_t* pairs;
     uint32_t* pair_count;
     uint32_t pair_regions, pair_region_cap, clip_pair_cap, pair_total;
@@ -137,6 +141,22 @@ constexpr uint32_t kBinParts = 16;
 __host__ __device__ constexpr uint32_t part_of(uint32_t rec) { return (rec >> 6) & (kBinParts - 1u); }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// v, computed here into a VGPR. An empty asm statement that takes and
+// returns the register: the compiler can neither recompute v next to each
+// use nor defer the wait for the loads v depends on into the conditional
+// blocks that use it.
+__device__ __forceinline__ uint32_t pin(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long pin(unsigned long long v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
 
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane)
 {
@@ -228,6 +248,18 @@ __device__ __forceinline__ uint32_t wave_incl_max(uint32_t v)
     return v;
 }
 
+// Inclusive sum over the wave's lanes, the same DPP pattern.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
+
 // Owner of work item `base + lane` when every lane owns `n` consecutive
 // items starting at `first` (an exclusive prefix of the n): 1 + the owning
 // lane, or 0 past the last item. Lanes whose items start in this pass of 64
@@ -299,17 +331,11 @@ __device__ __forceinline__ float wave_max_f(float v)
 // exclusive offset. Uniform control flow required.
 __device__ __forceinline__ uint32_t wave_add_amount(uint32_t* counter, uint32_t amount)
 {
-    uint32_t x = amount;
-    const uint32_t l = lane_id();
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (l >= (uint32_t)o) x += y;
-    }
-    const uint32_t total = __shfl(x, 63, 64);
+    const uint32_t x = wave_incl_sum(amount);
+    const uint32_t total = readlane_u32(x, 63);
     uint32_t base = 0;
-    if (l == 0 && total) base = atomicAdd(counter, total);
-    return __shfl(base, 0, 64) + x - amount;
+    if (lane_id() == 0 && total) base = atomicAdd(counter, total);
+    return readlane_u32(base, 0) + x - amount;
 }
 
 }  // namespace srg

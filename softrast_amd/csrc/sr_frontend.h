@@ -205,7 +205,7 @@ struct BinWalk {
     bool check;
 };
 
-__device__ __forceinline__ BinWalk bin_walk(const ViewDev& V, uint32_t bin_k, uint32_t bin_size, int32_t xmin,
+__device__ __forceinline__ BinWalk bin_walk(const ViewDev& V, uint32_t bin_k, uint32_t bin_log2, int32_t xmin,
                                             int32_t xmax, int32_t ymin, int32_t ymax)
 {
     BinWalk w;
@@ -218,10 +218,10 @@ __device__ __forceinline__ BinWalk bin_walk(const ViewDev& V, uint32_t bin_k, ui
         w.sbx0 = w.sby0 = w.nx = w.ncand = 0;
         return w;
     }
-    w.sbx0 = xlo / bin_size;
-    w.sby0 = ylo / bin_size;
-    w.nx = ((uint32_t)xmax - 1u) / bin_size - w.sbx0 + 1u;
-    const uint32_t ny = ((uint32_t)ymax - 1u) / bin_size - w.sby0 + 1u;
+    w.sbx0 = xlo >> bin_log2;
+    w.sby0 = ylo >> bin_log2;
+    w.nx = (((uint32_t)xmax - 1u) >> bin_log2) - w.sbx0 + 1u;
+    const uint32_t ny = (((uint32_t)ymax - 1u) >> bin_log2) - w.sby0 + 1u;
     w.ncand = w.nx * ny;
     return w;
 }
@@ -255,25 +255,30 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
 {
     BinWalk w;
     if (active) {
-        w = bin_walk(V, F.bin_k, F.bin_size, xmin, xmax, ymin, ymax);
+        w = bin_walk(V, F.bin_k, F.bin_log2, xmin, xmax, ymin, ymax);
     } else {
         w.sbx0 = w.sby0 = w.nx = w.ncand = 0;
         w.view_bins_x = w.bin_base = 0;
         w.check = false;
     }
     const uint32_t lane = lane_id();
-    uint32_t incl = w.ncand;
-#pragma unroll
-    for (uint32_t o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
+    const uint32_t incl = wave_incl_sum(w.ncand);
     const uint32_t total = readlane_u32(incl, 63);
     const uint32_t first = incl - w.ncand;
     const uint32_t pk_org = w.sbx0 | (w.sby0 << 16);                  // bin coordinates < 2^16
     const uint32_t pk_nx = max(w.nx, 1u) | ((w.check ? 1u : 0u) << 31);  // nx < 2^31
     const bool any_check = __ballot(w.check && w.ncand != 0) != 0ull;
-    const uint32_t K = F.bin_k;
+    // The wave's pairs take one reservation in its region, issued before the
+    // passes so that its round trip overlaps their work: candidate `it` goes
+    // to slot pbase + it, and one the corner test rejects leaves a hole
+    // (record kNoPair). pair_regions is a power of two.
+    const uint32_t region =
+        clip ? F.pair_regions + blockIdx.x % kClipPairRegions : blockIdx.x & (F.pair_regions - 1u);
+    const uint32_t cap = clip ? F.clip_pair_cap : F.pair_region_cap;
+    const uint32_t first_p = clip ? F.pair_regions * F.pair_region_cap + (region - F.pair_regions) * F.clip_pair_cap
+                                  : region * F.pair_region_cap;
+    uint32_t pres = 0;
+    if (lane == 0 && total) pres = atomicAdd(&F.pair_count[region * kPairStride], total);
     uint32_t carry = 0;
     for (uint32_t base = 0; base < total; base += 64) {
         const uint32_t it = base + lane;
@@ -285,7 +290,19 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
         const uint32_t bb = __shfl(w.bin_base, src, 64), vbx = __shfl(w.view_bins_x, src, 64);
         const uint32_t orec = __shfl(rec, src, 64);
         const uint32_t nx = pn & 0x7FFFFFFFu;
-        const uint32_t sbx = (org & 0xFFFFu) + i % nx, sby = (org >> 16) + i / nx;
+        // i / nx and i % nx (i < nx * ny <= 2^22, so i / nx < 2^11): the
+        // quotient from the fp32 reciprocal is within 2^-11 of i / nx, so its
+        // truncation is off by at most one, which one correction step fixes
+        uint32_t q = (uint32_t)((float)i * __builtin_amdgcn_rcpf((float)nx));
+        int32_t r = (int32_t)(i - __umul24(q, nx));
+        if (r < 0) {
+            q -= 1u;
+            r += (int32_t)nx;
+        } else if (r >= (int32_t)nx) {
+            q += 1u;
+            r -= (int32_t)nx;
+        }
+        const uint32_t sbx = (org & 0xFFFFu) + (uint32_t)r, sby = (org >> 16) + q;
         bool act = valid;
         if (any_check) {  // the reference's corner test for primitives spanning 3+ bin rows
             int32_t oc[3], odx[3], ody[3];
@@ -295,26 +312,28 @@ __device__ __forceinline__ void walk_bins(const FrameDev& F, const ViewDev& V, b
                 odx[k] = __shfl(dx[k], src, 64);
                 ody[k] = __shfl(dy[k], src, 64);
             }
-            if (act && (pn >> 31) && !tile_accepts(oc, odx, ody, (sbx / K) * kBin, (sby / K) * kBin)) act = false;
+            if (act && (pn >> 31) &&
+                !tile_accepts(oc, odx, ody, (sbx >> F.bin_klog2) * kBin, (sby >> F.bin_klog2) * kBin))
+                act = false;
         }
         const uint32_t key = act ? bb + sby * vbx + sbx : 0u;
         const uint32_t pkey = part_of(orec) * F.total_bins + key;  // the (part, bin) counter
         const uint32_t slot = wave_add_slot_lds(ht, F.part_count, pkey, act);
-        const uint32_t region = clip ? F.pair_regions + blockIdx.x % kClipPairRegions : blockIdx.x % F.pair_regions;
-        const uint32_t cap = clip ? F.clip_pair_cap : F.pair_region_cap;
-        const uint32_t first_p = clip ? F.pair_regions * F.pair_region_cap + (region - F.pair_regions) * F.clip_pair_cap
-                                      : region * F.pair_region_cap;
-        const uint32_t ri = wave_add_amount(&F.pair_count[region * kPairStride], act ? 1u : 0u);
-        if (act) {
-            if (ri < cap) {
-                const uint32_t p = first_p + ri, np = F.pair_total;
-                F.pairs[p] = orec;
+        // the reservation, read here (pin keeps the read, and so its wait,
+        // inside the pass instead of right after the atomic)
+        const uint32_t pbase = readlane_u32(pin(pres), 0);
+        const uint32_t ri = pbase + it;
+        if (valid && ri < cap) {
+            const uint32_t p = first_p + ri, np = F.pair_total;
+            F.pairs[p] = act ? orec : kNoPair;
+            if (act) {
                 F.pairs[np + p] = pkey;
                 F.pairs[2u * np + p] = slot;
-            } else {
-                atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
-                atomicMax(&F.counters[clip ? C_CLIP_PAIR_NEED : C_PAIR_NEED], ri + 1u);
             }
+        }
+        if (base == 0 && lane == 0 && pbase + total > cap) {
+            atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
+            atomicMax(&F.counters[clip ? C_CLIP_PAIR_NEED : C_PAIR_NEED], pbase + total);
         }
     }
 }
@@ -588,12 +607,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
         // the wave's fan triangles, one per lane: lane j takes fan item t of
         // the lane whose inclusive fan count first exceeds t
         const uint32_t lane = lane_id();
-        uint32_t incl = fits ? nfan : 0u;
-#pragma unroll
-        for (uint32_t o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(incl, o, 64);
-            if (lane >= o) incl += y;
-        }
+        const uint32_t incl = wave_incl_sum(fits ? nfan : 0u);
         const uint32_t total = readlane_u32(incl, 63);
         const uint32_t first = incl - (fits ? nfan : 0u);
         for (uint32_t t0 = 0; t0 < total; t0 += 64) {
@@ -867,7 +881,9 @@ __global__ __launch_bounds__(256) void k_scatter(FrameDev F)
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const uint32_t p = first_p + i;
-        const uint32_t rec = F.pairs[p], pkey = F.pairs[np + p], slot = F.pairs[2u * np + p];
+        const uint32_t rec = F.pairs[p];
+        if (rec == kNoPair) continue;  // a candidate bin the corner test rejected
+        const uint32_t pkey = F.pairs[np + p], slot = F.pairs[2u * np + p];
         const uint32_t key = pkey - part_of(rec) * F.total_bins;
         F.entries[F.bin_start[key] + F.part_off[pkey] + slot] = rec;
     }

@@ -858,6 +858,8 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         F.ntris = NT;
         F.bin_k = K;
         F.bin_size = kBin / K;
+        F.bin_log2 = (uint32_t)__builtin_ctz(F.bin_size);
+        F.bin_klog2 = (uint32_t)__builtin_ctz(K);
         F.clip_cap = (uint32_t)want_clip;
         F.entry_cap = (uint32_t)want_entries;
         F.rec_raster = A.rec_raster;
@@ -876,7 +878,8 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         F.entries = A.entries;
         F.pairs = A.pairs;
         F.pair_count = A.counters + kPairCounterBase;
-        F.pair_regions = std::max<uint32_t>(1, std::min<uint32_t>(nblocks, kPairRegions));
+        // a power of two, so that k_setup finds its region with a mask
+        F.pair_regions = 1u << (31 - __builtin_clz(std::max<uint32_t>(1, std::min<uint32_t>(nblocks, kPairRegions))));
         F.pair_region_cap = (uint32_t)(want_entries / F.pair_regions);
         F.clip_pair_cap = (uint32_t)(want_entries / 2 / kClipPairRegions);
         F.pair_total = F.pair_regions * F.pair_region_cap + kClipPairRegions * F.clip_pair_cap;
