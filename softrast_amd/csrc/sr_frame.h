@@ -15,7 +15,7 @@ struct DrawDev {
     uint32_t pos_num, attr_num, tri_base, view;
     uint32_t shader, tex, nslots, deriv_slot;  // deriv_slot: first slot of (u, v) at uv_offset
     uint32_t slot_attr[kWideSlots];
-    uint32_t block_base;  // first k_setup workgroup of the draw (kSetupThreads triangles each)
+    uint32_t pad_b;
     uint32_t tex_info;    // the texture's max(wlog2, hlog2) | nmips << 8 (0 without one): for scheduling features
     uint32_t pad[2];
     float mvp[16];
@@ -82,10 +82,26 @@ constexpr uint32_t kNoPair = 0xFFFFFFFFu;  // record field of a pair slot left e
 // (its unclipped primitive), [ntris, ntris + clip_cap) fan triangles of
 // clipped triangles. Order key of a primitive: (global triangle << 3) | fan,
 // fan = 7 for an unclipped triangle, 0..6 for fan triangles.
+// Draws that share their geometry -- index, position and attribute buffers,
+// strides, counts and kept attribute slots, as the views of a batch drawing
+// one scene do: k_setup fetches each triangle once for all of them and sets
+// it up once per draw. A draw whose geometry no other draw shares is a
+// group of one.
+struct SetupGroupDev {
+    uint32_t block_base;   // first k_setup workgroup of the group (kSetupThreads triangles each)
+    uint32_t ntris;
+    uint32_t member_base;  // the group's draws: group_members[member_base, member_base + nmembers)
+    uint32_t nmembers;
+};
+constexpr uint32_t kMaxGroupMembers = 16;
+
 struct FrameDev {
     const DrawDev* draws;
     const ViewDev* views;
     const TexDev* texs;
+    const SetupGroupDev* groups;
+    const uint32_t* group_members;  // draw indices, group by group
+    uint32_t ngroups;
     uint32_t ndraws, nviews, total_bins, ntris;
     uint32_t bin_k, bin_size, clip_cap, entry_cap;
     uint32_t bin_log2, bin_klog2;  // log2 of bin_size and of bin_k (both powers of two)
@@ -96,7 +112,7 @@ struct FrameDev {
     float4* rec_ext;       // 3 x float4 per record: attribute planes 4-7 (wide frames only)
     const SponzaDev* sponza;  // SponzaShader constants (wide frames only)
     const uint32_t* sched_gate;  // learned order: [1] != 0 on frames that predict and record (else null)
-    uint32_t* rec_meta;    // draw index; 0xFFFFFFFF = no primitive
+    uint32_t* rec_meta;    // draw index (written only for slots that hold a primitive)
     uint32_t* clip_queue;  // global triangle ids routed to the clipper
     uint32_t* clip_base;   // per global triangle: first fan record (relative to ntris)
     uint32_t* clip_order;  // per fan record: order key

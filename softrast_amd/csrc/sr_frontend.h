@@ -371,10 +371,11 @@ __device__ __forceinline__ void write_record(const FrameDev& F, uint32_t r, cons
     F.rec_meta[r] = draw;
 }
 
-// FetchIndices / FetchPositions / attribute fetch + MVP (Binning.cpp:167-221, 474-479).
+// FetchIndices / FetchPositions / attribute fetch (Binning.cpp:167-221): the
+// object-space positions and the kept attribute slots of the three vertices.
 // Returns false (and flags the frame) if an index is out of range.
 template <uint32_t NS>
-__device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev& D, uint32_t tri, float4 v[3],
+__device__ __forceinline__ bool fetch_vertices(const FrameDev& F, const DrawDev& D, uint32_t tri, float p[3][3],
                                                float a[3][kWideSlots])
 {
     uint32_t idx[3];
@@ -393,12 +394,26 @@ __device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev&
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const auto p = global_ptr((const float*)(D.pos + (size_t)idx[k] * D.pos_stride));
-        v[k] = transform(D.mvp, p[0], p[1], p[2]);
+        const auto q = global_ptr((const float*)(D.pos + (size_t)idx[k] * D.pos_stride));
+        p[k][0] = q[0];
+        p[k][1] = q[1];
+        p[k][2] = q[2];
         const auto at = global_ptr((const float*)(D.attr + (size_t)idx[k] * D.attr_stride));
 #pragma unroll
         for (uint32_t s = 0; s < NS; ++s) a[k][s] = s < D.nslots ? at[D.slot_attr[s]] : 0.0f;
     }
+    return true;
+}
+
+// fetch_vertices + MVP (Binning.cpp:474-479).
+template <uint32_t NS>
+__device__ __forceinline__ bool fetch_triangle(const FrameDev& F, const DrawDev& D, uint32_t tri, float4 v[3],
+                                               float a[3][kWideSlots])
+{
+    float p[3][3];
+    if (!fetch_vertices<NS>(F, D, tri, p, a)) return false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = transform(D.mvp, p[k][0], p[k][1], p[k][2]);
     return true;
 }
 
@@ -429,9 +444,21 @@ __device__ __forceinline__ void block_stats_flush(const FrameDev& F, BlockStats*
 
 // NS: attribute planes kept per record (kMaxSlots, or kWideSlots in a frame
 // that has a draw reading eight varyings).
-template <uint32_t NS>
-__global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
+//
+// SHARED: the launch covers groups of several draws (a loop over the draws
+// per fetched triangle, which needs more registers); otherwise groups of one.
+// The launch takes groups [g_lo, g_hi), its workgroup b being the frame's
+// setup workgroup b + block_first.
+template <uint32_t NS, bool SHARED>
+__global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F, const SetupGroupDev* __restrict__ groups,
+                                                         const uint32_t* __restrict__ members,
+                                                         const DrawDev* __restrict__ draws,
+                                                         const ViewDev* __restrict__ views, uint32_t g_lo,
+                                                         uint32_t g_hi, uint32_t block_first)
 {
+    // The tables come in as restrict pointers as well: the kernel's stores
+    // cannot change them, so the compiler reads each draw's fields (the
+    // matrix above all) into scalar registers even between those stores.
     __shared__ BlockStats bs;
     __shared__ uint32_t s_hash[kSetupThreads / 64 * kWaveScratch];
     uint32_t* ht = s_hash + (threadIdx.x >> 6) * kWaveScratch;
@@ -439,43 +466,49 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F)
     const bool stats = (F.flags & FF_STATS) != 0;
     if (stats && threadIdx.x == 0) bs = BlockStats{ 0, 0 };
     if (stats) __syncthreads();
-    // the draw of this workgroup: the last with block_base <= blockIdx.x
-    uint32_t d = 0, hi = F.ndraws;
-    while (hi - d > 1) {
-        const uint32_t mid = (d + hi) >> 1;
-        if (F.draws[mid].block_base <= blockIdx.x) d = mid;
+    // the setup group of this workgroup: the last with block_base <= b
+    const uint32_t b = blockIdx.x + block_first;
+    uint32_t gi = g_lo, hi = g_hi;
+    while (hi - gi > 1) {
+        const uint32_t mid = (gi + hi) >> 1;
+        if (groups[mid].block_base <= b) gi = mid;
         else hi = mid;
     }
-    // copies, read before the kernel stores anything: the compiler then
-    // keeps them in scalar registers (scalar loads need memory that no
-    // store of the kernel may have changed)
-    const DrawDev D = F.draws[d];
-    const ViewDev V = F.views[D.view];
-    const uint32_t tri = (blockIdx.x - D.block_base) * kSetupThreads + threadIdx.x;
-    const bool valid = tri < D.ntris;
-    const uint32_t g = D.tri_base + tri;
-
-    float4 v[3];
+    const SetupGroupDev G = groups[gi];
+    const uint32_t tri = (b - G.block_base) * kSetupThreads + threadIdx.x;
+    const bool valid = tri < G.ntris;
+    // the triangle's vertices, fetched once for every draw of the group
+    float p[3][3];
     float a[3][kWideSlots];
-    Prim s;
-    bool set_up = false, survive = false, clipped = false;
-    if (valid && fetch_triangle<NS>(F, D, tri, v, a)) {
-        const uint32_t c0 = clip_codes(v[0]), c1 = clip_codes(v[1]), c2 = clip_codes(v[2]);
-        const uint32_t mor = c0 | c1 | c2;
-        if (mor == 0) set_up = setup_prim<NS>(s, v, a, D.nslots, V.width, V.height);
-        else clipped = (c0 & c1 & c2) == 0;
-        // a primitive drawing nothing still counts as binned by the reference
-        survive = set_up && !covers_nothing(s);
-    }
-    if (valid) {
+    const bool fetched = valid && fetch_vertices<NS>(F, draws[members[G.member_base]], tri, p, a);
+    const uint32_t nm = SHARED ? G.nmembers : 1u;
+#pragma unroll 1
+    for (uint32_t m = 0; m < nm; ++m) {
+        const uint32_t d = members[G.member_base + m];
+        const DrawDev& D = draws[d];
+        const ViewDev& V = views[D.view];
+        const uint32_t g = D.tri_base + tri;
+        Prim s;
+        bool set_up = false, survive = false, clipped = false;
+        if (fetched) {
+            float4 v[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) v[k] = transform(D.mvp, p[k][0], p[k][1], p[k][2]);
+            const uint32_t c0 = clip_codes(v[0]), c1 = clip_codes(v[1]), c2 = clip_codes(v[2]);
+            const uint32_t mor = c0 | c1 | c2;
+            if (mor == 0) set_up = setup_prim<NS>(s, v, a, D.nslots, V.width, V.height);
+            else clipped = (c0 & c1 & c2) == 0;
+            // a primitive drawing nothing still counts as binned by the reference
+            survive = set_up && !covers_nothing(s);
+        }
+        // a slot without a primitive is never referenced: nothing is written there
         if (survive) write_record<NS>(F, g, s, d);
-        else F.rec_meta[g] = 0xFFFFFFFFu;
-    }
-    const uint32_t q = wave_add_amount(&F.counters[C_CLIPQ], clipped ? 1u : 0u);
-    if (clipped) F.clip_queue[q] = g;  // at most one slot per triangle: never overflows
+        const uint32_t q = wave_add_amount(&F.counters[C_CLIPQ], clipped ? 1u : 0u);
+        if (clipped) F.clip_queue[q] = g;  // at most one slot per triangle: never overflows
 
-    if (stats) block_stats_add(&bs, set_up ? reference_tile_entries(s) : 0u, set_up ? 1u : 0u);
-    walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, g, ht, false);
+        if (stats) block_stats_add(&bs, set_up ? reference_tile_entries(s) : 0u, set_up ? 1u : 0u);
+        walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, g, ht, false);
+    }
     if (stats) {
         __syncthreads();
         block_stats_flush(F, &bs);
@@ -641,8 +674,6 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
                 if (ok) {
                     write_record<NS>(F, r, s, od);
                     F.clip_order[obase + j] = (og << 3) | j;
-                } else {
-                    F.rec_meta[r] = 0xFFFFFFFFu;
                 }
             }
             if (F.flags & FF_STATS) {

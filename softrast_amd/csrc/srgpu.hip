@@ -24,6 +24,7 @@
 #include <mutex>
 #include <new>
 #include <thread>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -520,6 +521,19 @@ void launch_backend_t(srg_ctx* ctx, uint32_t wg, bool stats, const FrameDev& F)
     }
 }
 
+// k_setup over the frame's setup groups: groups [0, nsingle) of one draw
+// (workgroups [0, single_blocks)), then the shared ones (the rest).
+template <uint32_t NS>
+void launch_setup(srg_ctx* ctx, const FrameDev& F, uint32_t nsingle, uint32_t single_blocks, uint32_t nblocks)
+{
+    if (single_blocks)
+        hipLaunchKernelGGL((k_setup<NS, false>), dim3(single_blocks), dim3(kSetupThreads), 0, ctx->stream, F,
+                           F.groups, F.group_members, F.draws, F.views, 0u, nsingle, 0u);
+    if (nblocks > single_blocks)
+        hipLaunchKernelGGL((k_setup<NS, true>), dim3(nblocks - single_blocks), dim3(kSetupThreads), 0, ctx->stream,
+                           F, F.groups, F.group_members, F.draws, F.views, nsingle, F.ngroups, single_blocks);
+}
+
 int launch_backend(srg_ctx* ctx, BackendLayout L, const FrameDev& F)
 {
     // resident workgroups per CU: LDS holds 8 bytes per strip pixel
@@ -724,9 +738,46 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             d.tex_info = td.empty ? 0u : (std::max(td.wlog2, td.hlog2) | (td.nmips << 8));
         }
         std::memcpy(d.mvp, s.mvp, sizeof(d.mvp));
-        d.block_base = nblocks;
-        nblocks += (d.ntris + kSetupThreads - 1) / kSetupThreads;
         total_tris += d.ntris;
+    }
+    // ---- setup groups: draws that share their geometry, at most
+    // kMaxGroupMembers each, in draw order ----
+    // groups of one first (k_setup<NS, false>), then the shared ones
+    // (k_setup<NS, true>), each launch over its own range of workgroups
+    std::vector<SetupGroupDev> gdev;
+    std::vector<uint32_t> gmem;
+    uint32_t nsingle = 0, single_blocks = 0;
+    {
+        std::vector<std::vector<uint32_t>> lists;
+        std::unordered_map<std::string, uint32_t> open_group;
+        for (size_t i = 0; i < ddev.size(); ++i) {
+            const DrawDev& d = ddev[i];
+            if (d.ntris == 0) continue;
+            std::string key((const char*)&d.idx, 3 * sizeof(const uint8_t*));
+            const uint32_t fields[] = { d.idx_stride, d.pos_stride, d.attr_stride, d.ntris, d.pos_num, d.attr_num,
+                                        d.nslots };
+            key.append((const char*)fields, sizeof(fields));
+            key.append((const char*)d.slot_attr, sizeof(d.slot_attr));
+            auto it = open_group.find(key);
+            if (it == open_group.end() || lists[it->second].size() >= kMaxGroupMembers) {
+                open_group[key] = (uint32_t)lists.size();
+                lists.emplace_back();
+                it = open_group.find(key);
+            }
+            lists[it->second].push_back((uint32_t)i);
+        }
+        for (int pass = 0; pass < 2; ++pass)
+            for (const auto& l : lists) {
+                if ((l.size() > 1) != (pass == 1)) continue;
+                const uint32_t nt = ddev[l[0]].ntris;
+                gdev.push_back({ nblocks, nt, (uint32_t)gmem.size(), (uint32_t)l.size() });
+                gmem.insert(gmem.end(), l.begin(), l.end());
+                nblocks += (nt + kSetupThreads - 1) / kSetupThreads;
+                if (pass == 0) {
+                    nsingle = (uint32_t)gdev.size();
+                    single_blocks = nblocks;
+                }
+            }
     }
     if (total_tris >= (1ull << 29)) return SRG_ERR_INVALID;  // order key is (triangle << 3) | fan
     // setup records keep two attribute planes (9 floats of shade record) when
@@ -741,6 +792,8 @@ int run_frame(srg_ctx* ctx, bool noalloc)
     const size_t o_view = off; off = align_up(off + sizeof(ViewDev) * vdev.size(), 256);
     const size_t o_tex = off;  off = align_up(off + sizeof(TexDev) * std::max<size_t>(tdev.size(), 1), 256);
     const size_t o_sponza = off; off = align_up(off + (wide ? sizeof(SponzaDev) : 0), 256);
+    const size_t o_group = off; off = align_up(off + sizeof(SetupGroupDev) * std::max<size_t>(gdev.size(), 1), 256);
+    const size_t o_member = off; off = align_up(off + sizeof(uint32_t) * std::max<size_t>(gmem.size(), 1), 256);
     int st = SRG_OK;
     FrameArenas A;
     Carver C{ (uint8_t*)ctx->scratch_dev, noalloc ? (size_t)SRG_NOALLOC_SCRATCH_BYTES : 0u };
@@ -756,6 +809,8 @@ int run_frame(srg_ctx* ctx, bool noalloc)
     std::memcpy(ctx->h_tables + o_view, vdev.data(), sizeof(ViewDev) * vdev.size());
     if (!tdev.empty()) std::memcpy(ctx->h_tables + o_tex, tdev.data(), sizeof(TexDev) * tdev.size());
     if (wide) std::memcpy(ctx->h_tables + o_sponza, &ctx->sponza, sizeof(SponzaDev));
+    if (!gdev.empty()) std::memcpy(ctx->h_tables + o_group, gdev.data(), sizeof(SetupGroupDev) * gdev.size());
+    if (!gmem.empty()) std::memcpy(ctx->h_tables + o_member, gmem.data(), sizeof(uint32_t) * gmem.size());
 
     // ---- arenas ----
     const uint32_t TB = (uint32_t)std::max<uint64_t>(total_bins, 1);
@@ -852,6 +907,9 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         F.draws = (const DrawDev*)(A.tables + o_draw);
         F.views = (const ViewDev*)(A.tables + o_view);
         F.texs = (const TexDev*)(A.tables + o_tex);
+        F.groups = (const SetupGroupDev*)(A.tables + o_group);
+        F.group_members = (const uint32_t*)(A.tables + o_member);
+        F.ngroups = (uint32_t)gdev.size();
         F.ndraws = (uint32_t)ddev.size();
         F.nviews = (uint32_t)vdev.size();
         F.total_bins = (uint32_t)total_bins;
@@ -943,13 +1001,13 @@ int run_frame(srg_ctx* ctx, bool noalloc)
                         hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, ctx->stream, v.primid, 0xFFFFFFFFu,
                                            (size_t)v.tiles_x * v.tiles_y * kTilePixels);
             if (wide) {
-                if (nblocks) hipLaunchKernelGGL(k_setup<kWideSlots>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
+                launch_setup<kWideSlots>(ctx, F, nsingle, single_blocks, nblocks);
                 if (NT) hipLaunchKernelGGL(k_clip<kWideSlots>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
             } else if (narrow) {
-                if (nblocks) hipLaunchKernelGGL(k_setup<2>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
+                launch_setup<2>(ctx, F, nsingle, single_blocks, nblocks);
                 if (NT) hipLaunchKernelGGL(k_clip<2>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
             } else {
-                if (nblocks) hipLaunchKernelGGL(k_setup<kMaxSlots>, dim3(nblocks), dim3(kSetupThreads), 0, ctx->stream, F);
+                launch_setup<kMaxSlots>(ctx, F, nsingle, single_blocks, nblocks);
                 if (NT) hipLaunchKernelGGL(k_clip<kMaxSlots>, dim3(clip_blocks), dim3(kClipThreads), 0, ctx->stream, F);
             }
             FrameDev Fs = F;  // with ordering, the natural queue goes to the spare buffer first
