@@ -911,30 +911,13 @@ __global__ __launch_bounds__(256) void k_scatter(FrameDev F)
                                  : F.pair_regions * F.pair_region_cap + (region - F.pair_regions) * F.clip_pair_cap;
     const uint32_t np = F.pair_total;
     const uint32_t stride = gridDim.x * blockDim.x;
-    // U pairs per lane and step, their loads issued together:
-    // each pair's store waits on two dependent gathers (bin start, part
-    // offset), so independent pairs in flight hide that latency
-    constexpr uint32_t U = 4;
-    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += U * stride) {
-        uint32_t rec[U], pkey[U], slot[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const uint32_t i = i0 + u * stride;
-            const uint32_t p = first_p + i;
-            rec[u] = i < n ? F.pairs[p] : kNoPair;
-            pkey[u] = i < n ? F.pairs[np + p] : 0u;
-            slot[u] = i < n ? F.pairs[2u * np + p] : 0u;
-        }
-        uint32_t at[U];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            // a hole (a candidate bin the corner test rejected) stores nothing
-            const uint32_t key = pkey[u] - part_of(rec[u]) * F.total_bins;
-            at[u] = rec[u] != kNoPair ? F.bin_start[key] + F.part_off[pkey[u]] + slot[u] : 0u;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-            if (rec[u] != kNoPair) F.entries[at[u]] = rec[u];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t p = first_p + i;
+        const uint32_t rec = F.pairs[p];
+        if (rec == kNoPair) continue;  // a candidate bin the corner test rejected
+        const uint32_t pkey = F.pairs[np + p], slot = F.pairs[2u * np + p];
+        const uint32_t key = pkey - part_of(rec) * F.total_bins;
+        F.entries[F.bin_start[key] + F.part_off[pkey] + slot] = rec;
     }
 }
 
