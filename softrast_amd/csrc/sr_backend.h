@@ -416,15 +416,16 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
         const float zdx = ffrom(o[11 * 64]), zdy = ffrom(o[12 * 64]), zc0 = ffrom(o[13 * 64]);
         const unsigned long long lw = (unsigned long long)pin(o[14 * 64]);
         const uint32_t k = it - (m >> 7);
-        const uint32_t cx0 = pc & 127u, ccw = (pc >> 7) & 127u, cy0 = (pc >> 14) & 127u, cy1 = (pc >> 21) & 127u;
+        const uint32_t cx0 = pc & 127u, cy0 = (pc >> 14) & 127u;
         const bool tight = ((pc >> 28) & 1u) != 0;
         const uint32_t snp = pd >> 17;
         const uint32_t brow = __umul24(k, pd & 0x1FFFFu) >> 16;  // k < 512, magic < 2^17
         const uint32_t x = 2u * ((cx0 >> 1) + (k - brow * snp)), yb = (cy0 & ~7u) + 8u * brow;
         const uint32_t xb = x & ~7u, l = x & 7u;
-        // the pair's columns inside the owner's candidate rectangle (checked, not assumed)
-        const bool in0 = x - cx0 < ccw, in1 = x + 1u - cx0 < ccw;
-        const bool inside = valid && yb < cy1 && (in0 || in1);
+        // A valid item lies in its owner's candidate rectangle by its
+        // numbering: the np pairs counted are those overlapping cx0..cx1 and
+        // the nby block rows those overlapping cy0..cy1.
+        const bool inside = valid;
         int32_t thr = 1;  // tight: all three edge values > 0; an exact 0 is decided below
         bool walk_rows = inside;
         if (inside && !tight) {
