@@ -285,12 +285,14 @@ static void clip_plane(clip_buffer* b, uint32_t plane)
     for (uint32_t i1 = 0; i1 < b->n_in; ++i1) {
         float d1 = dot4(cp, in_v[i1]);
         int in0 = d0 >= 0.0f, in1 = d1 >= 0.0f;
-        if (in0) {
+        /* the reference asserts room for every emission (Binning.cpp:122,
+         * :133, :145); one that would overflow CLIP_VERTS is dropped */
+        if (in0 && b->n_out < CLIP_VERTS) {
             uint32_t o = b->n_out++;
             out_v[o] = in_v[i0];
             memcpy(out_a[o], in_a[i0], sizeof(float) * SRO_MAX_VARYINGS);
         }
-        if (in0 ^ in1) {
+        if ((in0 ^ in1) && b->n_out < CLIP_VERTS) {
             uint32_t o = b->n_out++;
             uint32_t a, c; float t;
             if (in1) { t = d1 / (d1 - d0); a = i1; c = i0; }

@@ -7,6 +7,7 @@
 //   k_clip   slow path over the queued triangles: Sutherland-Hodgman clip,
 //            fan, setup of each fan triangle, bin counts
 //   k_scan   exclusive scan of bin counts + the back end's work queue
+//            (three passes; k_scan_single in one workgroup for small frames)
 //   k_scatter one lane per (record, bin) pair found by the count walks:
 //            writes the record index into its bin list
 #pragma once
@@ -565,11 +566,15 @@ __device__ __forceinline__ uint32_t clip_polygon(const ClipPoly<NS>& poly, uint3
         for (uint32_t i1 = 0; i1 < n; ++i1) {
             const float d1 = clip_dot(plane, poly.pos(pin, i1));
             const bool in0 = d0 >= 0.0f, in1 = d1 >= 0.0f;
-            if (in0) {
+            // a convex polygon gains at most one vertex per plane (9 after six
+            // planes); the reference asserts that an emission finds room
+            // (Binning.cpp:122, :133, :145), so one that would not -- only
+            // rounding on a degenerate polygon could ask for it -- is dropped
+            if (in0 && nout < 9u) {
                 for (uint32_t c = 0; c < ClipPoly<NS>::kComps; ++c) poly.at(po, nout, c) = poly.at(pin, i0, c);
                 ++nout;
             }
-            if (in0 != in1) {
+            if (in0 != in1 && nout < 9u) {
                 uint32_t ia, ib;
                 float tt;
                 if (in1) { tt = d1 / (d1 - d0); ia = i1; ib = i0; }
@@ -766,6 +771,19 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_blocks(FrameDev F, uint32
     }
 }
 
+// The frame totals: entries (the end of the last bin's list) and queue
+// length, the entry arena's overflow check and the clip regions' pair count.
+__device__ __forceinline__ void scan_totals(const FrameDev& F, uint32_t tc, uint32_t tq)
+{
+    F.bin_start[F.total_bins] = tc;
+    F.counters[C_ENTRIES] = tc;
+    F.counters[C_QLEN] = tq;
+    if (tc > F.entry_cap) atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
+    uint32_t cp = 0;
+    for (uint32_t i = 0; i < kClipPairRegions; ++i) cp += F.pair_count[(F.pair_regions + i) * kPairStride];
+    F.counters[C_CLIP_PAIRS] = cp;
+}
+
 // Pass 2 (one workgroup): block offsets and the frame totals.
 __global__ __launch_bounds__(kScanThreads) void k_scan_top(FrameDev F, uint32_t* blk, uint32_t nblk)
 {
@@ -785,15 +803,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_top(FrameDev F, uint32_t*
         tc += bc;
         tq += bq;
     }
-    if (threadIdx.x == 0) {
-        F.bin_start[F.total_bins] = tc;
-        F.counters[C_ENTRIES] = tc;
-        F.counters[C_QLEN] = tq;
-        if (tc > F.entry_cap) atomicOr(&F.counters[C_FLAGS], FLAG_ENTRY_OVERFLOW);
-        uint32_t cp = 0;
-        for (uint32_t i = 0; i < kClipPairRegions; ++i) cp += F.pair_count[(F.pair_regions + i) * kPairStride];
-        F.counters[C_CLIP_PAIRS] = cp;
-    }
+    if (threadIdx.x == 0) scan_totals(F, tc, tq);
 }
 
 // Pass 3: bin starts and queue slots.
@@ -815,6 +825,53 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(FrameDev F, const u
         }
         F.bin_count[t] = off;
     }
+}
+
+// The three passes in one workgroup, for frames of at most kScanSingleBins
+// bins (a view or a few), which saves two launches where a launch costs
+// more than the work: each thread takes kScanSinglePer consecutive bins,
+// loads all their part counts at once and keeps them for the part offsets.
+constexpr uint32_t kScanSinglePer = 2;
+constexpr uint32_t kScanSingleBins = kScanSinglePer * kScanThreads;
+__global__ __launch_bounds__(kScanThreads) void k_scan_single(FrameDev F)
+{
+    const uint32_t t0 = threadIdx.x * kScanSinglePer, nb = F.total_bins;
+    uint32_t cnt[kScanSinglePer][kBinParts], c[kScanSinglePer];
+    bool enq[kScanSinglePer];
+    uint32_t sc = 0, sq = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanSinglePer; ++j) {
+        const uint32_t t = t0 + j;
+        c[j] = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kBinParts; ++k) {
+            cnt[j][k] = t < nb ? F.part_count[k * nb + t] : 0u;
+            c[j] += cnt[j][k];
+        }
+        enq[j] = t < nb && (c[j] || view_cleared(F, t));
+        sc += c[j];
+        sq += enq[j] ? 1u : 0u;
+    }
+    uint32_t tc, tq;
+    block_scan2(sc, sq, tc, tq);  // the thread's first entry and queue slot
+#pragma unroll
+    for (uint32_t j = 0; j < kScanSinglePer; ++j) {
+        const uint32_t t = t0 + j;
+        if (t < nb) {
+            F.bin_start[t] = sc;
+            if (enq[j]) F.queue[sq] = t;
+            uint32_t off = 0;  // the bin's parts, one after another
+#pragma unroll
+            for (uint32_t k = 0; k < kBinParts; ++k) {
+                F.part_off[k * nb + t] = off;
+                off += cnt[j][k];
+            }
+            F.bin_count[t] = off;
+        }
+        sc += c[j];
+        sq += enq[j] ? 1u : 0u;
+    }
+    if (threadIdx.x == 0) scan_totals(F, tc, tq);
 }
 
 // Longest-first ordering of the back end's queue: bins are bucketed by a
@@ -900,24 +957,35 @@ __global__ __launch_bounds__(kOrderBlock) void k_order_scatter(FrameDev F, const
 // Bin lists from the pairs of the count walks: entry (bin start + the part's
 // offset in the bin + the pair's slot) = record. An overflowed frame (pairs or
 // entries past the arena) is rendered again with larger arenas.
-// grid: (blocks per region, pair_regions + kClipPairRegions)
-__global__ __launch_bounds__(256) void k_scatter(FrameDev F)
+//
+// grid: (regions, chunks); workgroup (r, c) takes chunks c, c + gridDim.y, ...
+// of `chunk` pairs (a multiple of 256) of region r. Workgroups are dispatched
+// x fastest, so
+// chunk c of every region is written before chunk c + 1 of any: the regions
+// fill at the same rate (the set-up's workgroups take them in turn), so the
+// pairs in flight were counted at about the same time, and a bin part's
+// entries -- whose slots follow the order of counting -- are written close
+// together in time, filling cache lines before they leave the cache.
+__global__ __launch_bounds__(256) void k_scatter(FrameDev F, uint32_t chunk)
 {
     if (F.counters[C_FLAGS] & (FLAG_ENTRY_OVERFLOW | FLAG_CLIP_OVERFLOW)) return;
-    const uint32_t region = blockIdx.y;
+    const uint32_t region = blockIdx.x;
     const uint32_t n = F.pair_count[region * kPairStride];  // within the region's capacity without overflow
     const uint32_t first_p = region < F.pair_regions
                                  ? region * F.pair_region_cap
                                  : F.pair_regions * F.pair_region_cap + (region - F.pair_regions) * F.clip_pair_cap;
     const uint32_t np = F.pair_total;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t p = first_p + i;
-        const uint32_t rec = F.pairs[p];
-        if (rec == kNoPair) continue;  // a candidate bin the corner test rejected
-        const uint32_t pkey = F.pairs[np + p], slot = F.pairs[2u * np + p];
-        const uint32_t key = pkey - part_of(rec) * F.total_bins;
-        F.entries[F.bin_start[key] + F.part_off[pkey] + slot] = rec;
+    for (uint32_t c0 = blockIdx.y * chunk; c0 < n; c0 += gridDim.y * chunk) {
+        for (uint32_t k = threadIdx.x; k < chunk; k += 256) {
+            const uint32_t i = c0 + k;
+            if (i >= n) break;
+            const uint32_t p = first_p + i;
+            const uint32_t rec = F.pairs[p];
+            if (rec == kNoPair) continue;  // a candidate bin the corner test rejected
+            const uint32_t pkey = F.pairs[np + p], slot = F.pairs[2u * np + p];
+            const uint32_t key = pkey - part_of(rec) * F.total_bins;
+            F.entries[F.bin_start[key] + F.part_off[pkey] + slot] = rec;
+        }
     }
 }
 

@@ -948,8 +948,13 @@ int run_frame(srg_ctx* ctx, bool noalloc)
                   ((ctx->cfg.flags & SRG_CFG_KEEP_PRIMID) ? FF_PRIMID : 0u) |
                   ((ctx->cfg.flags & SRG_CFG_TILE_COSTS) ? FF_TILE_COSTS : 0u);
 
+        // k_scatter: a workgroup per (region, chunk of a set-up region's
+        // capacity); a pair per thread while the chunks number at most 512,
+        // up to eight above (fewer, longer-lived workgroups)
+        const uint32_t emit_chunk =
+            256u * std::max<uint32_t>(1, std::min<uint32_t>(8, (F.pair_region_cap + 256u * 512u - 1) / (256u * 512u)));
         const uint32_t emit_blocks =
-            std::max<uint32_t>(1, (uint32_t)std::min<uint64_t>((F.pair_region_cap + 255) / 256, 64));
+            std::max<uint32_t>(1, std::min<uint32_t>((F.pair_region_cap + emit_chunk - 1) / emit_chunk, 4096));
         const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + kClipThreads - 1) / kClipThreads, 2048));
         const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((kBinParts * TB + 255) / 256, 1024));
         const uint32_t scan_blocks = (TB + kScanThreads - 1) / kScanThreads;
@@ -1012,12 +1017,18 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             }
             FrameDev Fs = F;  // with ordering, the natural queue goes to the spare buffer first
             if (ordered) Fs.queue = A.queue2;
-            hipLaunchKernelGGL(k_scan_blocks, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs, A.scan_blk);
-            hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs, A.scan_blk, scan_blocks);
-            hipLaunchKernelGGL(k_scan_apply, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs,
-                               (const uint32_t*)A.scan_blk);
-            hipLaunchKernelGGL(k_scatter, dim3(emit_blocks, F.pair_regions + kClipPairRegions), dim3(256), 0,
-                               ctx->stream, F);
+            if (TB <= kScanSingleBins) {
+                hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs);
+            } else {
+                hipLaunchKernelGGL(k_scan_blocks, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs,
+                                   A.scan_blk);
+                hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, ctx->stream, Fs, A.scan_blk,
+                                   scan_blocks);
+                hipLaunchKernelGGL(k_scan_apply, dim3(scan_blocks), dim3(kScanThreads), 0, ctx->stream, Fs,
+                                   (const uint32_t*)A.scan_blk);
+            }
+            hipLaunchKernelGGL(k_scatter, dim3(F.pair_regions + kClipPairRegions, emit_blocks), dim3(256), 0,
+                               ctx->stream, F, emit_chunk);
             if (learned) {
                 hipLaunchKernelGGL(k_bin_predict, dim3(std::min<uint32_t>((TB + 3) / 4, 16384u)),
                                    dim3(256), 0, ctx->stream, F, ctx->sched_feat.ptr,
