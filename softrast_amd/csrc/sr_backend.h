@@ -381,8 +381,10 @@ __device__ __forceinline__ void raster_wave(unsigned long long* keys, const Entr
     const uint32_t total = bcast(incl, 63);
     // per-entry values the column workers need, packed; the first block row
     // is cy0 rounded down to 8
+    // k / np == (k * magic) >> 16 for k < 512 when magic is ceil(2^16 / np)
+    // or one more (np <= 64); the 1-ulp reciprocal gives one of the two
     const uint32_t npp = max(np, 1u);
-    const uint32_t magic = (65536u + npp - 1u) / npp;  // k / np == (k * magic) >> 16 for k < 512
+    const uint32_t magic = (uint32_t)(65536.0f * __builtin_amdgcn_rcpf((float)npp)) + 1u;
     const uint32_t pk_cand = e.cx0 | (cw << 7) | (e.cy0 << 14) | (e.cy1 << 21) | ((e.tight ? 1u : 0u) << 28);
     const uint32_t pk_div = magic | (npp << 17);
     const uint32_t first = incl - n;

@@ -506,7 +506,8 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F, const Setup
         // a slot without a primitive is never referenced: nothing is written there
         if (survive) write_record<NS>(F, g, s, d);
         const uint32_t q = wave_add_amount(&F.counters[C_CLIPQ], clipped ? 1u : 0u);
-        if (clipped) F.clip_queue[q] = g;  // at most one slot per triangle: never overflows
+        // at most one slot per triangle: never overflows
+        if (clipped) reinterpret_cast<uint2*>(F.clip_queue)[q] = make_uint2(g, d);
 
         if (stats) block_stats_add(&bs, set_up ? reference_tile_entries(s) : 0u, set_up ? 1u : 0u);
         walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, g, ht, false);
@@ -592,17 +593,6 @@ __device__ __forceinline__ uint32_t clip_polygon(const ClipPoly<NS>& poly, uint3
     return n;
 }
 
-__device__ __forceinline__ uint32_t draw_of_triangle(const FrameDev& F, uint32_t g)
-{
-    uint32_t lo = 0, hi = F.ndraws;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (F.draws[mid].tri_base <= g) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Slow path: the queued triangles that cross a clip plane, one per lane of
 // one-wave workgroups; the fan triangles (0, i-1, i) (Binning.cpp:525-533) get
 // records after the submitted triangles' and bin counts like any other.
@@ -621,8 +611,9 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
         const bool active = qi < n;
         uint32_t g = 0, d = 0, nfan = 0, pin = 0;
         if (active) {
-            g = F.clip_queue[qi];
-            d = draw_of_triangle(F, g);
+            const uint2 e = reinterpret_cast<const uint2*>(F.clip_queue)[qi];
+            g = e.x;
+            d = e.y;
             const DrawDev& D = F.draws[d];
             float4 v[3];
             float a[3][kWideSlots];
@@ -829,49 +820,40 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_apply(FrameDev F, const u
 
 // The three passes in one workgroup, for frames of at most kScanSingleBins
 // bins (a view or a few), which saves two launches where a launch costs
-// more than the work: each thread takes kScanSinglePer consecutive bins,
-// loads all their part counts at once and keeps them for the part offsets.
-constexpr uint32_t kScanSinglePer = 2;
-constexpr uint32_t kScanSingleBins = kScanSinglePer * kScanThreads;
+// more than the work: blocks of kScanThreads bins one after another with
+// running totals, each bin's part counts loaded once and kept in registers
+// for its parts' offsets.
+constexpr uint32_t kScanSingleBins = 4u * kScanThreads;
 __global__ __launch_bounds__(kScanThreads) void k_scan_single(FrameDev F)
 {
-    const uint32_t t0 = threadIdx.x * kScanSinglePer, nb = F.total_bins;
-    uint32_t cnt[kScanSinglePer][kBinParts], c[kScanSinglePer];
-    bool enq[kScanSinglePer];
-    uint32_t sc = 0, sq = 0;
-#pragma unroll
-    for (uint32_t j = 0; j < kScanSinglePer; ++j) {
-        const uint32_t t = t0 + j;
-        c[j] = 0;
+    const uint32_t nb = F.total_bins;
+    uint32_t run_c = 0, run_q = 0;
+    for (uint32_t base = 0; base < nb; base += kScanThreads) {
+        const uint32_t t = base + threadIdx.x;
+        uint32_t cnt[kBinParts], c = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kBinParts; ++k) {
-            cnt[j][k] = t < nb ? F.part_count[k * nb + t] : 0u;
-            c[j] += cnt[j][k];
+            cnt[k] = t < nb ? F.part_count[k * nb + t] : 0u;
+            c += cnt[k];
         }
-        enq[j] = t < nb && (c[j] || view_cleared(F, t));
-        sc += c[j];
-        sq += enq[j] ? 1u : 0u;
-    }
-    uint32_t tc, tq;
-    block_scan2(sc, sq, tc, tq);  // the thread's first entry and queue slot
-#pragma unroll
-    for (uint32_t j = 0; j < kScanSinglePer; ++j) {
-        const uint32_t t = t0 + j;
+        const bool enq = t < nb && (c || view_cleared(F, t));
+        uint32_t sc = c, sq = enq ? 1u : 0u, tc, tq;
+        block_scan2(sc, sq, tc, tq);
         if (t < nb) {
-            F.bin_start[t] = sc;
-            if (enq[j]) F.queue[sq] = t;
+            F.bin_start[t] = run_c + sc;
+            if (enq) F.queue[run_q + sq] = t;
             uint32_t off = 0;  // the bin's parts, one after another
 #pragma unroll
             for (uint32_t k = 0; k < kBinParts; ++k) {
                 F.part_off[k * nb + t] = off;
-                off += cnt[j][k];
+                off += cnt[k];
             }
             F.bin_count[t] = off;
         }
-        sc += c[j];
-        sq += enq[j] ? 1u : 0u;
+        run_c += tc;
+        run_q += tq;
     }
-    if (threadIdx.x == 0) scan_totals(F, tc, tq);
+    if (threadIdx.x == 0) scan_totals(F, run_c, run_q);
 }
 
 // Longest-first ordering of the back end's queue: bins are bucketed by a

@@ -829,7 +829,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         A.order_hist = C.take<uint32_t>(n_hist);
         A.scan_blk = C.take<uint32_t>(n_scan);
         A.bin_cost = C.take<uint64_t>(TB);
-        A.clip_queue = C.take<uint32_t>(NQ);
+        A.clip_queue = C.take<uint32_t>(2 * (size_t)NQ);
         A.clip_base = C.take<uint32_t>(NQ);
         if (!C.ok) return SRG_ERR_OVERFLOW;
     } else {
@@ -838,7 +838,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             (st = ctx->part_off.ensure((size_t)kBinParts * TB)) || (st = ctx->queue.ensure(TB)) ||
             (st = ctx->queue2.ensure(TB)) || (st = ctx->order_bkt.ensure(TB)) || (st = ctx->order_hist.ensure(n_hist)) ||
             (st = ctx->scan_blk.ensure(n_scan)) || (st = ctx->bin_cost.ensure(TB)) ||
-            (st = ctx->counters.ensure(kCounterWords)) || (st = ctx->clip_queue.ensure(NQ)) ||
+            (st = ctx->counters.ensure(kCounterWords)) || (st = ctx->clip_queue.ensure(2 * (size_t)NQ)) ||
             (st = ctx->clip_base.ensure(NQ)))
             return st;
         A.counters = ctx->counters.ptr;
