@@ -225,7 +225,7 @@ int srg_end_frame(srg_ctx* ctx);
  * used nor trained), the autotuner's choice for the frame's signature is used
  * only once it is locked (no sweep; otherwise the configured layout), and no
  * graph is captured. A frame whose arenas do not fit returns SRG_ERR_OVERFLOW
- * and draws nothing (its planes keep their pending clears); about 110-130
+ * and draws nothing (its planes keep their pending clears); about 115-135
  * bytes per triangle plus 16 per triangle-bin entry, i.e. roughly 350k
  * triangles per frame. Every call on one GPU shares that GPU's array: calls that overlap on
  * one GPU (two threads, two contexts) race and must be serialised by the
