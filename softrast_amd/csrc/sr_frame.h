@@ -113,7 +113,7 @@ struct FrameDev {
     const SponzaDev* sponza;  // SponzaShader constants (wide frames only)
     const uint32_t* sched_gate;  // learned order: [1] != 0 on frames that predict and record (else null)
     uint32_t* rec_meta;    // draw index (written only for slots that hold a primitive)
-    uint32_t* clip_queue;  // per triangle routed to the clipper: (global triangle id, draw index)
+    uint2* clip_queue;     // per triangle routed to the clipper: (global triangle id, draw index)
     uint32_t* clip_base;   // per global triangle: first fan record (relative to ntris)
     uint32_t* clip_order;  // per fan record: order key
     uint32_t* bin_count;   // entries per bin (written by the scan)

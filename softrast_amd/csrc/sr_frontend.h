@@ -507,7 +507,7 @@ __global__ __launch_bounds__(kSetupThreads) void k_setup(FrameDev F, const Setup
         if (survive) write_record<NS>(F, g, s, d);
         const uint32_t q = wave_add_amount(&F.counters[C_CLIPQ], clipped ? 1u : 0u);
         // at most one slot per triangle: never overflows
-        if (clipped) reinterpret_cast<uint2*>(F.clip_queue)[q] = make_uint2(g, d);
+        if (clipped) F.clip_queue[q] = make_uint2(g, d);
 
         if (stats) block_stats_add(&bs, set_up ? reference_tile_entries(s) : 0u, set_up ? 1u : 0u);
         walk_bins(F, V, survive, s.c, s.dx, s.dy, s.xmin, s.xmax, s.ymin, s.ymax, g, ht, false);
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(kClipThreads) void k_clip(FrameDev F)
         const bool active = qi < n;
         uint32_t g = 0, d = 0, nfan = 0, pin = 0;
         if (active) {
-            const uint2 e = reinterpret_cast<const uint2*>(F.clip_queue)[qi];
+            const uint2 e = F.clip_queue[qi];
             g = e.x;
             d = e.y;
             const DrawDev& D = F.draws[d];

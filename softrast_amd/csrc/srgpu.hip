@@ -223,7 +223,8 @@ struct srg_ctx {
     DevArray<float> rec_shade;
     DevArray<float4> rec_ext;         // attribute planes 4-7, frames with a SponzaShader draw
     SponzaDev sponza{};               // SponzaShader constants (srg_set_sponza_constants)
-    DevArray<uint32_t> rec_meta, clip_queue, clip_base, clip_order;
+    DevArray<uint32_t> rec_meta, clip_base, clip_order;
+    DevArray<uint2> clip_queue;
     DevArray<uint32_t> bin_count, bin_start, part_count, part_off, queue, queue2, entries, pairs, scan_blk;
     DevArray<uint16_t> order_bkt;   // per queue entry: its cost bucket
     DevArray<uint32_t> order_hist;  // per (queue block, bucket): count, then offset
@@ -607,7 +608,8 @@ struct FrameArenas {
     uint4* rec_raster = nullptr;
     float* rec_shade = nullptr;
     float4* rec_ext = nullptr;
-    uint32_t *rec_meta = nullptr, *clip_queue = nullptr, *clip_base = nullptr, *clip_order = nullptr;
+    uint32_t *rec_meta = nullptr, *clip_base = nullptr, *clip_order = nullptr;
+    uint2* clip_queue = nullptr;
     uint32_t *bin_count = nullptr, *bin_start = nullptr, *part_count = nullptr, *pairs = nullptr;
     uint32_t *part_off = nullptr, *queue = nullptr, *queue2 = nullptr, *entries = nullptr, *scan_blk = nullptr;
     uint16_t* order_bkt = nullptr;
@@ -829,7 +831,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         A.order_hist = C.take<uint32_t>(n_hist);
         A.scan_blk = C.take<uint32_t>(n_scan);
         A.bin_cost = C.take<uint64_t>(TB);
-        A.clip_queue = C.take<uint32_t>(2 * (size_t)NQ);
+        A.clip_queue = C.take<uint2>(NQ);
         A.clip_base = C.take<uint32_t>(NQ);
         if (!C.ok) return SRG_ERR_OVERFLOW;
     } else {
@@ -838,7 +840,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             (st = ctx->part_off.ensure((size_t)kBinParts * TB)) || (st = ctx->queue.ensure(TB)) ||
             (st = ctx->queue2.ensure(TB)) || (st = ctx->order_bkt.ensure(TB)) || (st = ctx->order_hist.ensure(n_hist)) ||
             (st = ctx->scan_blk.ensure(n_scan)) || (st = ctx->bin_cost.ensure(TB)) ||
-            (st = ctx->counters.ensure(kCounterWords)) || (st = ctx->clip_queue.ensure(2 * (size_t)NQ)) ||
+            (st = ctx->counters.ensure(kCounterWords)) || (st = ctx->clip_queue.ensure(NQ)) ||
             (st = ctx->clip_base.ensure(NQ)))
             return st;
         A.counters = ctx->counters.ptr;
