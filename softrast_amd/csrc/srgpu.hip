@@ -627,11 +627,13 @@ int run_frame(srg_ctx* ctx, bool noalloc)
 {
     // ---- views: planes drawn to, plus planes with a pending clear ----
     std::vector<ViewKey> views;
+    uint32_t last_view = ~0u;  // consecutive draws mostly share a plane
     auto view_of = [&](srg_fb* fb, uint32_t plane) -> uint32_t {
+        if (last_view < views.size() && views[last_view].fb == fb && views[last_view].plane == plane) return last_view;
         for (uint32_t i = 0; i < views.size(); ++i)
-            if (views[i].fb == fb && views[i].plane == plane) return i;
+            if (views[i].fb == fb && views[i].plane == plane) return last_view = i;
         views.push_back({ fb, plane });
-        return (uint32_t)views.size() - 1;
+        return last_view = (uint32_t)views.size() - 1;
     };
     std::vector<uint32_t> draw_view;
     draw_view.reserve(ctx->draws.size());
@@ -711,7 +713,7 @@ int run_frame(srg_ctx* ctx, bool noalloc)
         d.pos_num = s.position_num;
         d.attr_num = s.attribute_num;
         d.tri_base = (uint32_t)total_tris;
-        d.view = view_of(s.fb, s.plane);
+        d.view = draw_view[i];
         d.shader = s.shader;
         int st = shader_slots(s.shader, s.uv_offset, s.attribute_stride / 4u, d.slot_attr, d.nslots, d.deriv_slot);
         if (st) return st;
@@ -750,16 +752,37 @@ int run_frame(srg_ctx* ctx, bool noalloc)
     std::vector<uint32_t> gmem;
     uint32_t nsingle = 0, single_blocks = 0;
     {
+        // the geometry a draw fetches: buffers, strides, counts, kept slots
+        struct GeomKey {
+            const uint8_t* ptr[3];
+            uint32_t f[8];  // seven fields and a zero word
+            uint32_t slot_attr[kWideSlots];
+            bool operator==(const GeomKey& o) const { return std::memcmp(this, &o, sizeof(GeomKey)) == 0; }
+        };
+        struct GeomHash {
+            size_t operator()(const GeomKey& k) const
+            {
+                uint64_t h = 0;
+                const uint64_t* w = (const uint64_t*)&k;
+                for (size_t i = 0; i < sizeof(GeomKey) / 8; ++i) h = mix64(h, w[i]);
+                return (size_t)h;
+            }
+        };
+        static_assert(sizeof(GeomKey) % 8 == 0 && sizeof(DrawDev::slot_attr) == sizeof(GeomKey::slot_attr),
+                      "geometry key: whole words, every kept slot");
         std::vector<std::vector<uint32_t>> lists;
-        std::unordered_map<std::string, uint32_t> open_group;
+        std::unordered_map<GeomKey, uint32_t, GeomHash> open_group;
+        open_group.reserve(ddev.size());
         for (size_t i = 0; i < ddev.size(); ++i) {
             const DrawDev& d = ddev[i];
             if (d.ntris == 0) continue;
-            std::string key((const char*)&d.idx, 3 * sizeof(const uint8_t*));
+            GeomKey key;
+            std::memset(&key, 0, sizeof(key));
+            key.ptr[0] = d.idx; key.ptr[1] = d.pos; key.ptr[2] = d.attr;
             const uint32_t fields[] = { d.idx_stride, d.pos_stride, d.attr_stride, d.ntris, d.pos_num, d.attr_num,
                                         d.nslots };
-            key.append((const char*)fields, sizeof(fields));
-            key.append((const char*)d.slot_attr, sizeof(d.slot_attr));
+            std::memcpy(key.f, fields, sizeof(fields));
+            std::memcpy(key.slot_attr, d.slot_attr, sizeof(key.slot_attr));
             auto it = open_group.find(key);
             if (it == open_group.end() || lists[it->second].size() >= kMaxGroupMembers) {
                 open_group[key] = (uint32_t)lists.size();
