@@ -957,9 +957,10 @@ __global__ __launch_bounds__(256) void k_scatter(FrameDev F, uint32_t chunk)
                                  ? region * F.pair_region_cap
                                  : F.pair_regions * F.pair_region_cap + (region - F.pair_regions) * F.clip_pair_cap;
     const uint32_t np = F.pair_total;
-    for (uint32_t c0 = blockIdx.y * chunk; c0 < n; c0 += gridDim.y * chunk) {
+    const uint32_t nchunks = n / chunk + (n % chunk != 0u);
+    for (uint32_t c = blockIdx.y; c < nchunks; c += gridDim.y) {
         for (uint32_t k = threadIdx.x; k < chunk; k += 256) {
-            const uint32_t i = c0 + k;
+            const uint32_t i = c * chunk + k;  // < n + chunk: no wrap for n < 2^32 - chunk
             if (i >= n) break;
             const uint32_t p = first_p + i;
             const uint32_t rec = F.pairs[p];
