@@ -119,6 +119,16 @@ __device__ __forceinline__ uint32_t spread5(uint32_t v)
     return v;
 }
 __device__ __forceinline__ uint32_t morton5(uint32_t x, uint32_t y) { return spread5(x) | (spread5(y) << 1); }
+// spread5 of the low 5 bits of two coordinates at once, one in each 16-bit
+// half (the spread values stay below bit 9 of their half)
+__device__ __forceinline__ uint32_t spread5x2(uint32_t a, uint32_t b)
+{
+    uint32_t v = (a & 31u) | ((b & 31u) << 16);
+    v = (v | (v << 4)) & 0x010F010Fu;
+    v = (v | (v << 2)) & 0x01330133u;
+    v = (v | (v << 1)) & 0x01550155u;
+    return v;
+}
 
 // Device-memory view of a pointer read from a table: accesses through it
 // compile to global memory instructions. Through a generic pointer they are
@@ -180,8 +190,8 @@ __device__ __forceinline__ void sample_wrap(const TexDev& T, float u, float v, f
     const uint32_t tw = (uint32_t)(mw > 32 ? mw : 32) >> 5;
     const auto base = global_ptr((const uint32_t*)(T.texels + T.mip_offsets[mip]));
     const uint32_t ty0 = (y0 >> 5) * tw, ty1 = (y1 >> 5) * tw;
-    const uint32_t my0 = spread5(y0) << 1, my1 = spread5(y1) << 1;
-    const uint32_t mx0 = spread5(x0), mx1 = spread5(x1);
+    const uint32_t mx = spread5x2(x0, x1), my = spread5x2(y0, y1) << 1;
+    const uint32_t mx0 = mx & 0xFFFFu, mx1 = mx >> 16, my0 = my & 0xFFFFu, my1 = my >> 16;
     const uint32_t t00 = base[1024u * (ty0 + (x0 >> 5)) + (mx0 | my0)];
     const uint32_t t01 = base[1024u * (ty1 + (x0 >> 5)) + (mx0 | my1)];
     const uint32_t t10 = base[1024u * (ty0 + (x1 >> 5)) + (mx1 | my0)];
