@@ -205,6 +205,10 @@ struct srg_ctx {
     srg_config cfg{};
     int device = 0;
     int num_cus = 0;
+    // k_clip workgroups resident on the whole device at once, for two, four
+    // and eight kept planes: its grid, which strides over the clip queue, is
+    // no larger, so that every workgroup runs in the first round
+    uint32_t clip_resident[3] = { 2048, 2048, 2048 };
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // asynchronous blits
     std::unordered_map<uint64_t, Buffer> buffers;
@@ -980,7 +984,8 @@ int run_frame(srg_ctx* ctx, bool noalloc)
             256u * std::max<uint32_t>(1, std::min<uint32_t>(8, (F.pair_region_cap + 256u * 512u - 1) / (256u * 512u)));
         const uint32_t emit_blocks =
             std::max<uint32_t>(1, std::min<uint32_t>((F.pair_region_cap + emit_chunk - 1) / emit_chunk, 4096));
-        const uint32_t clip_blocks = std::max<uint32_t>(1, std::min<uint32_t>((NT + kClipThreads - 1) / kClipThreads, 2048));
+        const uint32_t clip_blocks = std::max<uint32_t>(
+            1, std::min<uint32_t>((NT + kClipThreads - 1) / kClipThreads, ctx->clip_resident[wide ? 2 : (narrow ? 0 : 1)]));
         const uint32_t init_blocks = std::max<uint32_t>(1, std::min<uint32_t>((kBinParts * TB + 255) / 256, 1024));
         const uint32_t scan_blocks = (TB + kScanThreads - 1) / kScanThreads;
         const bool ordered = ctx->cfg.tile_order != SRG_ORDER_NATURAL;
@@ -1256,6 +1261,16 @@ int srg_create(const srg_config* cfg, srg_ctx** out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) { delete ctx; return SRG_ERR_HIP; }
     ctx->num_cus = prop.multiProcessorCount;
+    {
+        const void* kc[3] = { reinterpret_cast<const void*>(&k_clip<2>), reinterpret_cast<const void*>(&k_clip<kMaxSlots>),
+                              reinterpret_cast<const void*>(&k_clip<kWideSlots>) };
+        for (int i = 0; i < 3; ++i) {
+            int per_cu = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kc[i], (int)kClipThreads, 0) == hipSuccess &&
+                per_cu > 0)
+                ctx->clip_resident[i] = (uint32_t)per_cu * (uint32_t)ctx->num_cus;
+        }
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) { delete ctx; return SRG_ERR_HIP; }
     if (hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking) != hipSuccess) {
         srg_destroy(ctx);
