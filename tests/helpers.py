@@ -60,6 +60,53 @@ def random_scene(seed, ntris, width, height, clip_heavy=False, index_stride=4, s
     return scenes.Scene(width, height, mvp, draws, texs, {"config": "random"})
 
 
+def many_fan_scene(seed, width, height, candidates=60000, min_verts=6):
+    """Triangles whose clipped polygons have min_verts to 9 vertices (4 to 7
+    fan triangles), chosen with the oracle's clipper from candidates built in
+    clip space -- an edge through a line where two frustum planes meet, the
+    third vertex anywhere -- and mapped back through the inverse camera
+    matrix. Returns (scene, histogram of polygon sizes kept)."""
+    from oracle import oracle as O
+    lib = O.load()
+    rng = np.random.default_rng(seed)
+    mvp = np.asarray(scenes.camera_mvp((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 60.0, width / height), np.float32)
+    m = mvp.reshape(4, 4)  # clip[r] = ((m[0,r] x + m[1,r] y) + m[2,r] z) + m[3,r], in float32
+    inv = np.linalg.inv(m.astype(np.float64).T)
+    kept, hist = [], {}
+    for _ in range(candidates):
+        w0 = rng.uniform(0.2, 5.0)
+        corner = np.array([rng.choice([-1, 1]) * w0, rng.choice([-1, 1]) * w0, rng.uniform(-1, 1) * w0, w0])
+        d = rng.normal(size=4)
+        third = rng.uniform(-3, 3, 4)
+        third[3] = abs(third[3]) + 0.1
+        clip64 = [corner + rng.uniform(0.2, 3) * d, corner - rng.uniform(0.2, 3) * d, third]
+        if min(c[3] for c in clip64) <= 0.01:
+            continue
+        obj = np.array([(inv @ c)[:3] / (inv @ c)[3] for c in clip64], np.float32)
+        cl = np.array([[((m[0, r] * x + m[1, r] * y) + m[2, r] * z) + m[3, r] for r in range(4)] for x, y, z in obj],
+                      np.float32)
+        codes = [lib.sro_clip_codes(O.fptr(np.ascontiguousarray(c))) for c in cl]
+        mor = codes[0] | codes[1] | codes[2]
+        if mor == 0 or (codes[0] & codes[1] & codes[2]):
+            continue
+        ov, oa, at = np.zeros(36, np.float32), np.zeros(72, np.float32), np.zeros(24, np.float32)
+        n = lib.sro_clip_triangle(O.fptr(np.ascontiguousarray(cl.reshape(-1))), O.fptr(at), mor, O.fptr(ov), O.fptr(oa))
+        if n >= min_verts:
+            kept.append(obj)
+            hist[int(n)] = hist.get(int(n), 0) + 1
+    pos = np.concatenate(kept)
+    v = np.zeros((len(pos), 8), np.float32)
+    v[:, 0:3] = pos
+    v[:, 3:6] = (0.0, 0.0, 1.0)
+    v[:, 6:8] = rng.uniform(-3, 3, (len(pos), 2))
+    idx = np.arange(len(pos), dtype=np.uint32).reshape(-1, 3)
+    flip = rng.random(len(idx)) < 0.5
+    idx[flip] = idx[flip][:, ::-1]
+    draw = scenes.Draw(v.view(scenes.VERTEX_DTYPE).reshape(-1), idx.reshape(-1).copy(), 0, 0, 6)
+    tex = [rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)]
+    return scenes.Scene(width, height, mvp, [draw], tex, {"config": "many_fan"}), dict(sorted(hist.items()))
+
+
 def build_textures(scene):
     from softrast_amd import TextureData
     out = []

@@ -49,6 +49,16 @@ def test_random_clip_heavy(seed):
     _check(sc, f"clip_heavy_{seed}")
 
 
+def test_many_fan_clips():
+    """Clipped polygons of 6 to 8 vertices (4 to 6 fan triangles per
+    submitted triangle; random clip-heavy scenes stop at 5 vertices): the
+    fan numbering, the fan records and their order keys match the oracle."""
+    sc, hist = helpers.many_fan_scene(5, 333, 211)
+    _say(f"[parity] case=many_fan polygon sizes {hist}")
+    assert max(hist) >= 8
+    _check(sc, "many_fan")
+
+
 @pytest.mark.parametrize("stride", [1, 2, 4])
 def test_index_strides(stride):
     n = 84 if stride == 1 else 1500
