@@ -60,7 +60,7 @@ def random_scene(seed, ntris, width, height, clip_heavy=False, index_stride=4, s
     return scenes.Scene(width, height, mvp, draws, texs, {"config": "random"})
 
 
-def many_fan_scene(seed, width, height, candidates=60000, min_verts=6):
+def many_fan_scene(seed, width, height, candidates=60000, min_verts=6, shader=0):
     """Triangles whose clipped polygons have min_verts to 9 vertices (4 to 7
     fan triangles), chosen with the oracle's clipper from candidates built in
     clip space -- an edge through a line where two frustum planes meet, the
@@ -102,7 +102,7 @@ def many_fan_scene(seed, width, height, candidates=60000, min_verts=6):
     idx = np.arange(len(pos), dtype=np.uint32).reshape(-1, 3)
     flip = rng.random(len(idx)) < 0.5
     idx[flip] = idx[flip][:, ::-1]
-    draw = scenes.Draw(v.view(scenes.VERTEX_DTYPE).reshape(-1), idx.reshape(-1).copy(), 0, 0, 6)
+    draw = scenes.Draw(v.view(scenes.VERTEX_DTYPE).reshape(-1), idx.reshape(-1).copy(), 0, shader, 6)
     tex = [rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)]
     return scenes.Scene(width, height, mvp, [draw], tex, {"config": "many_fan"}), dict(sorted(hist.items()))
 

@@ -204,6 +204,23 @@ def test_gpu_sponza_matches_oracle(case):
 
 
 @pytest.mark.gpu
+def test_gpu_sponza_many_fan_clips():
+    """The eight-plane clipper (k_clip<8>) on clipped polygons of 6 to 8
+    vertices, every attribute lerped through each plane, bit-exact."""
+    sc, hist = helpers.many_fan_scene(7, 320, 240, shader=3)
+    _say(f"[sponza] case=many_fan polygon sizes {hist}")
+    assert max(hist) >= 8
+    k = scenes.sponza_constants(7, near=True)
+    tex = helpers.build_textures(sc)
+    ref, rstats = helpers.oracle_render(sc, tex, nthreads=8, sponza=k)
+    col, dep, pid, st = helpers.gpu_render(sc, tex, sponza=k)
+    d = helpers.compare(ref, col, dep, pid)
+    _say(f"[sponza] case=many_fan diffs={d}")
+    assert not any(d.values()), d
+    assert st["frags_raster"] == rstats["frags_raster"]
+
+
+@pytest.mark.gpu
 def test_gpu_sponza_then_narrow_frame():
     """A context that renders a SponzaShader frame (eight planes per record)
     and then an UnlitDiffuse frame (four) gets both right: the record
