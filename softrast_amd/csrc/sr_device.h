@@ -109,24 +109,16 @@ __device__ __forceinline__ uint32_t pack_rgba(float r, float g, float b, float a
     return pack_channel(r) | (pack_channel(g) << 8) | (pack_channel(b) << 16) | (pack_channel(a) << 24);
 }
 
-// 5-bit Morton interleave: x bits even, y bits odd (Texture.cpp:36-67).
-__device__ __forceinline__ uint32_t spread5(uint32_t v)
-{
-    v &= 31u;
-    v = (v | (v << 4)) & 0x10Fu;  // bit 4 -> bit 8
-    v = (v | (v << 2)) & 0x133u;  // bits 2,3 -> 4,5
-    v = (v | (v << 1)) & 0x155u;  // odd bits -> even positions
-    return v;
-}
-__device__ __forceinline__ uint32_t morton5(uint32_t x, uint32_t y) { return spread5(x) | (spread5(y) << 1); }
-// spread5 of the low 5 bits of two coordinates at once, one in each 16-bit
-// half (the spread values stay below bit 9 of their half)
+// 5-bit Morton interleave (Texture.cpp:36-67): the low 5 bits of a
+// coordinate spread to the even bit positions (x; y takes the odd ones, one
+// shift up), for two coordinates at once, one in each 16-bit half (a spread
+// value stays below bit 9 of its half)
 __device__ __forceinline__ uint32_t spread5x2(uint32_t a, uint32_t b)
 {
     uint32_t v = (a & 31u) | ((b & 31u) << 16);
-    v = (v | (v << 4)) & 0x010F010Fu;
-    v = (v | (v << 2)) & 0x01330133u;
-    v = (v | (v << 1)) & 0x01550155u;
+    v = (v | (v << 4)) & 0x010F010Fu;  // bit 4 -> bit 8
+    v = (v | (v << 2)) & 0x01330133u;  // bits 2, 3 -> 4, 5
+    v = (v | (v << 1)) & 0x01550155u;  // odd bits -> even positions
     return v;
 }
 

@@ -238,20 +238,6 @@ __device__ __forceinline__ uint32_t wave_add_slot_lds(uint32_t* t, uint32_t* cou
     return slot;
 }
 
-__device__ __forceinline__ void wave_count_lds(uint32_t* t, uint32_t* counters, uint32_t key, bool active)
-{
-    uint32_t h = kHashSlots, rank = 0;
-    if (active) h = wave_hash_insert(t, key, rank);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (active && h == kHashSlots) atomicAdd(&counters[key], 1u);
-    if (h < kHashSlots && rank == 0) {
-        atomicAdd(&counters[key], t[kHashSlots + h]);
-        t[h] = kHashEmpty;
-        t[kHashSlots + h] = 0;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
-
 // Inclusive max over the wave's lanes (DPP: row shifts, then row broadcasts).
 __device__ __forceinline__ uint32_t wave_incl_max(uint32_t v)
 {
